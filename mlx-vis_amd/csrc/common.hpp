@@ -1,0 +1,143 @@
+// Shared helpers for the libmivis HIP sources: status/error plumbing, launch
+// checks, fault injection, and wavefront primitives for gfx950 (64 lanes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/mivis.h"
+
+namespace mv {
+
+// Records a printf-style message as the calling thread's last error.
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+// Returns nonzero when MIVIS_FAULT names `site` ("launch", "alloc", "all").
+int fault_injected(const char* site);
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Bump allocator over a caller-provided workspace.
+struct Carve {
+  char* base;
+  size_t off = 0;
+  explicit Carve(void* p) : base(static_cast<char*>(p)) {}
+  template <class T>
+  T* take(size_t count) {
+    off = align_up(off, 256);
+    T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+}  // namespace mv
+
+#define MV_TRY_HIP(expr)                                                              \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      mv::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
+                    __LINE__);                                                        \
+      return MV_EHIP;                                                                 \
+    }                                                                                 \
+  } while (0)
+
+#define MV_LAUNCHED(name)                                                               \
+  do {                                                                                  \
+    hipError_t _e = hipGetLastError();                                                  \
+    if (_e != hipSuccess) {                                                             \
+      mv::set_error("launch of %s failed: %s", name, hipGetErrorString(_e));           \
+      return MV_EHIP;                                                                   \
+    }                                                                                   \
+  } while (0)
+
+#define MV_REQUIRE(cond, ...)   \
+  do {                          \
+    if (!(cond)) {              \
+      mv::set_error(__VA_ARGS__); \
+      return MV_EINVAL;         \
+    }                           \
+  } while (0)
+
+#define MV_FAULT_POINT(site)                                              \
+  do {                                                                    \
+    if (mv::fault_injected(site)) {                                       \
+      mv::set_error("fault injected at '%s' (MIVIS_FAULT)", site);        \
+      return MV_EINJECTED;                                                \
+    }                                                                     \
+  } while (0)
+
+// ----------------------------------------------------------------------------
+// Device-side wavefront helpers (wave64).
+namespace mv {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  int lo = __shfl(static_cast<int>(static_cast<uint32_t>(v)), src);
+  int hi = __shfl(static_cast<int>(static_cast<uint32_t>(v >> 32)), src);
+  return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  int lo = __shfl_xor(static_cast<int>(static_cast<uint32_t>(v)), m);
+  int hi = __shfl_xor(static_cast<int>(static_cast<uint32_t>(v >> 32)), m);
+  return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int delta) {
+  int lo = __shfl_up(static_cast<int>(static_cast<uint32_t>(v)), delta);
+  int hi = __shfl_up(static_cast<int>(static_cast<uint32_t>(v >> 32)), delta);
+  return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
+  uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), src);
+  uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), src);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Ascending bitonic sort of one 64-bit key per lane across the wave.
+__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t key) {
+  const int l = lane_id();
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint64_t o = shfl_xor_u64(key, j);
+      bool up = (l & k) == 0;
+      bool lower = (l & j) == 0;
+      uint64_t mn = o < key ? o : key;
+      uint64_t mx = o < key ? key : o;
+      key = (lower == up) ? mn : mx;
+    }
+  }
+  return key;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    uint64_t o = shfl_xor_u64(v, m);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// Key of a (non-negative distance, index) pair: orders by distance, then index.
+__device__ __forceinline__ uint64_t dkey(float dist, uint32_t idx) {
+  return (static_cast<uint64_t>(__float_as_uint(dist)) << 32) | idx;
+}
+__device__ __forceinline__ float key_dist(uint64_t k) {
+  return __uint_as_float(static_cast<uint32_t>(k >> 32));
+}
+__device__ __forceinline__ uint32_t key_idx(uint64_t k) { return static_cast<uint32_t>(k); }
+
+}  // namespace mv

@@ -1,0 +1,13 @@
+"""mlx_vis_amd: the mlx-vis hot path (exact kNN, NNDescent, UMAP layout) on
+MI355X (gfx950) through hand-written HIP kernels in libmivis.so.
+
+Drop-in names follow mlx_vis/__init__.py:5-17 for this path:
+``UMAP``, ``NNDescent`` and the shared ``compute_knn`` operator.
+"""
+
+__version__ = "0.1.0"
+
+from ._knn import compute_knn, knn_brute_device
+from ._normalize import normalize_input
+
+__all__ = ["compute_knn", "knn_brute_device", "normalize_input"]

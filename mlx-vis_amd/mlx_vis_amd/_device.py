@@ -1,0 +1,36 @@
+"""Device plumbing: torch tensors carry device memory and the stream; every
+computation is a libmivis kernel."""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _capi
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("mlx_vis_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def as_device_f32(X) -> torch.Tensor:
+    """(n, d) float32 contiguous device tensor from numpy / torch input."""
+    if isinstance(X, torch.Tensor):
+        t = X.detach()
+        if t.dtype != torch.float32:
+            t = t.float()
+        if t.device.type != "cuda":
+            t = t.to(device(), non_blocking=False)
+        return t.contiguous()
+    arr = np.ascontiguousarray(np.asarray(X, dtype=np.float32))
+    return torch.from_numpy(arr).to(device())
+
+
+def workspace(nbytes: int) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
+
+
+def stream() -> int | None:
+    return _capi.stream_handle()

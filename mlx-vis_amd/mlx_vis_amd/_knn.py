@@ -1,0 +1,117 @@
+"""Shared kNN operator: brute force (libmivis MFMA screen + exact re-rank) or
+NNDescent.
+
+Mirrors ``compute_knn`` of mlx_vis/_knn.py:7-88: same arguments, same
+``auto`` rule (brute for n <= 20,000, else NNDescent, _knn.py:32-33), same
+NNDescent seed default (42, _knn.py:40), same return types (int32 indices,
+float32 distances, Euclidean unless ``return_euclidean=False``).
+
+The brute path returns the canonical neighbours: ascending exact distance,
+ties broken by the lower index (the reference's argsort order,
+_knn.py:71-72); the screen ``dtype`` ("fp32", "bf16", "fp8") changes speed,
+never the result.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _capi, _device
+
+
+def knn_brute_device(
+    X: torch.Tensor,
+    k: int,
+    Q: torch.Tensor | None = None,
+    *,
+    dtype: str = "fp32",
+    rerank: int = 2,
+    q_self: torch.Tensor | None = None,
+    exclude_self: bool | None = None,
+    euclid: bool = True,
+    stats: torch.Tensor | None = None,
+):
+    """Device-level brute kNN.
+
+    X: (n, d) float32 CUDA tensor. Q: (nq, d) queries (default X itself).
+    Self exclusion: with ``Q is None`` the query row i never returns i (the
+    +1e30 diagonal of _knn.py:64-69); ``q_self`` (nq,) int32 names the row of X
+    each query must not return (-1 for none).
+    Returns (idx int32 (nq, k), dist float32 (nq, k)) CUDA tensors.
+    """
+    if dtype not in _capi.DTYPES:
+        raise ValueError(f"unknown kNN dtype {dtype!r}; expected one of {sorted(_capi.DTYPES)}")
+    same = Q is None
+    if same:
+        Q = X
+    if X.dim() != 2 or Q.dim() != 2 or X.shape[1] != Q.shape[1]:
+        raise ValueError(f"bad shapes X {tuple(X.shape)} Q {tuple(Q.shape)}")
+    n, d = X.shape
+    nq = Q.shape[0]
+    if exclude_self is None:
+        exclude_self = same and q_self is None
+    self_base = 0 if (exclude_self and q_self is None) else -1
+    if q_self is not None:
+        q_self = q_self.to(device=X.device, dtype=torch.int32).contiguous()
+        if q_self.numel() != nq:
+            raise ValueError("q_self must have one entry per query row")
+    out_idx = torch.empty((nq, k), dtype=torch.int32, device=X.device)
+    out_dist = torch.empty((nq, k), dtype=torch.float32, device=X.device)
+    lib = _capi.lib()
+    import ctypes
+
+    nbytes = ctypes.c_size_t(0)
+    _capi.check(
+        lib.mv_knn_ws_bytes(nq, n, d, int(k), _capi.DTYPES[dtype], int(rerank), int(same), ctypes.byref(nbytes)),
+        "mv_knn_ws_bytes",
+    )
+    ws = _device.workspace(nbytes.value)
+    _capi.check(
+        lib.mv_knn_brute(
+            Q.data_ptr(), nq, X.data_ptr(), n, d, int(k), _capi.DTYPES[dtype], int(rerank),
+            _capi.ptr(q_self), self_base, int(bool(euclid)),
+            out_idx.data_ptr(), out_dist.data_ptr(), _capi.ptr(stats),
+            ws.data_ptr(), ws.numel(), _device.stream(),
+        ),
+        "mv_knn_brute",
+    )
+    return out_idx, out_dist
+
+
+def compute_knn(X, k, method="auto", verbose=False, random_state=None, return_euclidean=True,
+                dtype="fp32"):
+    """Compute k-nearest neighbours (drop-in for mlx_vis._knn.compute_knn).
+
+    Args:
+        X: (n, d) array (numpy, torch, or anything ``np.asarray`` accepts).
+        k: number of neighbours.
+        method: "auto" (brute for n <= 20000, else NNDescent), "brute", "nndescent".
+        random_state: NNDescent seed (default 42).
+        return_euclidean: Euclidean distances if True, squared if False.
+        dtype: brute-force screen precision ("fp32", "bf16", "fp8").
+
+    Returns:
+        (indices int32 (n, k), distances float32 (n, k)) numpy arrays.
+    """
+    if isinstance(X, torch.Tensor):
+        n = X.shape[0]
+    else:
+        X = np.asarray(X)
+        n = X.shape[0]
+    if method == "auto":
+        method = "brute" if n <= 20000 else "nndescent"
+    if method == "nndescent":
+        from ._nndescent import NNDescent
+
+        nn = NNDescent(k=k, verbose=verbose,
+                       random_state=random_state if random_state is not None else 42)
+        indices, dists = nn.build(X)
+        if not return_euclidean:
+            dists = dists ** 2
+        return indices, dists
+    if method != "brute":
+        raise ValueError(f"Unknown knn method: {method}")
+    Xd = _device.as_device_f32(X)
+    idx, dist = knn_brute_device(Xd, k, dtype=dtype, euclid=bool(return_euclidean))
+    return idx.cpu().numpy(), dist.cpu().numpy()

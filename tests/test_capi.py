@@ -1,0 +1,55 @@
+"""The C-ABI library loads, exports every declared symbol, and maps errors
+(CPU only: no call here reaches the GPU)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "mivis.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mv_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from mlx_vis_amd import _capi
+
+    lib = _capi.lib()
+    declared = _declared_symbols()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), f"{name} declared in mivis.h but not exported"
+    assert set(declared) == set(_capi.exported_symbols())
+    assert lib.mv_abi_version() == _capi.ABI_VERSION
+
+
+def test_ws_query_and_invalid_arguments_map_to_valueerror():
+    from mlx_vis_amd import _capi
+
+    lib = _capi.lib()
+    nbytes = ctypes.c_size_t(0)
+    _capi.check(lib.mv_knn_ws_bytes(1000, 1000, 128, 15, 0, 2, 1, ctypes.byref(nbytes)))
+    assert nbytes.value > 1000 * 128 * 4
+    with pytest.raises(ValueError, match="k=0"):
+        _capi.check(lib.mv_knn_ws_bytes(10, 10, 4, 0, 0, 2, 1, ctypes.byref(nbytes)))
+    # Invalid arguments are rejected before any device work.
+    with pytest.raises(ValueError, match="null pointer"):
+        _capi.check(lib.mv_knn_brute(None, 10, None, 10, 4, 3, 0, 2, None, 0, 1,
+                                     None, None, None, None, 0, None))
+    with pytest.raises(_capi.MivisError) as ei:
+        _capi.check(lib.mv_knn_ws_bytes(10, 10, 4, 3, 7, 2, 1, ctypes.byref(nbytes)))
+    assert ei.value.status == _capi.MV_EUNSUPPORTED
+
+
+def test_fault_injection_maps_to_mivis_error(monkeypatch):
+    from mlx_vis_amd import _capi
+
+    monkeypatch.setenv("MIVIS_FAULT", "launch")
+    with pytest.raises(_capi.MivisError) as ei:
+        _capi.check(_capi.lib().mv_row_sqnorms(ctypes.c_void_p(16), 4, 4, ctypes.c_void_p(16), None))
+    assert ei.value.status == _capi.MV_EINJECTED
+    assert "MIVIS_FAULT" in str(ei.value)

@@ -1,0 +1,89 @@
+"""Parity of the HIP brute-force kNN (through the C ABI) with the oracle.
+
+Bar: indices bit-exact against the canonical order (exact distance, lower
+index first); distances within 1e-5 relative of the FP64 oracle.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import knn as oknn
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _run(X, k, Q=None, **kw):
+    from mlx_vis_amd import knn_brute_device
+
+    Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
+    Qd = None if Q is None else torch.from_numpy(np.ascontiguousarray(Q, dtype=np.float32)).cuda()
+    stats = torch.zeros(4, dtype=torch.int32, device="cuda")
+    idx, dist = knn_brute_device(Xd, k, Qd, stats=stats, **kw)
+    torch.cuda.synchronize()
+    return idx.cpu().numpy(), dist.cpu().numpy(), stats.cpu().numpy()
+
+
+def _check(idx, dist, oi, od, euclid=True):
+    want_d = np.sqrt(od) if euclid else od
+    mism = np.argwhere(idx != oi)
+    assert mism.size == 0, f"{len(mism)} index mismatches, first at {mism[:5].tolist()}"
+    np.testing.assert_allclose(dist, want_d, rtol=RTOL, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,d,k", [(1000, 128, 15), (777, 3, 5), (513, 1, 7), (300, 784, 15),
+                                   (2000, 129, 32), (130, 64, 1), (129, 200, 16)])
+def test_self_knn_matches_oracle(n, d, k):
+    print(f"case n={n} d={d} k={k}", flush=True)
+    rng = np.random.default_rng(n + d + k)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    idx, dist, st = _run(X, k)
+    oi, od = oknn.exact_knn(X, X, k, self_base=0)
+    _check(idx, dist, oi, od)
+
+
+def test_squared_output_and_queries():
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((900, 40)).astype(np.float32) * 3 + 7
+    Q = rng.standard_normal((257, 40)).astype(np.float32) * 3 + 7
+    idx, dist, _ = _run(X, 12, Q, euclid=False)
+    oi, od = oknn.exact_knn(Q, X, 12)
+    _check(idx, dist, oi, od, euclid=False)
+
+
+def test_q_self_exclusion():
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((640, 16)).astype(np.float32)
+    sel = rng.choice(640, 100, replace=False).astype(np.int32)
+    qs = torch.from_numpy(sel)
+    idx, dist, _ = _run(X, 9, X[sel], q_self=qs)
+    oi, od = oknn.exact_knn(X[sel], X, 9, q_self=sel)
+    _check(idx, dist, oi, od)
+
+
+def test_duplicates_and_ties_resolved_by_index():
+    rng = np.random.default_rng(7)
+    base = rng.integers(0, 3, size=(150, 6)).astype(np.float32)
+    X = np.concatenate([base, base, base[:40]], axis=0)  # exact duplicates and equal distances
+    idx, dist, st = _run(X, 15)
+    oi, od = oknn.exact_knn(X, X, 15, self_base=0)
+    _check(idx, dist, oi, od)
+    print("stats", st)
+
+
+def test_constant_rows():
+    X = np.ones((300, 10), dtype=np.float32)
+    idx, dist, st = _run(X, 5)
+    oi, od = oknn.exact_knn(X, X, 5, self_base=0)
+    _check(idx, dist, oi, od)
+
+
+def test_c0_shape_bit_exact():
+    """10,000 x 128 N(0,1), seed 0, k=15 (the small plain-method workload)."""
+    X = np.random.default_rng(0).standard_normal((10000, 128), dtype=np.float32)
+    idx, dist, st = _run(X, 15)
+    oi, od = oknn.exact_knn(X, X, 15, self_base=0)
+    _check(idx, dist, oi, od)
+    print("stats", st)
