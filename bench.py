@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""UMAP fit_transform benchmark on one MI355X.
+
+Workload: X = 70,000 x 784 float32, a mixture of 10 isotropic Gaussians
+(centres N(0, 4^2) per dimension, sigma 1, equal weights, seed 1), min-max
+scaled to [0, 1], resident in HBM. One step = UMAP(n_neighbors=15,
+min_dist=0.1, n_epochs=500, negative_sample_rate=5, knn_method="brute")
+.fit_transform(X): exact brute-force kNN (k=15, FP32 MFMA screen + exact
+re-rank), fuzzy graph, spectral init, 500 epochs of layout SGD, and the copy of
+the embedding to the host.
+
+Prints one JSON line. `value` is seconds per fit_transform (lower is better).
+`roofline` covers the dominant kernel (the MFMA distance screen of the kNN),
+timed with HIP events on its launch stream; `cpu_baseline` is the NumPy
+restatement of the reference (the oracle) on a bounded sample of the same
+workload, scaled to the whole job.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "mlx-vis_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+MFMA_FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=70_000, help="rows of the workload (default: the full 70,000)")
+    ap.add_argument("--epochs", type=int, default=500)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--knn-reps", type=int, default=3)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def knn_phase_timing(Xd, reps):
+    """Times mv_knn_brute's phases with HIP events on its stream."""
+    import ctypes
+
+    import torch
+
+    from mlx_vis_amd import _capi, knn_brute_device
+
+    lib = _capi.lib()
+    lib.mv_knn_set_timing(1)
+    phases = []
+    stats = torch.zeros(4, dtype=torch.int32, device=Xd.device)
+    try:
+        for _ in range(reps):
+            knn_brute_device(Xd, 15, stats=stats)
+            out = (ctypes.c_float * 4)()
+            _capi.check(lib.mv_knn_last_phase_ms(out), "mv_knn_last_phase_ms")
+            phases.append(list(out))
+    finally:
+        lib.mv_knn_set_timing(0)
+    n = len(phases)
+    avg = [sum(p[i] for p in phases) / n for i in range(4)]
+    return avg, stats.cpu().tolist()
+
+
+def knn_preservation_gpu(knn_hi, Y):
+    """Mean overlap of the 15-NN sets in the input and the embedding."""
+    import torch
+
+    from mlx_vis_amd import knn_brute_device
+
+    Yd = torch.as_tensor(Y).cuda().contiguous()
+    lo, _ = knn_brute_device(Yd, knn_hi.shape[1])
+    hi = knn_hi.long()
+    lo = lo.long()
+    inter = (hi[:, :, None] == lo[:, None, :]).any(dim=2).sum(dim=1).float()
+    return float((inter / hi.shape[1]).mean().item())
+
+
+def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs=8, spec_iters=5):
+    """The oracle restatement of fit_transform on a bounded sample, scaled."""
+    import numpy as np
+
+    from oracle import knn as oknn
+    from oracle import umap as ou
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:  # noqa: BLE001
+        cores = 1
+    n = X.shape[0]
+    rng = np.random.default_rng(0)
+    rows = np.sort(rng.choice(n, budget_rows, replace=False))
+    t0 = time.perf_counter()
+    oknn.reference_knn_rows(X, rows, 15)
+    t_knn = (time.perf_counter() - t0) * n / budget_rows
+
+    t0 = time.perf_counter()
+    r, c, v = ou.fuzzy_simplicial_set(knn_idx, knn_dist, n_epochs)
+    ou.find_ab_params(1.0, 0.1)
+    t_graph = time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    ou.spectral_power(r, c, v.astype(np.float64), n, rng.standard_normal((n, 3)), iters=spec_iters)
+    t_spec = (time.perf_counter() - t0) * 100 / spec_iters
+
+    sets = ou.schedule(v, n_epochs)
+    total = sum(s.size for s in sets)
+    ep = [e for e in range(n_epochs) if sets[e].size][:budget_epochs]
+    Y = rng.uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    done = 0
+    t0 = time.perf_counter()
+    for e in ep:
+        act = sets[e]
+        A = act.size
+        ef, et = r[act], c[act]
+        nf = ef[np.arange(5 * A) % A]
+        nt = rng.integers(0, n, size=5 * A)
+        Y = ou.sgd_step(Y, ef, et, nf, nt, np.float32(1.0 - e / n_epochs), np.float32(1.577),
+                        np.float32(0.895), dtype=np.float32)
+        done += A
+    t_layout = (time.perf_counter() - t0) * total / max(done, 1)
+    total_s = t_knn + t_graph + t_spec + t_layout
+    return {
+        "value": round(total_s, 2),
+        "unit": "s",
+        "cores": int(cores),
+        "kind": "port",
+        "sample": (f"NumPy restatement of the reference (oracle), not MLX: kNN on {budget_rows} of {n} query rows "
+                   f"(x{n / budget_rows:.0f}); fuzzy graph + a/b in full; spectral init {spec_iters} of 100 "
+                   f"iterations (x{100 / spec_iters:.0f}); layout epochs {ep[0]}-{ep[-1]} ({len(ep)} epochs, "
+                   f"scaled by edge samples {total}/{done})"),
+        "parts_s": {"knn": round(t_knn, 2), "graph": round(t_graph, 2), "spectral": round(t_spec, 2),
+                    "layout": round(t_layout, 2)},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not built",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from mlx_vis_amd import UMAP
+    from mlx_vis_amd.datasets import gaussian_mixture_784
+
+    log("generating the 10-Gaussian 784-D mixture")
+    X = gaussian_mixture_784(args.n, seed=1)
+    Xd = torch.from_numpy(X).cuda()
+    n, d = X.shape
+    torch.cuda.synchronize()
+
+    def step():
+        um = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=args.epochs, negative_sample_rate=5,
+                  knn_method="brute", random_state=42)
+        Y = um.fit_transform(Xd)
+        return um, Y
+
+    for i in range(args.warmup):
+        log(f"warmup {i + 1}/{args.warmup}")
+        step()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        um, Y = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    log(f"fit_transform {dt * 1000:.1f} ms")
+
+    # phase breakdown of one more run (synchronised between phases; not timed above)
+    from mlx_vis_amd import knn_brute_device
+    from mlx_vis_amd._umap import find_ab_params, fuzzy_graph_device, run_epochs
+    from mlx_vis_amd._spectral import spectral_init
+
+    ph = {}
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ki, kd = knn_brute_device(Xd, 15)
+    torch.cuda.synchronize()
+    ph["knn"] = time.perf_counter() - t
+    t = time.perf_counter()
+    g = fuzzy_graph_device(ki, kd, args.epochs)
+    torch.cuda.synchronize()
+    ph["graph"] = time.perf_counter() - t
+    t = time.perf_counter()
+    Y0 = spectral_init(g, n, 2, 42)
+    torch.cuda.synchronize()
+    ph["init"] = time.perf_counter() - t
+    a, b = find_ab_params(1.0, 0.1)
+    t = time.perf_counter()
+    run_epochs(Y0, torch.zeros_like(Y0), g, 1, args.epochs, args.epochs, 5, 1.0, a, b, 1)
+    torch.cuda.synchronize()
+    ph["epochs"] = time.perf_counter() - t
+
+    log("kNN phase timing")
+    phases, stats = knn_phase_timing(Xd, args.knn_reps)
+    screen_ms = phases[1]
+    flops = 2.0 * n * n * d
+    screen_tflops = flops / (screen_ms * 1e-3) / 1e12
+    knn_total_ms = sum(phases)
+    quality = knn_preservation_gpu(um.knn_[0], Y)
+
+    line = {
+        "metric": "fit_transform s, 70K x 784 kNN+UMAP; kNN TFLOP/s; bf16/fp8 vs fp32 at exact kNN",
+        "value": round(dt, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1000, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 10-Gaussian 784-D mixture, min-max scaled (seed 1), resident in HBM",
+        "config": {"workload": f"UMAP fit_transform {n}x{d}, exact brute kNN k=15 (FP32), spectral init, "
+                               f"{args.epochs} epochs, 5 negatives", "n": n, "d": d, "k": 15,
+                   "n_epochs": args.epochs, "knn_method": "brute", "knn_dtype": "fp32"},
+        "knn": {"tflops_screen": round(screen_tflops, 2), "ms_total": round(knn_total_ms, 3),
+                "ms_phases": {"prepass": round(phases[0], 3), "screen": round(phases[1], 3),
+                              "rerank": round(phases[2], 3), "exact_scan": round(phases[3], 3)},
+                "rows_fp64_scan": stats[0], "rows_fp64_tiecheck": stats[1],
+                "tflops_whole_knn": round(flops / (knn_total_ms * 1e-3) / 1e12, 2)},
+        "phases_s": {k: round(v, 4) for k, v in ph.items()},
+        "quality": {"knn15_preservation": round(quality, 4), "n_edges": g.n_edges},
+        "roofline": {"kernel": "k_screen_f32", "bound": "mfma", "achieved": round(screen_tflops, 2),
+                     "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle restatement, bounded sample)")
+        line["cpu_baseline"] = cpu_baseline(X, um.knn_[0].cpu().numpy(), um.knn_[1].cpu().numpy(), args.epochs)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -82,9 +82,101 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
                  int32_t* out_idx, float* out_dist, int32_t* out_stats,
                  void* ws, size_t ws_bytes, void* stream);
 
+/* Phase timing of mv_knn_brute on the calling thread: with timing on, each
+ * call records HIP events on its stream; mv_knn_last_phase_ms waits for the
+ * last timed call and writes milliseconds of [prepass, screen, rerank,
+ * exact scan] to the host array out4. */
+int mv_knn_set_timing(int on);
+int mv_knn_last_phase_ms(float* out4);
+
 /* Row squared norms: out[i] = sum_t X[i,t]^2 in FP32 (mx.sum(X*X, 1) at
  * mlx_vis/_knn.py:49 and mlx_vis/_nndescent/nndescent.py:51,77). */
 int mv_row_sqnorms(const float* X, int64_t n, int64_t d, float* out, void* stream);
+
+
+/* ----------------------------------------------------------- NNDescent --
+ * Replaces NNDescent.build (mlx_vis/_nndescent/nndescent.py:34-403), the
+ * whole iteration schedule included: init distances and sort (53-66), the
+ * 100-D projection for iteration 0 and the 300-D one for iteration 1 (68-82,
+ * 227-235), FP16 dots in iterations 1-4 (236, 615-641), reverse candidates
+ * while update_frac >= 0.10 (241, 406-436), the full path while
+ * update_frac >= 0.5 and the NEW-first converging path below it (103-220),
+ * dedup/top-k (439-486), the FP32 recompute after iteration 4 (376-378),
+ * positional change flags and the early stop at update_frac < delta (380-398).
+ * The caller supplies the random draws of the reference's NumPy stream:
+ * init_idx (n, k) (nndescent.py:54-56), P1 (d, p1) and P2 (d, p2) (NULL with
+ * p = 0 when the projection is not smaller than d). out_changed (HOST, n_iters
+ * entries, nullable): changed slots per iteration; out_iters (HOST, nullable):
+ * iterations run. out_dist: Euclidean. Limits: k <= 15, d <= 4096.
+ * One device-to-host read of a counter per iteration synchronises `stream`.
+ */
+int mv_nnd_ws_bytes(int64_t n, int64_t d, int k, int64_t p1, int64_t p2, size_t* out_bytes);
+int mv_nnd_build(const float* X, int64_t n, int64_t d, int k, int n_iters, int max_candidates, float delta,
+                 const int32_t* init_idx, const float* P1, int64_t p1, const float* P2, int64_t p2, int verbose,
+                 int32_t* out_idx, float* out_dist, int64_t* out_changed, int* out_iters, void* ws,
+                 size_t ws_bytes, void* stream);
+/* Reverse candidates alone (_compute_reverse_candidates, nndescent.py:406-436):
+ * rev[j, t] = source row of the t-th edge pointing at j in flat (row-major)
+ * order, t < k; slots beyond the in-degree hold 0. ws sized by
+ * mv_nnd_ws_bytes(n, 1, k, 0, 0). */
+int mv_nnd_reverse(const int32_t* idx, int64_t n, int k, int32_t* rev, void* ws, size_t ws_bytes, void* stream);
+/* Squared distances of (i, idx[i, c]) as _gather_dists (half == 0,
+ * nndescent.py:573-612) or _gather_dists_half (half != 0, 615-641: FP16
+ * inputs, FP32 accumulation, FP16-rounded dot, FP32 norms). ws sized by
+ * mv_nnd_ws_bytes(n, d, k, 0, 0). */
+int mv_nnd_gather_dists(const float* X, int64_t n, int64_t d, const int32_t* idx, int k, int half,
+                        float* out_dist, void* ws, size_t ws_bytes, void* stream);
+
+/* ----------------------------------------------------------------- UMAP --
+ * Fuzzy simplicial set, prune and schedule — replaces
+ * UMAP._fuzzy_simplicial_set (mlx_vis/_umap/umap.py:199-270) and the
+ * schedule precompute of UMAP._optimize (umap.py:399-404).
+ * Inputs: the kNN graph (n, k) int32 indices and FP32 Euclidean distances.
+ * Outputs over the dense (n, k) slot array (slot i*k + c is the forward edge
+ * i -> knn_idx[i, c]): w_sym (symmetrised weight w + w' - w*w', umap.py:262),
+ * eps (epochs_per_sample, -1 for slots pruned below max(w_sym)/n_epochs,
+ * umap.py:265-270, 402-403) and epn (epochs_per_next, initialised to eps;
+ * +inf for pruned slots). counters (device, 2 x uint32): [0] kept edges,
+ * [1] bit pattern of max(w_sym). sigma_out / rho_out (device, n, nullable):
+ * per-row bandwidth and rho (umap.py:208-236). k <= 64.
+ */
+int mv_umap_graph_ws_bytes(int64_t n, int k, size_t* out_bytes);
+int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k, int n_epochs,
+                        float* w_sym, float* eps, float* epn, uint32_t* counters, float* sigma_out,
+                        float* rho_out, void* ws, size_t ws_bytes, void* stream);
+
+/* Layout epochs [e0, e1) — replaces the epoch loop of UMAP._optimize
+ * (umap.py:423-447) with UMAP._sgd_step (umap.py:359-387). Y (n, dim) is
+ * updated in place; dY (n, dim) is a zero-filled delta buffer (zero again on
+ * return). Each epoch e is synchronous (all gradients read Y as of the start
+ * of e): edges with eps > 0 and epn <= e are active (then epn += eps); an
+ * active edge moves head and tail with the clipped (+-4) attractive gradient
+ * and the head with neg_rate clipped repulsive gradients against nodes drawn
+ * uniformly from [0, n) by Philox-4x32-10 keyed by seed; the step size is
+ * lr * (1 - e / n_epochs). tail = the kNN indices (n, k). dim <= 4, k <= 64.
+ */
+int mv_umap_epochs(float* Y, float* dY, int64_t n, int dim, const int32_t* tail, int k, const float* eps,
+                   float* epn, int e0, int e1, int n_epochs, int neg_rate, float lr, float a, float b,
+                   uint64_t seed, void* stream);
+
+/* One synchronous step with explicit edge and negative lists, exactly
+ * UMAP._sgd_step (umap.py:359-387): Y_out = Y_in + updates, where every update
+ * is computed from Y_in (Y_out must not alias Y_in). For parity checks. */
+int mv_umap_sgd_step(float* Y_out, const float* Y_in, int64_t n, int dim, const int32_t* ef, const int32_t* et,
+                     int64_t A, const int32_t* neg_from, const int32_t* neg_to, int64_t n_neg, float alpha,
+                     float a, float b, void* stream);
+
+/* Spectral initialisation — replaces UMAP._spectral_init's power iteration
+ * (umap.py:298-340): `iters` products of D^-1/2 W D^-1/2 (degrees from the
+ * heads of the kept slots, umap.py:307-314) with re-orthonormalisation of the
+ * nc columns after each product. V (n, nc) holds the start vectors on entry
+ * and the orthonormal result on return. 2 <= nc <= 5. */
+int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes);
+int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, int64_t n, int k, int nc,
+                     int iters, float* V, void* ws, size_t ws_bytes, void* stream);
+
+/* p[0:count] = 0 on the stream (a fill kernel, safe under graph capture). */
+int mv_zero_f32(float* p, int64_t count, void* stream);
 
 #ifdef __cplusplus
 }

@@ -670,6 +670,25 @@ static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, b
   return L;
 }
 
+// Optional phase timing (mv_knn_set_timing): HIP events recorded on the launch
+// stream around the prepass, screen, rerank and exact-scan launches.
+struct PhaseTimer {
+  bool on = false;
+  bool valid = false;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int ensure() {
+    for (auto& e : ev)
+      if (!e) MV_TRY_HIP(hipEventCreate(&e));
+    return MV_OK;
+  }
+  int mark(int i, hipStream_t s) {
+    if (!on) return MV_OK;
+    MV_TRY_HIP(hipEventRecord(ev[i], s));
+    return MV_OK;
+  }
+};
+static thread_local PhaseTimer g_timer;
+
 static int cu_count_cached() {
   static int cached[64] = {0};
   int dev = 0;
@@ -771,6 +790,11 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   uint32_t* sqmax = reinterpret_cast<uint32_t*>(B.ctr + 1);
   int32_t* stats = out_stats ? out_stats : B.ctr + 2;
 
+  if (g_timer.on) {
+    if (int rc = g_timer.ensure()) return rc;
+    g_timer.valid = false;
+  }
+  if (int rc = g_timer.mark(0, s)) return rc;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, B.ctr, 8);
   MV_LAUNCHED("k_zero_i32");
   if (out_stats) {
@@ -815,8 +839,10 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   sa.cand = B.cand;
   const int per = 8 / S;
   const int64_t grid = ceil_div(row_blocks, per) * 8;
+  if (int rc = g_timer.mark(1, s)) return rc;
   hipLaunchKernelGGL(k_screen_f32, dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, sa);
   MV_LAUNCHED("k_screen_f32");
+  if (int rc = g_timer.mark(2, s)) return rc;
 
   RerankArgs ra;
   ra.Q = Q;
@@ -840,6 +866,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   ra.stats = stats;
   hipLaunchKernelGGL(k_rerank, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, ra);
   MV_LAUNCHED("k_rerank");
+  if (int rc = g_timer.mark(3, s)) return rc;
 
   ScanArgs xa;
   xa.Q = Q;
@@ -858,6 +885,25 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   const int64_t sgrid = nq < 2048 ? nq : 2048;
   hipLaunchKernelGGL(k_exact_scan, dim3(static_cast<unsigned>(sgrid)), dim3(256), 0, s, xa);
   MV_LAUNCHED("k_exact_scan");
+  if (int rc = g_timer.mark(4, s)) return rc;
+  g_timer.valid = g_timer.on;
+  return MV_OK;
+}
+
+int mv_knn_set_timing(int on) {
+  g_timer.on = on != 0;
+  g_timer.valid = false;
+  return MV_OK;
+}
+
+int mv_knn_last_phase_ms(float* out4) {
+  MV_REQUIRE(out4, "mv_knn_last_phase_ms: null output");
+  if (!g_timer.valid) {
+    mv::set_error("mv_knn_last_phase_ms: no timed mv_knn_brute call (enable with mv_knn_set_timing)");
+    return MV_EINVAL;
+  }
+  MV_TRY_HIP(hipEventSynchronize(g_timer.ev[4]));
+  for (int i = 0; i < 4; ++i) MV_TRY_HIP(hipEventElapsedTime(&out4[i], g_timer.ev[i], g_timer.ev[i + 1]));
   return MV_OK;
 }
 

@@ -1,0 +1,696 @@
+// UMAP fuzzy graph and edge-wise layout SGD for gfx950.
+//
+// Replaces, in mlx_vis/_umap/umap.py:
+//   _fuzzy_simplicial_set (199-270)  -> k_fuzzy_rows, k_fuzzy_sym, k_fuzzy_sched
+//   _optimize schedule    (400-414)  -> the per-edge epochs_per_next test in k_epoch
+//   _optimize epoch loop  (423-447)  -> k_epoch + k_apply per epoch
+//   _sgd_step             (359-387)  -> k_sgd_step (explicit negatives, for parity)
+//
+// Graph layout in HBM: the reference keeps the forward edges (i, knn[i, c]) in
+// row-major order (umap.py:243-247) and compacts the survivors of the prune
+// (umap.py:265-270). Here the n x k slot array is kept dense: slot e = i*k + c
+// has head i = e / k, tail knn[e], weight w[e], epochs_per_sample eps[e] (-1 for
+// a pruned slot) and epochs_per_next epn[e]. No compaction or sort is needed:
+// the reverse weight of (i, j) is found by scanning row j's k entries.
+//
+// Semantics kept from the reference:
+//   - sigma bisection per row in FP32, 64 iterations, tolerance 1e-5 around
+//     log2(k), bounds 1e-20 / 1e3, doubling while hi >= 1e3 (umap.py:214-236);
+//   - symmetrised weight w + w' - w*w' on forward edges only (umap.py:249-262);
+//   - prune at max(w_sym)/n_epochs (umap.py:265); schedule in FP32 with
+//     eps = n_epochs / (n_epochs * (w / w_max)) (umap.py:400-404);
+//   - an epoch is synchronous: every gradient of epoch e reads the embedding as
+//     it was at the start of e; updates are accumulated into a delta buffer and
+//     applied at the end (umap.py:361-387 returns a new Y from the old one);
+//   - per active edge: one attractive update of head and tail, and
+//     negative_sample_rate repulsive updates of the head only, each against a
+//     uniformly drawn node (umap.py:433-436, 384-386).
+// Negatives come from Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed
+// by the seed and counted by (epoch, edge slot), so a run is reproducible.
+#include <math.h>
+
+#include "common.hpp"
+
+namespace mv {
+namespace umap {
+
+constexpr int KMAX = 64;
+
+__global__ void k_zero_u32(uint32_t* p, int count) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = 0;
+}
+
+// ------------------------------------------------------------ fuzzy rows --
+// One thread per row: rho, sigma bisection, membership weights
+// (umap.py:208-240).
+// `target` is log2(k) rounded to FP32 on the host (umap.py:204 compares FP32
+// sums with it).
+__global__ __launch_bounds__(256) void k_fuzzy_rows(const float* __restrict__ dist, int64_t n, int k,
+                                                    float target, float* __restrict__ w,
+                                                    float* __restrict__ sigma_out, float* __restrict__ rho_out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* d = dist + i * k;
+  float dv[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) dv[j] = j < k ? d[j] : 0.f;
+
+  float rho = INFINITY;
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j)
+    if (j < k && dv[j] > 0.f) rho = fminf(rho, dv[j]);
+  rho = fmaxf(rho, 1e-8f);
+
+  float sh[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) sh[j] = fmaxf(dv[j] - rho, 0.f);
+
+  float lo = 1e-20f, hi = 1e3f, sigma = 1.f;
+  for (int it = 0; it < 64; ++it) {
+    float s = 0.f;
+#pragma unroll
+    for (int j = 1; j < KMAX; ++j)
+      if (j < k) s += expf(-__fdiv_rn(sh[j], sigma));
+    const bool conv = fabsf(s - target) < 1e-5f;
+    if (conv) break;
+    if (s > target) {
+      hi = sigma;
+      sigma = (lo + sigma) / 2.f;
+    } else if (s < target) {
+      lo = sigma;
+      sigma = hi >= 1e3f ? sigma * 2.f : (sigma + hi) / 2.f;
+    } else {
+      break;  // NaN sums never converge; the reference leaves sigma unchanged
+    }
+  }
+  const float den = fmaxf(sigma, 1e-10f);
+  float* wr = w + i * k;
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j)
+    if (j < k) wr[j] = expf(-__fdiv_rn(sh[j], den));
+  if (sigma_out) sigma_out[i] = sigma;
+  if (rho_out) rho_out[i] = rho;
+}
+
+// w_sym[e] = w + w' - w*w' where w' is the weight of the reverse edge if the
+// tail's own list contains the head (first matching column), else 0.
+__global__ __launch_bounds__(256) void k_fuzzy_sym(const int32_t* __restrict__ knn, const float* __restrict__ w,
+                                                   int64_t n, int k, float* __restrict__ wsym,
+                                                   uint32_t* __restrict__ wmax_bits) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  float v = 0.f;
+  if (e < n * k) {
+    const int64_t i = e / k;
+    const int64_t j = knn[e];
+    const float a = w[e];
+    float b = 0.f;
+    if (j >= 0 && j < n) {
+      const int32_t* row = knn + j * k;
+      for (int c = 0; c < k; ++c)
+        if (row[c] == i) {
+          b = w[j * k + c];
+          break;
+        }
+    }
+    v = a + b - a * b;
+    wsym[e] = v;
+  }
+  // wavefront max, then one atomic per wavefront (weights are >= 0, so the
+  // IEEE bit patterns order like the values)
+  uint32_t bits = __float_as_uint(v);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), m));
+    bits = o > bits ? o : bits;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(wmax_bits, bits);
+}
+
+// Prune and schedule (umap.py:265-270, 400-404) in FP32.
+__global__ __launch_bounds__(256) void k_fuzzy_sched(const float* __restrict__ wsym, int64_t count,
+                                                     int n_epochs, const uint32_t* __restrict__ wmax_bits,
+                                                     float* __restrict__ eps, float* __restrict__ epn,
+                                                     uint32_t* __restrict__ n_edges) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool keep = false;
+  if (e < count) {
+    const float wmax = __uint_as_float(*wmax_bits);
+    const float ne = static_cast<float>(n_epochs);
+    const float thr = __fdiv_rn(wmax, ne);
+    const float v = wsym[e];
+    keep = v >= thr;
+    float s = -1.f;
+    if (keep) {
+      const float ns = __fmul_rn(ne, __fdiv_rn(v, wmax));
+      s = ns > 0.f ? __fdiv_rn(ne, ns) : -1.f;
+    }
+    eps[e] = keep ? s : -1.f;
+    epn[e] = keep ? s : INFINITY;
+  }
+  const uint64_t m = ballot(keep);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(n_edges, static_cast<uint32_t>(__popcll(m)));
+}
+
+// -------------------------------------------------------------- gradients --
+
+struct Law {
+  float a, b, alpha;
+  // Attractive coefficient of umap.py:367-370 (FP32, same operation order).
+  __device__ __forceinline__ float attract(float d2) const {
+    d2 = fmaxf(d2, 1e-6f);
+    const float pw = powf(d2, b);
+    return -2.0f * a * b * powf(d2, b - 1.0f) / (1.0f + a * pw);
+  }
+  // Repulsive coefficient of umap.py:377-380.
+  __device__ __forceinline__ float repulse(float d2) const {
+    d2 = fmaxf(d2, 1e-6f);
+    const float pw = powf(d2, b);
+    return 2.0f * b / ((0.001f + d2) * (1.0f + a * pw));
+  }
+};
+
+__device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 4.0f); }
+
+// Philox-4x32-10.
+struct Philox {
+  __device__ static __forceinline__ void round(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                                               uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
+    const uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+  }
+  __device__ static __forceinline__ uint4 gen(uint4 ctr, uint32_t k0, uint32_t k1) {
+    uint32_t c0 = ctr.x, c1 = ctr.y, c2 = ctr.z, c3 = ctr.w;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      round(c0, c1, c2, c3, k0, k1);
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+  }
+};
+
+__device__ __forceinline__ uint32_t uniform_below(uint32_t r, uint32_t n) {
+  return static_cast<uint32_t>((static_cast<uint64_t>(r) * n) >> 32);
+}
+
+// One synchronous epoch over the dense slot array. Lanes are grouped per
+// head row (G lanes, G = next power of two >= k) so the head's attractive and
+// repulsive updates are reduced in registers and written with one atomic per
+// component per row; tails get one atomic per active edge.
+template <int D, int G>
+__global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ dY,
+                                               const int32_t* __restrict__ tail, const float* __restrict__ eps,
+                                               float* __restrict__ epn, int64_t n, int k, int epoch,
+                                               int neg_rate, Law law, uint32_t seed_lo, uint32_t seed_hi) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = gid / G;      // head row
+  const int c = static_cast<int>(gid % G);
+  const bool in_row = i < n && c < k;
+  const int64_t e = i * k + c;
+  float acc[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) acc[q] = 0.f;
+  bool active = false;
+  if (in_row) {
+    const float s = eps[e];
+    if (s > 0.f) {
+      const float nx = epn[e];
+      if (nx <= static_cast<float>(epoch)) {
+        active = true;
+        epn[e] = nx + s;
+      }
+    }
+  }
+  if (active) {
+    const int64_t j = tail[e];
+    float yi[D], diff[D];
+    float d2 = 0.f;
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      yi[q] = Y[i * D + q];
+      diff[q] = yi[q] - Y[j * D + q];
+      d2 += diff[q] * diff[q];
+    }
+    const float ca = law.attract(d2);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const float g = clip4(ca * diff[q]) * law.alpha;
+      acc[q] += g;
+      atomicAdd(&dY[j * D + q], -g);
+    }
+    uint4 r = make_uint4(0, 0, 0, 0);
+    for (int s2 = 0; s2 < neg_rate; ++s2) {
+      if ((s2 & 3) == 0)
+        r = Philox::gen(make_uint4(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32),
+                                   static_cast<uint32_t>(epoch), static_cast<uint32_t>(s2 >> 2)),
+                        seed_lo, seed_hi);
+      const uint32_t word = (s2 & 3) == 0 ? r.x : (s2 & 3) == 1 ? r.y : (s2 & 3) == 2 ? r.z : r.w;
+      const int64_t t = uniform_below(word, static_cast<uint32_t>(n));
+      float nd[D];
+      float n2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        nd[q] = yi[q] - Y[t * D + q];
+        n2 += nd[q] * nd[q];
+      }
+      const float cr = law.repulse(n2);
+#pragma unroll
+      for (int q = 0; q < D; ++q) acc[q] += clip4(cr * nd[q]) * law.alpha;
+    }
+  }
+  // Reduce the head's updates over its G lanes.
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) acc[q] += __shfl_xor(acc[q], m);
+  }
+  const uint64_t bal = ballot(active);
+  if (c == 0 && i < n) {
+    // a row's G lanes are contiguous within the wavefront
+    const int base = (threadIdx.x & 63) & ~(G - 1);
+    const uint64_t mask = (G == 64) ? ~0ull : (((1ull << G) - 1) << base);
+    if (bal & mask) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) atomicAdd(&dY[i * D + q], acc[q]);
+    }
+  }
+}
+
+// Y += dY; dY = 0.
+__global__ __launch_bounds__(256) void k_apply(float* __restrict__ Y, float* __restrict__ dY, int64_t count) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) {
+    Y[t] += dY[t];
+    dY[t] = 0.f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zero_f32(float* __restrict__ p, int64_t count) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_copy_f32(float* __restrict__ dst, const float* __restrict__ src,
+                                                  int64_t count) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) dst[t] = src[t];
+}
+
+// Exact _sgd_step (umap.py:361-387) with explicit edge and negative lists:
+// Y_out must hold a copy of Y_in; every update reads Y_in.
+__global__ __launch_bounds__(256) void k_sgd_step(const float* __restrict__ Yin, float* __restrict__ Yout,
+                                                  int64_t n, int dim, const int32_t* __restrict__ ef,
+                                                  const int32_t* __restrict__ et, int64_t A,
+                                                  const int32_t* __restrict__ nf, const int32_t* __restrict__ nt,
+                                                  int64_t n_neg, Law law) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < A) {
+    const int64_t f = ef[t], g = et[t];
+    float d2 = 0.f;
+    for (int q = 0; q < dim; ++q) {
+      const float df = Yin[f * dim + q] - Yin[g * dim + q];
+      d2 += df * df;
+    }
+    const float ca = law.attract(d2);
+    for (int q = 0; q < dim; ++q) {
+      const float df = Yin[f * dim + q] - Yin[g * dim + q];
+      const float gr = clip4(ca * df) * law.alpha;
+      atomicAdd(&Yout[f * dim + q], gr);
+      atomicAdd(&Yout[g * dim + q], -gr);
+    }
+  } else if (t < A + n_neg) {
+    const int64_t u = t - A;
+    const int64_t f = nf[u], g = nt[u];
+    float d2 = 0.f;
+    for (int q = 0; q < dim; ++q) {
+      const float df = Yin[f * dim + q] - Yin[g * dim + q];
+      d2 += df * df;
+    }
+    const float cr = law.repulse(d2);
+    for (int q = 0; q < dim; ++q) {
+      const float df = Yin[f * dim + q] - Yin[g * dim + q];
+      atomicAdd(&Yout[f * dim + q], clip4(cr * df) * law.alpha);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- spectral --
+// Power iteration of D^-1/2 W D^-1/2 on the kept forward edges with
+// re-orthonormalisation after every product (umap.py:298-337). The reference
+// orthonormalises with modified Gram-Schmidt; here the Gram matrix of the
+// product is accumulated during the product (FP64) and the Cholesky factor R
+// (U = Q R, positive diagonal) gives the same Q = U R^-1, applied lazily by the
+// next product. Degrees come from the heads only (umap.py:307-311).
+
+__global__ __launch_bounds__(256) void k_spec_prep(const float* __restrict__ wsym, const float* __restrict__ eps,
+                                                   int64_t n, int k, float* __restrict__ dis) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float deg = 0.f;
+  for (int c = 0; c < k; ++c) {
+    const int64_t e = i * k + c;
+    if (eps[e] > 0.f) deg += wsym[e];
+  }
+  dis[i] = 1.0f / sqrtf(fmaxf(deg, 1e-10f));
+}
+
+__global__ __launch_bounds__(256) void k_spec_wnorm(const float* __restrict__ wsym, const float* __restrict__ eps,
+                                                    const int32_t* __restrict__ tail, const float* __restrict__ dis,
+                                                    int64_t n, int k, float* __restrict__ wn) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * k) return;
+  const int64_t i = e / k;
+  wn[e] = eps[e] > 0.f ? wsym[e] * dis[i] * dis[tail[e]] : 0.f;
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_spec_spmv(const float* __restrict__ U, const float* __restrict__ Rinv,
+                                                   const float* __restrict__ wn, const int32_t* __restrict__ tail,
+                                                   int64_t n, int k, float* __restrict__ Unext,
+                                                   double* __restrict__ gram) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  float t[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) t[q] = 0.f;
+  if (i < n) {
+    for (int c = 0; c < k; ++c) {
+      const int64_t e = i * k + c;
+      const float w = wn[e];
+      if (w != 0.f) {
+        const int64_t j = tail[e];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) t[q] += w * U[j * NC + q];
+      }
+    }
+  }
+  float u[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    float acc = 0.f;
+#pragma unroll
+    for (int p = 0; p <= q; ++p) acc += t[p] * Rinv[p * NC + q];   // Rinv is upper triangular
+    u[q] = acc;
+  }
+  if (i < n) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) Unext[i * NC + q] = u[q];
+  }
+  int idx = 0;
+#pragma unroll
+  for (int p = 0; p < NC; ++p) {
+#pragma unroll
+    for (int q = p; q < NC; ++q) {
+      double v = static_cast<double>(u[p]) * static_cast<double>(u[q]);
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+      if ((threadIdx.x & 63) == 0) atomicAdd(&gram[idx], v);
+      ++idx;
+    }
+  }
+}
+
+// Cholesky of the packed Gram matrix (G = R^T R, R upper, pivots
+// sqrt(. + 1e-10) as the reference's norms, umap.py:335) and R^-1. Clears gram.
+template <int NC>
+__global__ void k_spec_chol(double* __restrict__ gram, float* __restrict__ Rinv, float* __restrict__ Rinv_prev) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double G[NC][NC];
+  int idx = 0;
+  for (int p = 0; p < NC; ++p)
+    for (int q = p; q < NC; ++q) {
+      G[p][q] = gram[idx];
+      G[q][p] = gram[idx];
+      gram[idx] = 0.0;
+      ++idx;
+    }
+  double R[NC][NC];
+  for (int p = 0; p < NC; ++p)
+    for (int q = 0; q < NC; ++q) R[p][q] = 0.0;
+  for (int j = 0; j < NC; ++j) {
+    double s = G[j][j];
+    for (int i = 0; i < j; ++i) s -= R[i][j] * R[i][j];
+    const double rjj = sqrt(fmax(s, 0.0) + 1e-10);
+    R[j][j] = rjj;
+    for (int q = j + 1; q < NC; ++q) {
+      double v = G[j][q];
+      for (int i = 0; i < j; ++i) v -= R[i][j] * R[i][q];
+      R[j][q] = v / rjj;
+    }
+  }
+  // Inverse of an upper-triangular matrix by back substitution.
+  double Ri[NC][NC];
+  for (int p = 0; p < NC; ++p)
+    for (int q = 0; q < NC; ++q) Ri[p][q] = 0.0;
+  for (int q = 0; q < NC; ++q) {
+    Ri[q][q] = 1.0 / R[q][q];
+    for (int p = q - 1; p >= 0; --p) {
+      double v = 0.0;
+      for (int m = p + 1; m <= q; ++m) v += R[p][m] * Ri[m][q];
+      Ri[p][q] = -v / R[p][p];
+    }
+  }
+  for (int p = 0; p < NC; ++p)
+    for (int q = 0; q < NC; ++q) {
+      if (Rinv_prev) Rinv_prev[p * NC + q] = Rinv[p * NC + q];
+      Rinv[p * NC + q] = static_cast<float>(Ri[p][q]);
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U, const float* __restrict__ Rinv,
+                                                    int64_t n, float* __restrict__ V) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    float acc = 0.f;
+#pragma unroll
+    for (int p = 0; p <= q; ++p) acc += U[i * NC + p] * Rinv[p * NC + q];
+    V[i * NC + q] = acc;
+  }
+}
+
+__global__ void k_eye_f32(float* __restrict__ M, int nc) {
+  const int t = threadIdx.x;
+  if (t < nc * nc) M[t] = (t / nc == t % nc) ? 1.f : 0.f;
+}
+
+template <int NC>
+static int spectral_run(const float* wsym, const float* eps, const int32_t* tail, int64_t n, int k, int iters,
+                        float* V, char* ws, hipStream_t s) {
+  Carve cv(ws);
+  float* dis = cv.take<float>(n);
+  float* wn = cv.take<float>(static_cast<size_t>(n) * k);
+  float* U0 = cv.take<float>(static_cast<size_t>(n) * NC);
+  float* U1 = cv.take<float>(static_cast<size_t>(n) * NC);
+  float* Rinv = cv.take<float>(NC * NC);
+  double* gram = cv.take<double>(NC * NC);
+  const unsigned rb = static_cast<unsigned>(ceil_div(n, 256));
+  hipLaunchKernelGGL(k_spec_prep, dim3(rb), dim3(256), 0, s, wsym, eps, n, k, dis);
+  MV_LAUNCHED("k_spec_prep");
+  hipLaunchKernelGGL(k_spec_wnorm, dim3(static_cast<unsigned>(ceil_div(n * k, 256))), dim3(256), 0, s, wsym, eps,
+                     tail, dis, n, k, wn);
+  MV_LAUNCHED("k_spec_wnorm");
+  hipLaunchKernelGGL(k_eye_f32, dim3(1), dim3(64), 0, s, Rinv, NC);
+  MV_LAUNCHED("k_eye_f32");
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s, reinterpret_cast<float*>(gram),
+                     static_cast<int64_t>(2 * NC * NC));
+  MV_LAUNCHED("k_zero_f32");
+  hipLaunchKernelGGL(k_copy_f32, dim3(static_cast<unsigned>(ceil_div(n * NC, 256))), dim3(256), 0, s, U0, V,
+                     n * NC);
+  MV_LAUNCHED("k_copy_f32");
+  float* cur = U0;
+  float* nxt = U1;
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL((k_spec_spmv<NC>), dim3(rb), dim3(256), 0, s, cur, Rinv, wn, tail, n, k, nxt, gram);
+    MV_LAUNCHED("k_spec_spmv");
+    hipLaunchKernelGGL((k_spec_chol<NC>), dim3(1), dim3(64), 0, s, gram, Rinv, static_cast<float*>(nullptr));
+    MV_LAUNCHED("k_spec_chol");
+    float* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  hipLaunchKernelGGL((k_spec_apply<NC>), dim3(rb), dim3(256), 0, s, cur, Rinv, n, V);
+  MV_LAUNCHED("k_spec_apply");
+  return MV_OK;
+}
+
+static size_t spectral_ws(int64_t n, int k, int nc) {
+  return align_up(n * 4, 256) + align_up(static_cast<size_t>(n) * k * 4, 256) +
+         2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + 2 * align_up(static_cast<size_t>(nc) * nc * 8, 256) +
+         1024;
+}
+
+template <int D, int G>
+static int launch_epoch(const float* Y, float* dY, const int32_t* tail, const float* eps, float* epn, int64_t n,
+                        int k, int epoch, int neg_rate, Law law, uint64_t seed, hipStream_t s) {
+  const int64_t threads = n * G;
+  const unsigned blocks = static_cast<unsigned>(ceil_div(threads, 256));
+  hipLaunchKernelGGL((k_epoch<D, G>), dim3(blocks), dim3(256), 0, s, Y, dY, tail, eps, epn, n, k, epoch, neg_rate,
+                     law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  MV_LAUNCHED("k_epoch");
+  return MV_OK;
+}
+
+template <int D>
+static int launch_epoch_g(const float* Y, float* dY, const int32_t* tail, const float* eps, float* epn,
+                          int64_t n, int k, int epoch, int neg_rate, Law law, uint64_t seed, hipStream_t s) {
+  if (k <= 8) return launch_epoch<D, 8>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  if (k <= 16) return launch_epoch<D, 16>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  if (k <= 32) return launch_epoch<D, 32>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  return launch_epoch<D, 64>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+}
+
+}  // namespace umap
+}  // namespace mv
+
+using namespace mv;
+using namespace mv::umap;
+
+extern "C" {
+
+int mv_umap_graph_ws_bytes(int64_t n, int k, size_t* out_bytes) {
+  MV_REQUIRE(out_bytes, "mv_umap_graph_ws_bytes: out_bytes is NULL");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX, "mv_umap_graph_ws_bytes: bad n=%lld k=%d", (long long)n, k);
+  *out_bytes = align_up(static_cast<size_t>(n) * k * sizeof(float), 256) + 256;
+  return MV_OK;
+}
+
+int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k, int n_epochs,
+                        float* w_sym, float* eps, float* epn, uint32_t* counters, float* sigma_out,
+                        float* rho_out, void* ws, size_t ws_bytes, void* stream) {
+  MV_REQUIRE(knn_idx && knn_dist && w_sym && eps && epn && counters, "mv_umap_fuzzy_graph: null pointer");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX, "mv_umap_fuzzy_graph: bad n=%lld k=%d (k <= %d)", (long long)n, k,
+             KMAX);
+  MV_REQUIRE(n_epochs >= 1, "mv_umap_fuzzy_graph: n_epochs=%d < 1", n_epochs);
+  size_t need = 0;
+  mv_umap_graph_ws_bytes(n, k, &need);
+  if (!ws || ws_bytes < need) {
+    mv::set_error("mv_umap_fuzzy_graph: workspace of %zu bytes < required %zu", ws_bytes, need);
+    return MV_EWORKSPACE;
+  }
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float* w = static_cast<float*>(ws);
+  const int64_t count = n * k;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counters, 2);
+  MV_LAUNCHED("k_zero_u32");
+  hipLaunchKernelGGL(k_fuzzy_rows, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, knn_dist, n, k,
+                     static_cast<float>(log2(static_cast<double>(k))), w, sigma_out, rho_out);
+  MV_LAUNCHED("k_fuzzy_rows");
+  hipLaunchKernelGGL(k_fuzzy_sym, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, knn_idx, w,
+                     n, k, w_sym, counters + 1);
+  MV_LAUNCHED("k_fuzzy_sym");
+  hipLaunchKernelGGL(k_fuzzy_sched, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, w_sym,
+                     count, n_epochs, counters + 1, eps, epn, counters);
+  MV_LAUNCHED("k_fuzzy_sched");
+  return MV_OK;
+}
+
+int mv_umap_epochs(float* Y, float* dY, int64_t n, int dim, const int32_t* tail, int k, const float* eps,
+                   float* epn, int e0, int e1, int n_epochs, int neg_rate, float lr, float a, float b,
+                   uint64_t seed, void* stream) {
+  MV_REQUIRE(Y && dY && tail && eps && epn, "mv_umap_epochs: null pointer");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= 64, "mv_umap_epochs: bad n=%lld k=%d", (long long)n, k);
+  MV_REQUIRE(dim >= 1 && dim <= 4, "mv_umap_epochs: dim=%d outside [1, 4]", dim);
+  MV_REQUIRE(n < (1ll << 32), "mv_umap_epochs: n too large");
+  MV_REQUIRE(0 <= e0 && e0 <= e1 && e1 <= n_epochs, "mv_umap_epochs: bad epoch range [%d, %d) of %d", e0, e1,
+             n_epochs);
+  MV_REQUIRE(neg_rate >= 0, "mv_umap_epochs: negative_sample_rate < 0");
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t count = n * dim;
+  const unsigned ablocks = static_cast<unsigned>(ceil_div(count, 256));
+  for (int ep = e0; ep < e1; ++ep) {
+    Law law;
+    law.a = a;
+    law.b = b;
+    law.alpha = static_cast<float>(static_cast<double>(lr) * (1.0 - static_cast<double>(ep) / n_epochs));
+    int rc = MV_OK;
+    switch (dim) {
+      case 1: rc = launch_epoch_g<1>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
+      case 2: rc = launch_epoch_g<2>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
+      case 3: rc = launch_epoch_g<3>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
+      default: rc = launch_epoch_g<4>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
+    }
+    if (rc != MV_OK) return rc;
+    hipLaunchKernelGGL(k_apply, dim3(ablocks), dim3(256), 0, s, Y, dY, count);
+    MV_LAUNCHED("k_apply");
+  }
+  return MV_OK;
+}
+
+int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes) {
+  MV_REQUIRE(out_bytes, "mv_umap_spectral_ws_bytes: out_bytes is NULL");
+  MV_REQUIRE(n >= 1 && k >= 1 && nc >= 2 && nc <= 5, "mv_umap_spectral_ws_bytes: bad n=%lld k=%d nc=%d",
+             (long long)n, k, nc);
+  *out_bytes = spectral_ws(n, k, nc);
+  return MV_OK;
+}
+
+int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, int64_t n, int k, int nc,
+                     int iters, float* V, void* ws, size_t ws_bytes, void* stream) {
+  MV_REQUIRE(w_sym && eps && tail && V, "mv_umap_spectral: null pointer");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX && nc >= 2 && nc <= 5 && iters >= 0,
+             "mv_umap_spectral: bad n=%lld k=%d nc=%d iters=%d", (long long)n, k, nc, iters);
+  if (!ws || ws_bytes < spectral_ws(n, k, nc)) {
+    mv::set_error("mv_umap_spectral: workspace of %zu bytes < required %zu", ws_bytes, spectral_ws(n, k, nc));
+    return MV_EWORKSPACE;
+  }
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char* w = static_cast<char*>(ws);
+  switch (nc) {
+    case 2: return spectral_run<2>(w_sym, eps, tail, n, k, iters, V, w, s);
+    case 3: return spectral_run<3>(w_sym, eps, tail, n, k, iters, V, w, s);
+    case 4: return spectral_run<4>(w_sym, eps, tail, n, k, iters, V, w, s);
+    default: return spectral_run<5>(w_sym, eps, tail, n, k, iters, V, w, s);
+  }
+}
+
+int mv_zero_f32(float* p, int64_t count, void* stream) {
+  MV_REQUIRE(p || count == 0, "mv_zero_f32: null pointer");
+  MV_FAULT_POINT("launch");
+  if (count == 0) return MV_OK;
+  hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), p, count);
+  MV_LAUNCHED("k_zero_f32");
+  return MV_OK;
+}
+
+int mv_umap_sgd_step(float* Y_out, const float* Y_in, int64_t n, int dim, const int32_t* ef, const int32_t* et,
+                     int64_t A, const int32_t* neg_from, const int32_t* neg_to, int64_t n_neg, float alpha,
+                     float a, float b, void* stream) {
+  MV_REQUIRE(Y_out && Y_in, "mv_umap_sgd_step: null embedding pointer");
+  MV_REQUIRE(A == 0 || (ef && et), "mv_umap_sgd_step: null edge list");
+  MV_REQUIRE(n_neg == 0 || (neg_from && neg_to), "mv_umap_sgd_step: null negative list");
+  MV_REQUIRE(n >= 1 && dim >= 1 && A >= 0 && n_neg >= 0, "mv_umap_sgd_step: bad sizes");
+  MV_REQUIRE(Y_out != Y_in, "mv_umap_sgd_step: Y_out must not alias Y_in");
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t count = n * dim;
+  hipLaunchKernelGGL(k_copy_f32, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, Y_out, Y_in,
+                     count);
+  MV_LAUNCHED("k_copy_f32");
+  Law law;
+  law.a = a;
+  law.b = b;
+  law.alpha = alpha;
+  const int64_t total = A + n_neg;
+  if (total > 0) {
+    hipLaunchKernelGGL(k_sgd_step, dim3(static_cast<unsigned>(ceil_div(total, 256))), dim3(256), 0, s, Y_in, Y_out,
+                       n, dim, ef, et, A, neg_from, neg_to, n_neg, law);
+    MV_LAUNCHED("k_sgd_step");
+  }
+  return MV_OK;
+}
+
+}  // extern "C"

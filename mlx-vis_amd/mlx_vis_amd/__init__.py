@@ -9,5 +9,7 @@ __version__ = "0.1.0"
 
 from ._knn import compute_knn, knn_brute_device
 from ._normalize import normalize_input
+from ._nndescent import NNDescent
+from ._umap import UMAP, find_ab_params
 
-__all__ = ["compute_knn", "knn_brute_device", "normalize_input"]
+__all__ = ["UMAP", "NNDescent", "compute_knn", "knn_brute_device", "normalize_input", "find_ab_params"]

@@ -29,6 +29,8 @@ COMMON_FLAGS = [
     f"--offload-arch={ARCH}",
     # Packed FP32 instructions are kept out of generated code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # float atomicAdd on device memory maps to the hardware instruction
+    "-munsafe-fp-atomics",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-variable",

@@ -50,6 +50,21 @@ _SIGNATURES: dict[str, list] = {
     "mv_knn_ws_bytes": [_i64, _i64, _i64, _i32, _i32, _i32, _i32, C.POINTER(_sz)],
     "mv_knn_brute": [_p, _i64, _p, _i64, _i64, _i32, _i32, _i32, _p, _i64, _i32, _p, _p, _p, _p, _sz, _p],
     "mv_row_sqnorms": [_p, _i64, _i64, _p, _p],
+    "mv_knn_set_timing": [_i32],
+    "mv_knn_last_phase_ms": [_p],
+    "mv_umap_graph_ws_bytes": [_i64, _i32, C.POINTER(_sz)],
+    "mv_umap_fuzzy_graph": [_p, _p, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _sz, _p],
+    "mv_umap_epochs": [_p, _p, _i64, _i32, _p, _i32, _p, _p, _i32, _i32, _i32, _i32, _f32, _f32, _f32,
+                       _u64, _p],
+    "mv_umap_sgd_step": [_p, _p, _i64, _i32, _p, _p, _i64, _p, _p, _i64, _f32, _f32, _f32, _p],
+    "mv_zero_f32": [_p, _i64, _p],
+    "mv_nnd_ws_bytes": [_i64, _i64, _i32, _i64, _i64, C.POINTER(_sz)],
+    "mv_nnd_build": [_p, _i64, _i64, _i32, _i32, _i32, _f32, _p, _p, _i64, _p, _i64, _i32, _p, _p, _p, _p, _p,
+                     _sz, _p],
+    "mv_nnd_reverse": [_p, _i64, _i32, _p, _p, _sz, _p],
+    "mv_nnd_gather_dists": [_p, _i64, _i64, _p, _i32, _i32, _p, _p, _sz, _p],
+    "mv_umap_spectral_ws_bytes": [_i64, _i32, _i32, C.POINTER(_sz)],
+    "mv_umap_spectral": [_p, _p, _p, _i64, _i32, _i32, _i32, _p, _p, _sz, _p],
 }
 _RESTYPES = {"mv_last_error": C.c_char_p}
 

@@ -149,3 +149,16 @@ def knn_loops(Q, X, k: int, q_self=None):
         res_i.append([j for _, j in pairs[:k]])
         res_d.append([v for v, _ in pairs[:k]])
     return np.array(res_i, dtype=np.int64), np.array(res_d, dtype=np.float64)
+
+
+def reference_knn_rows(X, rows, k: int, return_euclidean: bool = True):
+    """``reference_knn`` for a subset of query rows (the same per-row work:
+    FP32 norm expansion against all of X, +1e30 self mask, full stable argsort)."""
+    X = np.asarray(X, dtype=np.float32)
+    rows = np.asarray(rows)
+    sq = np.einsum("ij,ij->i", X, X, dtype=np.float32)
+    D = np.maximum(sq[rows, None] + sq[None, :] - np.float32(2.0) * (X[rows] @ X.T), np.float32(0.0))
+    D[np.arange(rows.size), rows] += np.float32(1e30)
+    idx = np.argsort(D, axis=1, kind="stable")[:, :k]
+    d = np.take_along_axis(D, idx, axis=1)
+    return idx.astype(np.int32), (np.sqrt(np.maximum(d, 0)) if return_euclidean else d).astype(np.float32)

@@ -1,0 +1,95 @@
+"""Parity of the HIP NNDescent with the NumPy restatement of the reference."""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import knn as oknn
+from oracle import metrics as om
+from oracle import nndescent as onn
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, d, seed, k_centers=8, scale=3.0):
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal((k_centers, d)) * scale
+    return (c[rng.integers(0, k_centers, n)] + rng.standard_normal((n, d))).astype(np.float32)
+
+
+def _ws(n, d, k):
+    from mlx_vis_amd import _capi, _device
+
+    nb = ctypes.c_size_t(0)
+    _capi.check(_capi.lib().mv_nnd_ws_bytes(n, d, k, 0, 0, ctypes.byref(nb)))
+    return _device.workspace(nb.value)
+
+
+def test_reverse_candidates_bit_exact():
+    from mlx_vis_amd import _capi
+
+    rng = np.random.default_rng(0)
+    n, k = 5000, 15
+    idx = rng.integers(0, n, size=(n, k)).astype(np.int32)
+    idx[:50] = 7  # a hub with far more than k in-edges
+    ws = _ws(n, 1, k)
+    rev = torch.empty((n, k), dtype=torch.int32, device="cuda")
+    _capi.check(_capi.lib().mv_nnd_reverse(torch.from_numpy(idx).cuda().data_ptr(), n, k, rev.data_ptr(),
+                                           ws.data_ptr(), ws.numel(), None))
+    np.testing.assert_array_equal(rev.cpu().numpy(), onn.reverse_candidates(idx))
+
+
+@pytest.mark.parametrize("half", [0, 1])
+def test_gather_dists(half):
+    from mlx_vis_amd import _capi
+
+    n, d, k = 3000, 100, 15
+    X = _data(n, d, 1)
+    rng = np.random.default_rng(2)
+    idx = rng.integers(0, n, size=(n, k)).astype(np.int32)
+    ws = _ws(n, d, k)
+    out = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    _capi.check(_capi.lib().mv_nnd_gather_dists(torch.from_numpy(X).cuda().data_ptr(), n, d,
+                                                torch.from_numpy(idx).cuda().data_ptr(), k, half, out.data_ptr(),
+                                                ws.data_ptr(), ws.numel(), None))
+    got = out.cpu().numpy()
+    sq = np.einsum("ij,ij->i", X, X).astype(np.float32)
+    want = onn.gather_dists(X, sq, idx, half=bool(half))
+    scale = sq[:, None] + sq[idx]
+    if half:
+        # the dot is rounded to FP16: allow one FP16 ulp of the dot (x2)
+        dots = np.abs(np.einsum("id,icd->ic", X, X[idx]))
+        ulp = 2.0 ** (np.floor(np.log2(np.maximum(dots, 1e-30))) - 10)
+        assert (np.abs(got - want) <= 2 * ulp + 1e-5 * scale).all()
+    else:
+        np.testing.assert_allclose(got, want, rtol=0, atol=2e-6 * scale.max())
+
+
+@pytest.mark.parametrize("n,d,seed", [(4000, 64, 3), (3000, 784, 4), (6000, 128, 5)])
+def test_build_recall_not_below_reference(n, d, seed):
+    from mlx_vis_amd import NNDescent
+
+    print(f"case n={n} d={d}", flush=True)
+    X = _data(n, d, seed)
+    nn = NNDescent(k=15, n_iters=12, random_state=0)
+    gi, gd = nn.build(X)
+    oi, od, trace = onn.nndescent(X, k=15, n_iters=12, random_state=0, return_trace=True)
+    exact, _ = oknn.exact_knn(X, X, 15, self_base=0)
+    rg, ro = om.recall(gi, exact), om.recall(oi, exact)
+    print(f"recall gpu={rg:.4f} oracle={ro:.4f} iters gpu={nn.iterations_} {nn.changed_} oracle={trace}")
+    assert (gi != np.arange(n)[:, None]).all()
+    assert all(len(set(r)) == 15 for r in gi)
+    assert (np.diff(gd, axis=1) >= -1e-6).all()
+    assert rg >= ro - 0.002
+
+
+def test_knn_method_nndescent_via_compute_knn():
+    from mlx_vis_amd import compute_knn
+
+    X = _data(2500, 32, 6)
+    i1, d1 = compute_knn(X, 10, method="nndescent", random_state=1)
+    i2, d2 = compute_knn(X, 10, method="nndescent", random_state=1, return_euclidean=False)
+    np.testing.assert_array_equal(i1, i2)
+    np.testing.assert_allclose(d1 ** 2, d2, rtol=1e-6)

@@ -1,0 +1,149 @@
+"""Parity of the HIP UMAP graph, schedule, SGD step and layout with the oracle."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import knn as oknn
+from oracle import metrics as om
+from oracle import umap as ou
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph_inputs(n=3000, d=32, k=15, seed=0):
+    rng = np.random.default_rng(seed)
+    centers = rng.standard_normal((8, d)) * 3
+    X = (centers[rng.integers(0, 8, n)] + rng.standard_normal((n, d))).astype(np.float32)
+    idx, d2 = oknn.exact_knn(X, X, k, self_base=0)
+    return X, idx.astype(np.int32), np.sqrt(d2).astype(np.float32)
+
+
+def test_fuzzy_graph_matches_oracle():
+    from mlx_vis_amd._umap import fuzzy_graph_device
+
+    X, idx, dist = _graph_inputs()
+    n_epochs = 200
+    g = fuzzy_graph_device(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n_epochs,
+                           keep_sigma=True)
+    w, sigma, rho = ou.fuzzy_rows(dist, 15)
+    np.testing.assert_array_equal(g.rho.cpu().numpy(), rho)
+    np.testing.assert_allclose(g.sigma.cpu().numpy(), sigma, rtol=1e-5)
+    ws = ou.symmetrize(idx, w)
+    np.testing.assert_allclose(g.w_sym.cpu().numpy(), ws, rtol=2e-5, atol=1e-7)
+    keep = ou.prune(ws, n_epochs)
+    gk = g.eps.cpu().numpy() > 0
+    # prune decisions may differ only for weights within rounding of the threshold
+    thr = ws.max() / n_epochs
+    diff = gk != keep
+    assert np.all(np.abs(ws[diff] - thr) < 1e-5 * max(thr, 1e-6) + 1e-7)
+    assert g.n_edges == int(gk.sum())
+    assert abs(g.w_max - ws.max()) <= 1e-6
+
+
+def test_schedule_bit_exact():
+    """epochs_per_next after e epochs equals the FP32 restatement given the same weights."""
+    from mlx_vis_amd._umap import fuzzy_graph_device, run_epochs
+
+    X, idx, dist = _graph_inputs(n=1500)
+    n_epochs = 60
+    g = fuzzy_graph_device(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n_epochs)
+    wsym = g.w_sym.cpu().numpy().reshape(-1)
+    kept = g.eps.cpu().numpy().reshape(-1) > 0
+    eps_or = ou.epochs_per_sample(wsym[kept], n_epochs)
+    np.testing.assert_array_equal(g.eps.cpu().numpy().reshape(-1)[kept], eps_or)
+    Y = torch.zeros((1500, 2), dtype=torch.float32, device="cuda")
+    dY = torch.zeros_like(Y)
+    run_epochs(Y, dY, g, 1, n_epochs, n_epochs, 0, 1.0, 1.577, 0.895, 7)
+    epn = g.epn.cpu().numpy().reshape(-1)[kept]
+    ref = eps_or.copy()
+    for epoch in range(n_epochs):
+        act = ref <= epoch
+        ref[act] = (ref[act] + eps_or[act]).astype(np.float32)
+    np.testing.assert_array_equal(epn, ref)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_single_sgd_step_matches_fp64_oracle(dim):
+    """One layout-gradient step on fixed inputs, tolerance 1e-5."""
+    from mlx_vis_amd._umap import sgd_step_device
+
+    rng = np.random.default_rng(11)
+    n, A = 5000, 4096
+    Y = rng.uniform(-10, 10, size=(n, dim)).astype(np.float32)
+    ef = rng.integers(0, n, A).astype(np.int32)
+    et = rng.integers(0, n, A).astype(np.int32)
+    nf = ef[np.arange(5 * A) % A]
+    nt = rng.integers(0, n, 5 * A).astype(np.int32)
+    Y[et[:10]] = Y[ef[:10]] + 1e-4  # near-coincident pairs hit the 1e-6 clamp
+    a, b, alpha = 1.5769434603113077, 0.8950608779109733, 0.75
+    got = sgd_step_device(torch.from_numpy(Y).cuda(), *(torch.from_numpy(v).cuda() for v in (ef, et, nf, nt)),
+                          alpha, a, b).cpu().numpy()
+    want = ou.sgd_step(Y, ef, et, nf, nt, np.float32(alpha), np.float32(a), np.float32(b), dtype=np.float64)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
+
+
+def test_layout_quality_matches_oracle_run():
+    """Same graph, a/b and init: 15-NN preservation and trustworthiness of the
+    GPU layout within 0.01 of the oracle's epoch loop."""
+    from mlx_vis_amd._umap import FuzzyGraph, fuzzy_graph_device, run_epochs
+
+    n, n_epochs = 3000, 200
+    X, idx, dist = _graph_inputs(n=n, d=32, seed=3)
+    a, b = ou.find_ab_params(1.0, 0.1)
+    Y0 = np.random.RandomState(0).uniform(-10, 10, size=(n, 2)).astype(np.float32)
+    g = fuzzy_graph_device(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n_epochs)
+    Y = torch.from_numpy(Y0).cuda()
+    run_epochs(Y, torch.zeros_like(Y), g, 1, n_epochs, n_epochs, 5, 1.0, a, b, 1234)
+    Yg = Y.cpu().numpy()
+    rows, cols, vals = ou.fuzzy_simplicial_set(idx, dist, n_epochs)
+    Yo = ou.optimize(rows, cols, vals, Y0, a, b, n_epochs, seed=0)
+    sample = np.random.default_rng(0).choice(n, 600, replace=False)
+    pg = om.knn_preservation(X, Yg, 15)
+    po = om.knn_preservation(X, Yo, 15)
+    tg = om.trustworthiness(X, Yg, 15, rows=sample)
+    to = om.trustworthiness(X, Yo, 15, rows=sample)
+    print(f"preservation gpu={pg:.4f} oracle={po:.4f}; trust gpu={tg:.4f} oracle={to:.4f}")
+    assert np.isfinite(Yg).all()
+    assert abs(pg - po) < 0.01
+    assert abs(tg - to) < 0.01
+
+
+def test_spectral_init_subspace_matches_oracle():
+    from mlx_vis_amd._umap import fuzzy_graph_device
+    from mlx_vis_amd import _capi, _device
+    import ctypes
+
+    n = 2000
+    X, idx, dist = _graph_inputs(n=n, d=16, seed=5)
+    g = fuzzy_graph_device(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), 200)
+    V0 = np.random.RandomState(1).standard_normal((n, 3)).astype(np.float32)
+    V = torch.from_numpy(V0).cuda()
+    lib = _capi.lib()
+    nb = ctypes.c_size_t(0)
+    _capi.check(lib.mv_umap_spectral_ws_bytes(n, 15, 3, ctypes.byref(nb)))
+    ws = _device.workspace(nb.value)
+    _capi.check(lib.mv_umap_spectral(g.w_sym.data_ptr(), g.eps.data_ptr(), g.knn_idx.data_ptr(), n, 15, 3, 100,
+                                     V.data_ptr(), ws.data_ptr(), ws.numel(), None))
+    Vg = V.cpu().numpy().astype(np.float64)
+    rows, cols, vals = ou.fuzzy_simplicial_set(idx, dist, 200)
+    Vo = ou.spectral_power(rows, cols, vals.astype(np.float64), n, V0, iters=100)
+    # orthonormal columns, and the same leading vector
+    np.testing.assert_allclose(Vg.T @ Vg, np.eye(3), atol=1e-4)
+    assert abs(abs(Vg[:, 0] @ Vo[:, 0]) - 1.0) < 1e-4
+    # same 3-D subspace
+    s = np.linalg.svd(Vg.T @ Vo, compute_uv=False)
+    assert s.min() > 0.99
+
+
+def test_fit_transform_end_to_end():
+    from mlx_vis_amd import UMAP
+
+    X, _, _ = _graph_inputs(n=2500, d=24, seed=9)
+    calls = []
+    um = UMAP(n_epochs=50, random_state=0, knn_method="brute")
+    Y = um.fit_transform(X, epoch_callback=lambda e, y: calls.append((e, y.shape)))
+    assert Y.shape == (2500, 2) and Y.dtype == np.float32 and np.isfinite(Y).all()
+    assert calls[0][0] == 0 and calls[1][0] == 2 and calls[-1][0] == 50 and len(calls) == 50
+    Y2 = UMAP(n_epochs=50, random_state=0, knn_method="brute", init="random").fit_transform(X)
+    assert om.knn_preservation(X, Y2, 15) > 0.3
