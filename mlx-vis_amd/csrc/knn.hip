@@ -121,16 +121,109 @@ __global__ __launch_bounds__(256) void k_row_sqnorms(const float* __restrict__ X
   if (l == 0) out[r] = acc;
 }
 
+enum { DT_F32 = 0, DT_BF16 = 1, DT_FP8 = 2 };
+__host__ __device__ constexpr int elem_bytes(int dt) { return dt == DT_F32 ? 4 : dt == DT_BF16 ? 2 : 1; }
+// Scale operand of the FP8 MFMA that leaves products unscaled.
+constexpr int FP8_UNIT_SCALE = 127;
+
+__device__ __forceinline__ uint16_t bf16_rne(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// Centred, quantised, zero-padded copy for the BF16 / FP8 screens, one
+// wavefront per row. Per row: sq = |x_hat|^2 of the dequantised vector,
+// err = an upper bound of |x_c - x_hat| (x_c the FP32 centred row), and for FP8
+// inv = 1 / scale with scale = 448 / max|x_c| (per-row absmax). sqmax / errmax
+// receive the bit patterns of the largest sq / err over rows < n.
+template <int DT>
+__global__ __launch_bounds__(256) void k_center_q(const float* __restrict__ X, int64_t n, int64_t d, int64_t d_pad,
+                                                  int64_t n_pad, const float* __restrict__ mu, char* __restrict__ Xq,
+                                                  float* __restrict__ sq, float* __restrict__ err,
+                                                  float* __restrict__ inv, uint32_t* __restrict__ sqmax,
+                                                  uint32_t* __restrict__ errmax) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (r >= n_pad) return;
+  const bool live = r < n;
+  const float* in = X + r * d;
+  float scale = 1.f;
+  if (DT == DT_FP8 && live) {
+    float amax = 0.f;
+    for (int64_t t = l; t < d; t += 64) amax = fmaxf(amax, fabsf(in[t] - mu[t]));
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) amax = fmaxf(amax, __shfl_xor(amax, m));
+    scale = amax > 0.f ? 448.f / amax : 1.f;
+  }
+  const float iv = 1.f / scale;
+  float acc = 0.f, e2 = 0.f, v2 = 0.f;
+  for (int64_t t = l; t < d_pad; t += 64) {
+    const float v = (live && t < d) ? in[t] - mu[t] : 0.f;
+    float xh;
+    if (DT == DT_BF16) {
+      const uint16_t q = bf16_rne(v);
+      reinterpret_cast<uint16_t*>(Xq + r * d_pad * 2)[t] = q;
+      xh = bf16_to_f32(q);
+    } else {
+      const int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v * scale, 0.f, 0, false);
+      reinterpret_cast<uint8_t*>(Xq + r * d_pad)[t] = static_cast<uint8_t>(pk & 0xFF);
+      xh = __builtin_amdgcn_cvt_f32_fp8(pk, 0) * iv;
+    }
+    acc = fmaf(xh, xh, acc);
+    const float dv = v - xh;
+    e2 = fmaf(dv, dv, e2);
+    v2 = fmaf(v, v, v2);
+  }
+  acc = wave_sum_f32(acc);
+  e2 = wave_sum_f32(e2);
+  v2 = wave_sum_f32(v2);
+  if (l == 0) {
+    // sqrt of the FP32 sum with 0.1 % slack, plus the centring rounding bound
+    const float eb = live ? sqrtf(e2) * 1.001f + 1.2e-7f * sqrtf(v2) + 1e-20f : 0.f;
+    sq[r] = acc;
+    err[r] = eb;
+    if (inv) inv[r] = iv;
+    if (live) {
+      atomicMax(sqmax, __float_as_uint(acc));
+      atomicMax(errmax, __float_as_uint(eb));
+    }
+  }
+}
+
+// Gathers the FP32 centred rows listed in rows[0:*count] (original query ids)
+// for the FP32 re-screen, with their norms and self indices.
+__global__ __launch_bounds__(256) void k_gather_rows(const float* __restrict__ Qc, const float* __restrict__ sqq,
+                                                     int64_t d_pad, const int32_t* __restrict__ rows,
+                                                     const int32_t* __restrict__ count,
+                                                     const int32_t* __restrict__ q_self, int64_t self_base,
+                                                     float* __restrict__ out, float* __restrict__ sq_out,
+                                                     int32_t* __restrict__ self_out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (i >= *count) return;
+  const int64_t r = rows[i];
+  for (int64_t t = l; t < d_pad; t += 64) out[i * d_pad + t] = Qc[r * d_pad + t];
+  if (l == 0) {
+    sq_out[i] = sqq[r];
+    self_out[i] = q_self ? q_self[r] : (self_base >= 0 ? static_cast<int32_t>(self_base + r) : -1);
+  }
+}
+
 // ----------------------------------------------------------------- screen --
 
 struct ScreenArgs {
-  const float* Qc;        // [nq_pad, d_pad] centred queries
-  const float* Xc;        // [n_pad, d_pad] centred data
-  const float* sqq;       // [nq_pad]
+  const char* Qq;         // [nq_pad, d_pad] centred (quantised) queries
+  const char* Xq;         // [n_pad, d_pad] centred (quantised) data
+  const float* sqq;       // [nq_pad] norms of the (dequantised) queries
   const float* sqx;       // [n_pad]
+  const float* invq;      // FP8: per-row 1/scale of the queries (else unused)
+  const float* invx;      // FP8: per-row 1/scale of the data
   const int32_t* q_self;  // nullable
   int64_t self_base;      // used when q_self == NULL; < 0: no exclusion
-  int64_t d_pad;
+  int64_t row_bytes;      // bytes per padded row (multiple of 128)
+  const int32_t* nq_dev;  // nullable: device-side query count (re-screen)
   int nq, n, C, S;
   int tiles_per_split, n_col_tiles, n_row_blocks, nq_pad;
   uint64_t* cand;         // [S, nq_pad, C] keys
@@ -148,31 +241,102 @@ __device__ __forceinline__ void list_insert(uint64_t& lk, uint64_t kv, uint64_t&
   tau = readlane_u64(lk, C - 1);
 }
 
-__global__ __launch_bounds__(NT, 1) void k_screen_f32(ScreenArgs a) {
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+
+// One 128-byte K chunk of the staged A (rows) and B (columns) tiles into the
+// 4x4 grid of 16x16 accumulators of this wavefront.
+template <int DT>
+__device__ __forceinline__ void mfma_chunk(const float* As, const float* Bs, int wr, int wc, int l,
+                                           v4f (&acc)[4][4]) {
+  if constexpr (DT == DT_F32) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      v4f af[4], bf[4];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+        af[mi] = *reinterpret_cast<const v4f*>(As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4);
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+        bf[ni] = *reinterpret_cast<const v4f*>(Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 4; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mi][j], bf[ni][j], acc[mi][ni], 0, 0, 0);
+    }
+  } else if constexpr (DT == DT_BF16) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      v8bf af[4], bf[4];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+        af[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(
+                                              As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4));
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+        bf[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(
+                                              Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4));
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+    }
+  } else {
+    v8i af[4], bf[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const float* p = As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + (l >> 4) * 8;
+      const v4f lo = *reinterpret_cast<const v4f*>(p), hi = *reinterpret_cast<const v4f*>(p + 4);
+      af[mi] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+    }
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const float* p = Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + (l >> 4) * 8;
+      const v4f lo = *reinterpret_cast<const v4f*>(p), hi = *reinterpret_cast<const v4f*>(p + 4);
+      bf[ni] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[mi], bf[ni], acc[mi][ni], 0, 0, 0,
+                                                                        FP8_UNIT_SCALE, 0, FP8_UNIT_SCALE);
+  }
+}
+
+template <int DT>
+__global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
   __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
   __shared__ uint64_t s_list[BM * CMAX];
   __shared__ uint64_t s_tau[BM];
   __shared__ float s_sqq[BM];
   __shared__ int s_self[BM];
+  __shared__ float s_inv[BM];
 
   const int bid = blockIdx.x;
   const int xcd = bid & 7;
   const int per = 8 / a.S;
   const int split = xcd % a.S;
   const int rb = (bid >> 3) * per + xcd / a.S;
-  if (rb >= a.n_row_blocks) return;
+  const int nq = a.nq_dev ? *a.nq_dev : a.nq;
+  if (rb >= a.n_row_blocks || static_cast<int64_t>(rb) * BM >= nq) return;
 
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63;
   const int wr = w >> 1, wc = w & 1;
   const int C = a.C;
-  const int64_t d_pad = a.d_pad;
+  const int64_t rbytes = a.row_bytes;
 
   for (int r = tid; r < BM; r += NT) {
     const int64_t grow = static_cast<int64_t>(rb) * BM + r;
     s_sqq[r] = a.sqq[grow];
+    s_inv[r] = DT == DT_FP8 ? a.invq[grow] : 1.f;
     int64_t self = -1;
-    if (grow < a.nq) {
+    if (grow < nq) {
       if (a.q_self) self = a.q_self[grow];
       else if (a.self_base >= 0) self = a.self_base + grow;
     }
@@ -184,14 +348,14 @@ __global__ __launch_bounds__(NT, 1) void k_screen_f32(ScreenArgs a) {
 
   const int t0 = split * a.tiles_per_split;
   const int t1 = min(t0 + a.tiles_per_split, a.n_col_tiles);
-  const float* Ab = a.Qc + static_cast<int64_t>(rb) * BM * d_pad;
+  const char* Ab = a.Qq + static_cast<int64_t>(rb) * BM * rbytes;
   const int lr = tid >> 3;            // staging row (plus 32*i)
-  const int lkk = (tid & 7) * 4;      // staging k offset within the chunk
-  const int nk = static_cast<int>(d_pad / KC);
+  const int lkk = (tid & 7) * 4;      // staging offset within the chunk (floats)
+  const int nk = static_cast<int>(rbytes / 128);
 
   for (int t = t0; t < t1; ++t) {
     const int col0 = t * BN;
-    const float* Bb = a.Xc + static_cast<int64_t>(col0) * d_pad;
+    const char* Bb = a.Xq + static_cast<int64_t>(col0) * rbytes;
 
     v4f acc[4][4];
 #pragma unroll
@@ -202,8 +366,8 @@ __global__ __launch_bounds__(NT, 1) void k_screen_f32(ScreenArgs a) {
     float4 ra[4], rv[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      ra[i] = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32 * i) * d_pad + lkk);
-      rv[i] = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32 * i) * d_pad + lkk);
+      ra[i] = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32 * i) * rbytes + lkk * 4);
+      rv[i] = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32 * i) * rbytes + lkk * 4);
     }
     {
       float* As = s_stage;
@@ -218,37 +382,21 @@ __global__ __launch_bounds__(NT, 1) void k_screen_f32(ScreenArgs a) {
 
     for (int kc = 0; kc < nk; ++kc) {
       const int cur = kc & 1;
-      const bool more = kc + 1 < nk;
-      if (more) {
-        const int64_t ko = static_cast<int64_t>(kc + 1) * KC + lkk;
+      // Prefetch of the next chunk; the last iteration re-reads its own
+      // (in-bounds) chunk so that loads and stores stay unconditional.
+      const int kn = kc + 1 < nk ? kc + 1 : kc;
+      {
+        const int64_t ko = static_cast<int64_t>(kn) * 128 + lkk * 4;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          ra[i] = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32 * i) * d_pad + ko);
-          rv[i] = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32 * i) * d_pad + ko);
+          ra[i] = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32 * i) * rbytes + ko);
+          rv[i] = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32 * i) * rbytes + ko);
         }
       }
       const float* As = s_stage + cur * (2 * BM * LDSA);
       const float* Bs = As + BM * LDSA;
-#pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        v4f af[4], bf[4];
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
-          af[mi] = *reinterpret_cast<const v4f*>(As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + g * 16 +
-                                                 (l >> 4) * 4);
-#pragma unroll
-        for (int ni = 0; ni < 4; ++ni)
-          bf[ni] = *reinterpret_cast<const v4f*>(Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + g * 16 +
-                                                 (l >> 4) * 4);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-          for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-            for (int ni = 0; ni < 4; ++ni)
-              acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mi][j], bf[ni][j], acc[mi][ni], 0, 0, 0);
-      }
-      if (more) {
+      mfma_chunk<DT>(As, Bs, wr, wc, l, acc);
+      {
         float* As2 = s_stage + (cur ^ 1) * (2 * BM * LDSA);
         float* Bs2 = As2 + BM * LDSA;
 #pragma unroll
@@ -262,12 +410,13 @@ __global__ __launch_bounds__(NT, 1) void k_screen_f32(ScreenArgs a) {
 
     // Epilogue: D = |q|^2 + |x|^2 - 2 q.x, clamped at 0; excluded columns -> +inf.
     float* Dt = s_stage;
-    float sqxv[4];
+    float sqxv[4], invc[4];
     int colv[4];
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) {
       colv[ni] = col0 + wc * 64 + ni * 16 + (l & 15);
       sqxv[ni] = a.sqx[colv[ni]];
+      invc[ni] = DT == DT_FP8 ? a.invx[colv[ni]] : 1.f;
     }
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
@@ -276,9 +425,11 @@ __global__ __launch_bounds__(NT, 1) void k_screen_f32(ScreenArgs a) {
         const int row = wr * 64 + mi * 16 + (l >> 4) * 4 + r;
         const float sr = s_sqq[row];
         const int self = s_self[row];
+        const float ir = s_inv[row];
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
-          float v = fmaxf(sr + sqxv[ni] - 2.f * acc[mi][ni][r], 0.f);
+          const float dot = DT == DT_FP8 ? acc[mi][ni][r] * ir * invc[ni] : acc[mi][ni][r];
+          float v = fmaxf(sr + sqxv[ni] - 2.f * dot, 0.f);
           if (colv[ni] >= a.n || colv[ni] == self) v = INFINITY;
           Dt[row * DTS + wc * 64 + ni * 16 + (l & 15)] = v;
         }
@@ -316,7 +467,7 @@ __global__ __launch_bounds__(NT, 1) void k_screen_f32(ScreenArgs a) {
   for (int rr = 0; rr < BM / 4; ++rr) {
     const int row = w * (BM / 4) + rr;
     const int64_t grow = static_cast<int64_t>(rb) * BM + row;
-    if (grow < a.nq && l < C)
+    if (grow < nq && l < C)
       a.cand[(static_cast<int64_t>(split) * a.nq_pad + grow) * C + l] = s_list[row * CMAX + l];
   }
 }
@@ -331,7 +482,12 @@ struct RerankArgs {
   const uint64_t* cand;
   const float* sqq;            // centred query norms (screen space)
   const uint32_t* sqmax_bits;  // largest centred data norm
-  float err_fac;               // screen error bound: err_fac * (sqq[i] + sqmax)
+  float err_fac;               // screen arithmetic bound: err_fac * (sqq[i] + sqmax)
+  const float* eq;             // quantisation error bound per query (nullptr: FP32 screen)
+  const uint32_t* emax_bits;   // largest quantisation error bound of the data
+  const int32_t* nq_dev;       // nullable: device-side row count (re-screen)
+  const int32_t* row_map;      // nullable: original query id of row i (re-screen)
+  int fail_stat;               // stats slot counting rows without a certificate
   int euclid;
   int32_t* out_idx;
   float* out_dist;
@@ -382,7 +538,9 @@ __device__ __forceinline__ void wave_sort_pair64(uint64_t& dbits, uint32_t& idx)
 __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + w;
-  if (i >= a.nq) return;
+  const int nq = a.nq_dev ? *a.nq_dev : a.nq;
+  if (i >= nq) return;
+  const int orow = a.row_map ? a.row_map[i] : i;
   const int C = a.C, k = a.k;
   const int M = a.S * C;
 
@@ -410,7 +568,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   const int cidx = mine != KEY_NONE ? static_cast<int>(key_idx(mine)) : 0;
 
   // 2. FP32 direct-difference distances: 4 candidates at a time, 16 lanes each.
-  const float* qrow = a.Q + static_cast<int64_t>(i) * a.d;
+  const float* qrow = a.Q + static_cast<int64_t>(orow) * a.d;
   const int g = l >> 4, sub = l & 15;
   float d32 = INFINITY;
   for (int r = 0; r * 4 < nvalid; ++r) {
@@ -465,24 +623,32 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
     out_d = a.euclid ? sqrtf(ds) : ds;
   }
 
-  // 5. Certificate: every non-candidate j has screen value >= tau, hence true
-  //    distance >= tau - err_i; the k-th candidate must lie strictly below.
+  // 5. Certificate: every non-candidate j has screen value >= tau. With an FP32
+  //    screen its true squared distance is >= tau - arith_i; with a quantised
+  //    screen |x_i - x_j| >= sqrt(tau - arith_i) - E_i - E_max (E: norms of the
+  //    quantisation errors). The k-th candidate must lie strictly below.
   bool cert = nvalid < C;
   if (!cert) {
     const float tau = key_dist(tau_key);
     const float sqmax = __uint_as_float(*a.sqmax_bits);
-    const float err_i = a.err_fac * (a.sqq[i] + sqmax) + 1e-30f;
-    cert = kth_ub < tau - err_i;
+    const float arith = a.err_fac * (a.sqq[i] + sqmax) + 1e-30f;
+    if (!a.eq) {
+      cert = kth_ub < tau - arith;
+    } else {
+      const double t = fmax(static_cast<double>(tau) - static_cast<double>(arith), 0.0);
+      const double r = sqrt(t) - static_cast<double>(a.eq[i]) - static_cast<double>(__uint_as_float(*a.emax_bits));
+      cert = r > 0.0 && static_cast<double>(kth_ub) < r * r * (1.0 - 1e-6);
+    }
   }
   if (cert) {
     if (l < k) {
-      a.out_idx[static_cast<int64_t>(i) * k + l] = static_cast<int32_t>(out_i);
-      a.out_dist[static_cast<int64_t>(i) * k + l] = out_d;
+      a.out_idx[static_cast<int64_t>(orow) * k + l] = static_cast<int32_t>(out_i);
+      a.out_dist[static_cast<int64_t>(orow) * k + l] = out_d;
     }
   } else if (l == 0) {
     const int p = atomicAdd(a.fb_count, 1);
-    a.fb_rows[p] = i;
-    atomicAdd(&a.stats[2], 1);
+    a.fb_rows[p] = orow;
+    atomicAdd(&a.stats[a.fail_stat], 1);
   }
 }
 
@@ -613,30 +779,31 @@ __global__ __launch_bounds__(256) void k_exact_scan(ScanArgs a) {
 // ------------------------------------------------------------------- host --
 
 struct Layout {
-  int64_t nq_pad, n_pad, d_pad, nchunks;
-  int C;
+  int64_t nq_pad, n_pad, d_pad, nchunks;   // d_pad: FP32 row (multiple of 32)
+  int64_t dq_pad;                          // quantised row elements (128-byte rows)
+  int C, dtype;
   bool same;
   size_t bytes;
-  // offsets are produced by carving in a fixed order (see carve()).
 };
 
 struct Buffers {
   double* part;
   float* mu;
-  float* Xc;
-  float* sqx;
-  float* Qc;
-  float* sqq;
+  float *Xc, *sqx, *Qc, *sqq;                   // FP32 centred copies
+  char *Xq, *Qq;                                // quantised copies (dtype != FP32)
+  float *sqxq, *sqqq, *ex, *eq, *invx, *invq;   // quantised norms, error bounds, 1/scale
   uint64_t* cand;
-  int32_t* ctr;     // [0] fallback count, [1] sqmax bits, [2..5] stats
-  int32_t* fb_rows;
+  int32_t* ctr;   // [0] fb1 count, [1] sqmax32, [2] sqmaxq, [3] emax, [4] fb2 count, [8..11] stats
+  int32_t *fb1, *fb2;
+  float *Qc2, *sqq2;                            // FP32 re-screen rows
+  int32_t* self2;
 };
 
 static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L, size_t* total) {
   Carve c(ws);
-  Buffers b;
+  Buffers b{};
   b.part = c.take<double>(static_cast<size_t>(L.nchunks) * d);
-  b.mu = c.take<float>(L.d_pad);
+  b.mu = c.take<float>(L.d_pad > L.dq_pad ? L.d_pad : L.dq_pad);
   b.Xc = c.take<float>(static_cast<size_t>(L.n_pad) * L.d_pad);
   b.sqx = c.take<float>(L.n_pad);
   if (L.same) {
@@ -646,23 +813,47 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
     b.Qc = c.take<float>(static_cast<size_t>(L.nq_pad) * L.d_pad);
     b.sqq = c.take<float>(L.nq_pad);
   }
+  if (L.dtype != MV_F32) {
+    const size_t rb = static_cast<size_t>(L.dq_pad) * elem_bytes(L.dtype);
+    b.Xq = c.take<char>(static_cast<size_t>(L.n_pad) * rb);
+    b.sqxq = c.take<float>(L.n_pad);
+    b.ex = c.take<float>(L.n_pad);
+    b.invx = c.take<float>(L.n_pad);
+    if (L.same) {
+      b.Qq = b.Xq;
+      b.sqqq = b.sqxq;
+      b.eq = b.ex;
+      b.invq = b.invx;
+    } else {
+      b.Qq = c.take<char>(static_cast<size_t>(L.nq_pad) * rb);
+      b.sqqq = c.take<float>(L.nq_pad);
+      b.eq = c.take<float>(L.nq_pad);
+      b.invq = c.take<float>(L.nq_pad);
+    }
+    b.Qc2 = c.take<float>(static_cast<size_t>(L.nq_pad) * L.d_pad);
+    b.sqq2 = c.take<float>(L.nq_pad);
+    b.self2 = c.take<int32_t>(L.nq_pad);
+  }
   b.cand = c.take<uint64_t>(static_cast<size_t>(MAX_SPLIT) * L.nq_pad * L.C);
-  b.ctr = c.take<int32_t>(8);
-  b.fb_rows = c.take<int32_t>(nq > 0 ? nq : 1);
+  b.ctr = c.take<int32_t>(16);
+  b.fb1 = c.take<int32_t>(nq > 0 ? nq : 1);
+  b.fb2 = c.take<int32_t>(nq > 0 ? nq : 1);
   *total = c.off + 256;
   return b;
 }
 
-static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, bool same) {
+static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, bool same, int dtype) {
   Layout L;
   L.nq_pad = static_cast<int64_t>(align_up(nq > 0 ? nq : 1, BM));
   L.n_pad = static_cast<int64_t>(align_up(n, BN));
   L.d_pad = static_cast<int64_t>(align_up(d, KC));
+  L.dq_pad = dtype == MV_F32 ? L.d_pad : static_cast<int64_t>(align_up(d, 128 / elem_bytes(dtype)));
   L.nchunks = ceil_div(n, ROW_CHUNK);
   int C = rerank * k;
   if (C < k) C = k;
   if (C > CMAX) C = CMAX;
   L.C = C;
+  L.dtype = dtype;
   L.same = same;
   size_t total = 0;
   carve(nullptr, nq, n, d, L, &total);
@@ -723,6 +914,13 @@ static int choose_split(int64_t row_blocks, int64_t col_tiles, int C, int cus) {
   return best;
 }
 
+template <int DT>
+int launch_screen(const ScreenArgs& sa, int64_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_screen<DT>), dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, sa);
+  MV_LAUNCHED("k_screen");
+  return MV_OK;
+}
+
 }  // namespace knn
 }  // namespace mv
 
@@ -738,11 +936,11 @@ int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rera
              (long long)nq, (long long)n, (long long)d);
   MV_REQUIRE(k >= 1 && k <= 32, "mv_knn_ws_bytes: k=%d outside [1, 32]", k);
   MV_REQUIRE(rerank >= 1, "mv_knn_ws_bytes: rerank=%d < 1", rerank);
-  if (dtype != MV_F32) {
+  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8) {
     mv::set_error("mv_knn_ws_bytes: dtype %d not supported", dtype);
     return MV_EUNSUPPORTED;
   }
-  Layout L = make_layout(nq, n, d, k, rerank, same_qx != 0);
+  Layout L = make_layout(nq, n, d, k, rerank, same_qx != 0, dtype);
   *out_bytes = L.bytes;
   return MV_OK;
 }
@@ -771,13 +969,13 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   const bool excl = q_self != nullptr || self_base >= 0;
   MV_REQUIRE(k <= n - (excl ? 1 : 0), "mv_knn_brute: k=%d exceeds the %lld selectable rows", k,
              (long long)(n - (excl ? 1 : 0)));
-  if (dtype != MV_F32) {
+  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8) {
     mv::set_error("mv_knn_brute: dtype %d not supported", dtype);
     return MV_EUNSUPPORTED;
   }
   MV_FAULT_POINT("launch");
   const bool same = (Q == X) && nq == n;
-  Layout L = make_layout(nq, n, d, k, rerank, same);
+  Layout L = make_layout(nq, n, d, k, rerank, same, dtype);
   if (ws_bytes < L.bytes || !ws) {
     mv::set_error("mv_knn_brute: workspace of %zu bytes < required %zu", ws_bytes, L.bytes);
     return MV_EWORKSPACE;
@@ -786,16 +984,20 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   size_t total = 0;
   Buffers B = carve(ws, nq, n, d, L, &total);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int32_t* fb_count = B.ctr;
-  uint32_t* sqmax = reinterpret_cast<uint32_t*>(B.ctr + 1);
-  int32_t* stats = out_stats ? out_stats : B.ctr + 2;
+  int32_t* fb1_count = B.ctr;
+  uint32_t* sqmax32 = reinterpret_cast<uint32_t*>(B.ctr + 1);
+  uint32_t* sqmaxq = reinterpret_cast<uint32_t*>(B.ctr + 2);
+  uint32_t* emax = reinterpret_cast<uint32_t*>(B.ctr + 3);
+  int32_t* fb2_count = B.ctr + 4;
+  int32_t* stats = out_stats ? out_stats : B.ctr + 8;
+  const int64_t self_b = q_self ? -1 : self_base;
 
   if (g_timer.on) {
     if (int rc = g_timer.ensure()) return rc;
     g_timer.valid = false;
   }
   if (int rc = g_timer.mark(0, s)) return rc;
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, B.ctr, 8);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, B.ctr, 16);
   MV_LAUNCHED("k_zero_i32");
   if (out_stats) {
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, out_stats, 4);
@@ -805,29 +1007,40 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
                                             static_cast<unsigned>(L.nchunks)),
                      dim3(256), 0, s, X, n, d, B.part);
   MV_LAUNCHED("k_colsum_partial");
-  hipLaunchKernelGGL(k_colsum_final, dim3(static_cast<unsigned>(ceil_div(L.d_pad, 256))), dim3(256), 0, s,
-                     B.part, L.nchunks, n, d, L.d_pad, B.mu);
+  const int64_t mu_len = L.d_pad > L.dq_pad ? L.d_pad : L.dq_pad;
+  hipLaunchKernelGGL(k_colsum_final, dim3(static_cast<unsigned>(ceil_div(mu_len, 256))), dim3(256), 0, s, B.part,
+                     L.nchunks, n, d, mu_len, B.mu);
   MV_LAUNCHED("k_colsum_final");
-  hipLaunchKernelGGL(k_center_f32, dim3(static_cast<unsigned>(L.n_pad / 4)), dim3(256), 0, s, X, n, d,
-                     L.d_pad, L.n_pad, B.mu, B.Xc, B.sqx, sqmax);
+  hipLaunchKernelGGL(k_center_f32, dim3(static_cast<unsigned>(L.n_pad / 4)), dim3(256), 0, s, X, n, d, L.d_pad,
+                     L.n_pad, B.mu, B.Xc, B.sqx, sqmax32);
   MV_LAUNCHED("k_center_f32");
   if (!same) {
     hipLaunchKernelGGL(k_center_f32, dim3(static_cast<unsigned>(L.nq_pad / 4)), dim3(256), 0, s, Q, nq, d,
                        L.d_pad, L.nq_pad, B.mu, B.Qc, B.sqq, static_cast<uint32_t*>(nullptr));
     MV_LAUNCHED("k_center_f32");
   }
+  if (dtype != MV_F32) {
+    auto kq = dtype == MV_BF16 ? k_center_q<DT_BF16> : k_center_q<DT_FP8>;
+    hipLaunchKernelGGL(kq, dim3(static_cast<unsigned>(L.n_pad / 4)), dim3(256), 0, s, X, n, d, L.dq_pad, L.n_pad,
+                       B.mu, B.Xq, B.sqxq, B.ex, B.invx, sqmaxq, emax);
+    MV_LAUNCHED("k_center_q");
+    if (!same) {
+      uint32_t* dummy = reinterpret_cast<uint32_t*>(B.ctr + 5);
+      hipLaunchKernelGGL(kq, dim3(static_cast<unsigned>(L.nq_pad / 4)), dim3(256), 0, s, Q, nq, d, L.dq_pad,
+                         L.nq_pad, B.mu, B.Qq, B.sqqq, B.eq, B.invq, dummy, dummy);
+      MV_LAUNCHED("k_center_q");
+    }
+  }
 
   const int64_t row_blocks = L.nq_pad / BM;
   const int64_t col_tiles = L.n_pad / BN;
   const int S = choose_split(row_blocks, col_tiles, L.C, cu_count_cached());
+  const int per = 8 / S;
+  const int64_t grid = ceil_div(row_blocks, per) * 8;
   ScreenArgs sa;
-  sa.Qc = B.Qc;
-  sa.Xc = B.Xc;
-  sa.sqq = B.sqq;
-  sa.sqx = B.sqx;
   sa.q_self = q_self;
-  sa.self_base = q_self ? -1 : self_base;
-  sa.d_pad = L.d_pad;
+  sa.self_base = self_b;
+  sa.nq_dev = nullptr;
   sa.nq = static_cast<int>(nq);
   sa.n = static_cast<int>(n);
   sa.C = L.C;
@@ -837,13 +1050,31 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   sa.n_row_blocks = static_cast<int>(row_blocks);
   sa.nq_pad = static_cast<int>(L.nq_pad);
   sa.cand = B.cand;
-  const int per = 8 / S;
-  const int64_t grid = ceil_div(row_blocks, per) * 8;
+  if (dtype == MV_F32) {
+    sa.Qq = reinterpret_cast<const char*>(B.Qc);
+    sa.Xq = reinterpret_cast<const char*>(B.Xc);
+    sa.sqq = B.sqq;
+    sa.sqx = B.sqx;
+    sa.invq = sa.invx = nullptr;
+    sa.row_bytes = L.d_pad * 4;
+  } else {
+    sa.Qq = B.Qq;
+    sa.Xq = B.Xq;
+    sa.sqq = B.sqqq;
+    sa.sqx = B.sqxq;
+    sa.invq = B.invq;
+    sa.invx = B.invx;
+    sa.row_bytes = L.dq_pad * elem_bytes(dtype);
+  }
   if (int rc = g_timer.mark(1, s)) return rc;
-  hipLaunchKernelGGL(k_screen_f32, dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, sa);
-  MV_LAUNCHED("k_screen_f32");
-  if (int rc = g_timer.mark(2, s)) return rc;
+  int rc = dtype == MV_F32    ? launch_screen<DT_F32>(sa, grid, s)
+           : dtype == MV_BF16 ? launch_screen<DT_BF16>(sa, grid, s)
+                              : launch_screen<DT_FP8>(sa, grid, s);
+  if (rc) return rc;
+  if (int rc2 = g_timer.mark(2, s)) return rc2;
 
+  const float fac = static_cast<float>((2.0 * static_cast<double>(sa.row_bytes / elem_bytes(dtype)) + 10.0) *
+                                       5.9604644775390625e-8 * 1.01);
   RerankArgs ra;
   ra.Q = Q;
   ra.X = X;
@@ -855,18 +1086,57 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   ra.S = S;
   ra.nq_pad = static_cast<int>(L.nq_pad);
   ra.cand = B.cand;
-  ra.sqq = B.sqq;
-  ra.sqmax_bits = sqmax;
-  ra.err_fac = static_cast<float>((2.0 * static_cast<double>(L.d_pad) + 10.0) * 5.9604644775390625e-8 * 1.01);
+  ra.sqq = sa.sqq;
+  ra.sqmax_bits = dtype == MV_F32 ? sqmax32 : sqmaxq;
+  ra.err_fac = fac;
+  ra.eq = dtype == MV_F32 ? nullptr : B.eq;
+  ra.emax_bits = emax;
+  ra.nq_dev = nullptr;
+  ra.row_map = nullptr;
+  ra.fail_stat = 2;
   ra.euclid = euclid;
   ra.out_idx = out_idx;
   ra.out_dist = out_dist;
-  ra.fb_count = fb_count;
-  ra.fb_rows = B.fb_rows;
+  ra.fb_count = fb1_count;
+  ra.fb_rows = B.fb1;
   ra.stats = stats;
   hipLaunchKernelGGL(k_rerank, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, ra);
   MV_LAUNCHED("k_rerank");
-  if (int rc = g_timer.mark(3, s)) return rc;
+  if (int rc2 = g_timer.mark(3, s)) return rc2;
+
+  const int32_t* scan_count = fb1_count;
+  const int32_t* scan_rows = B.fb1;
+  if (dtype != MV_F32) {
+    // FP32 re-screen of the rows the quantised certificate could not settle.
+    hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, B.Qc, B.sqq,
+                       L.d_pad, B.fb1, fb1_count, q_self, self_b, B.Qc2, B.sqq2, B.self2);
+    MV_LAUNCHED("k_gather_rows");
+    ScreenArgs sb = sa;
+    sb.Qq = reinterpret_cast<const char*>(B.Qc2);
+    sb.Xq = reinterpret_cast<const char*>(B.Xc);
+    sb.sqq = B.sqq2;
+    sb.sqx = B.sqx;
+    sb.invq = sb.invx = nullptr;
+    sb.row_bytes = L.d_pad * 4;
+    sb.q_self = B.self2;
+    sb.self_base = -1;
+    sb.nq_dev = fb1_count;
+    if (int rc2 = launch_screen<DT_F32>(sb, grid, s)) return rc2;
+    RerankArgs rb2 = ra;
+    rb2.sqq = B.sqq2;
+    rb2.sqmax_bits = sqmax32;
+    rb2.err_fac = static_cast<float>((2.0 * static_cast<double>(L.d_pad) + 10.0) * 5.9604644775390625e-8 * 1.01);
+    rb2.eq = nullptr;
+    rb2.nq_dev = fb1_count;
+    rb2.row_map = B.fb1;
+    rb2.fail_stat = 3;
+    rb2.fb_count = fb2_count;
+    rb2.fb_rows = B.fb2;
+    hipLaunchKernelGGL(k_rerank, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, rb2);
+    MV_LAUNCHED("k_rerank");
+    scan_count = fb2_count;
+    scan_rows = B.fb2;
+  }
 
   ScanArgs xa;
   xa.Q = Q;
@@ -875,9 +1145,9 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   xa.n = static_cast<int>(n);
   xa.k = k;
   xa.q_self = q_self;
-  xa.self_base = q_self ? -1 : self_base;
-  xa.fb_count = fb_count;
-  xa.fb_rows = B.fb_rows;
+  xa.self_base = self_b;
+  xa.fb_count = scan_count;
+  xa.fb_rows = scan_rows;
   xa.euclid = euclid;
   xa.out_idx = out_idx;
   xa.out_dist = out_dist;
@@ -885,7 +1155,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   const int64_t sgrid = nq < 2048 ? nq : 2048;
   hipLaunchKernelGGL(k_exact_scan, dim3(static_cast<unsigned>(sgrid)), dim3(256), 0, s, xa);
   MV_LAUNCHED("k_exact_scan");
-  if (int rc = g_timer.mark(4, s)) return rc;
+  if (int rc2 = g_timer.mark(4, s)) return rc2;
   g_timer.valid = g_timer.on;
   return MV_OK;
 }

@@ -44,8 +44,10 @@ class NNDescent:
         idx = np.where(idx >= rows, idx + 1, idx).astype(np.int32)
         p1 = min(100, d)
         p2 = min(300, d)
-        P1 = (rs.randn(d, p1).astype(np.float32) / np.sqrt(p1)) if p1 < d else None
-        P2 = (rs.randn(d, p2).astype(np.float32) / np.sqrt(p2)) if p2 < d else None
+        # float32 draws divided by a NumPy float64 scalar promote to float64;
+        # the reference's mx.array(...) stores them as float32 (nndescent.py:75, 82)
+        P1 = (rs.randn(d, p1).astype(np.float32) / np.sqrt(p1)).astype(np.float32) if p1 < d else None
+        P2 = (rs.randn(d, p2).astype(np.float32) / np.sqrt(p2)).astype(np.float32) if p2 < d else None
         return idx, P1, P2
 
     def build_device(self, Xd: torch.Tensor):
@@ -57,8 +59,8 @@ class NNDescent:
         idx0, P1, P2 = self._draws(n, d, k)
         dev = Xd.device
         init = torch.from_numpy(idx0).to(dev)
-        P1d = torch.from_numpy(np.ascontiguousarray(P1)).to(dev) if P1 is not None else None
-        P2d = torch.from_numpy(np.ascontiguousarray(P2)).to(dev) if P2 is not None else None
+        P1d = torch.from_numpy(np.ascontiguousarray(P1, dtype=np.float32)).to(dev) if P1 is not None else None
+        P2d = torch.from_numpy(np.ascontiguousarray(P2, dtype=np.float32)).to(dev) if P2 is not None else None
         p1 = 0 if P1 is None else P1.shape[1]
         p2 = 0 if P2 is None else P2.shape[1]
         lib = _capi.lib()

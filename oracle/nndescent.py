@@ -76,12 +76,12 @@ def nndescent(X, k=15, n_iters=20, max_candidates=None, delta=0.015, random_stat
     dists = np.take_along_axis(dists, si, 1)
     p1, p2 = min(100, d), min(300, d)
     if p1 < d:
-        P = rs.randn(d, p1).astype(F32) / np.sqrt(p1)
+        P = (rs.randn(d, p1).astype(F32) / np.sqrt(p1)).astype(F32)   # mx.array stores float32
         X_low = (X @ P).astype(F32)
         sq_low = np.einsum("ij,ij->i", X_low, X_low).astype(F32)
     else:
         X_low, sq_low = X, sq
-    P2 = rs.randn(d, p2).astype(F32) / np.sqrt(p2) if p2 < d else None
+    P2 = (rs.randn(d, p2).astype(F32) / np.sqrt(p2)).astype(F32) if p2 < d else None
     uf = 1.0
     flags = np.ones((n, k), dtype=bool)
     rows_all = np.arange(n)

@@ -87,3 +87,30 @@ def test_c0_shape_bit_exact():
     oi, od = oknn.exact_knn(X, X, 15, self_base=0)
     _check(idx, dist, oi, od)
     print("stats", st)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("n,d,k", [(1000, 128, 15), (777, 3, 5), (300, 784, 15), (2000, 129, 32)])
+def test_precision_screens_give_identical_indices(dtype, n, d, k):
+    print(f"case dtype={dtype} n={n} d={d} k={k}", flush=True)
+    rng = np.random.default_rng(n + d + k)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    idx, dist, st = _run(X, k, dtype=dtype)
+    oi, od = oknn.exact_knn(X, X, k, self_base=0)
+    print("stats", st)
+    _check(idx, dist, oi, od)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+def test_precision_screens_queries_and_duplicates(dtype):
+    rng = np.random.default_rng(8)
+    base = rng.integers(0, 3, size=(150, 6)).astype(np.float32)
+    X = np.concatenate([base, base, base[:40]], axis=0)
+    idx, dist, st = _run(X, 15, dtype=dtype)
+    oi, od = oknn.exact_knn(X, X, 15, self_base=0)
+    _check(idx, dist, oi, od)
+    Xq = rng.standard_normal((900, 40)).astype(np.float32) * 3 + 7
+    Q = rng.standard_normal((257, 40)).astype(np.float32) * 3 + 7
+    idx, dist, _ = _run(Xq, 12, Q, euclid=False, dtype=dtype)
+    oi, od = oknn.exact_knn(Q, Xq, 12)
+    _check(idx, dist, oi, od, euclid=False)
