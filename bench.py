@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--epochs", type=int, default=500)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--knn-reps", type=int, default=3)
+    ap.add_argument("--only", choices=["knn", "copy"], default=None,
+                    help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
+    ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8"])
+    ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
     return ap.parse_args()
 
 
@@ -50,7 +54,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def knn_phase_timing(Xd, reps):
+def knn_phase_timing(Xd, reps, dtype="fp32"):
     """Times mv_knn_brute's phases with HIP events on its stream."""
     import ctypes
 
@@ -64,7 +68,7 @@ def knn_phase_timing(Xd, reps):
     stats = torch.zeros(4, dtype=torch.int32, device=Xd.device)
     try:
         for _ in range(reps):
-            knn_brute_device(Xd, 15, stats=stats)
+            idx, _ = knn_brute_device(Xd, 15, stats=stats, dtype=dtype)
             out = (ctypes.c_float * 4)()
             _capi.check(lib.mv_knn_last_phase_ms(out), "mv_knn_last_phase_ms")
             phases.append(list(out))
@@ -72,7 +76,7 @@ def knn_phase_timing(Xd, reps):
         lib.mv_knn_set_timing(0)
     n = len(phases)
     avg = [sum(p[i] for p in phases) / n for i in range(4)]
-    return avg, stats.cpu().tolist()
+    return avg, stats.cpu().tolist(), idx
 
 
 def knn_preservation_gpu(knn_hi, Y):
@@ -161,11 +165,26 @@ def main():
     from mlx_vis_amd import UMAP
     from mlx_vis_amd.datasets import gaussian_mixture_784
 
+    if args.only == "copy":
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4}), flush=True)
+        return
+
     log("generating the 10-Gaussian 784-D mixture")
     X = gaussian_mixture_784(args.n, seed=1)
     Xd = torch.from_numpy(X).cuda()
     n, d = X.shape
     torch.cuda.synchronize()
+
+    if args.only == "knn":
+        phases, stats, _ = knn_phase_timing(Xd, args.knn_reps, args.dtype)
+        print(json.dumps({"knn_dtype": args.dtype, "ms_phases": phases, "stats": stats,
+                          "tflops_screen": 2.0 * n * n * d / (phases[1] * 1e-3) / 1e12}), flush=True)
+        return
 
     def step():
         um = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=args.epochs, negative_sample_rate=5,
@@ -211,7 +230,7 @@ def main():
     ph["epochs"] = time.perf_counter() - t
 
     log("kNN phase timing")
-    phases, stats = knn_phase_timing(Xd, args.knn_reps)
+    phases, stats, ref_idx = knn_phase_timing(Xd, args.knn_reps)
     screen_ms = phases[1]
     flops = 2.0 * n * n * d
     screen_tflops = flops / (screen_ms * 1e-3) / 1e12
@@ -245,6 +264,32 @@ def main():
                      "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": None},
     }
+    if not args.no_extras:
+        prec = {}
+        for dt in ("bf16", "fp8"):
+            log(f"kNN {dt} screen")
+            ph_dt, st_dt, idx_dt = knn_phase_timing(Xd, args.knn_reps, dt)
+            tot = sum(ph_dt)
+            prec[dt] = {"ms_total": round(tot, 3), "ms_phases": [round(v, 3) for v in ph_dt],
+                        "speedup_vs_fp32": round(knn_total_ms / tot, 3),
+                        "identical_indices": bool(torch.equal(idx_dt, ref_idx)),
+                        "rows_uncertified_first_pass": st_dt[2], "rows_uncertified_after_fp32_rescreen": st_dt[3],
+                        "rows_fp64_scan": st_dt[0]}
+        line["precision"] = prec
+        log("NNDescent k=15, 12 iterations, max_candidates=30")
+        from mlx_vis_amd import NNDescent
+
+        nnd = NNDescent(k=15, n_iters=12, max_candidates=30, random_state=0)
+        nnd.build_device(Xd)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        gi, _ = nnd.build_device(Xd)
+        torch.cuda.synchronize()
+        t_nnd = time.perf_counter() - t
+        hi = ref_idx.long()
+        inter = (gi.long()[:, :, None] == hi[:, None, :]).any(dim=2).sum(dim=1).float()
+        line["nndescent"] = {"s": round(t_nnd, 4), "recall_vs_exact": round(float((inter / 15).mean().item()), 4),
+                             "iterations": nnd.iterations_, "changed": nnd.changed_}
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement, bounded sample)")
         line["cpu_baseline"] = cpu_baseline(X, um.knn_[0].cpu().numpy(), um.knn_[1].cpu().numpy(), args.epochs)
