@@ -317,11 +317,15 @@ __global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
   __shared__ int s_self[BM];
   __shared__ float s_inv[BM];
 
+  // Workgroups are dispatched round-robin over the 8 XCDs (bid mod 8). XCD x
+  // owns the row blocks rb = x (mod 8); consecutive workgroups of one XCD take
+  // the S column splits of one row block, so the ~32 co-resident workgroups
+  // of an XCD share each query slab S ways and each data tile 32/S ways in L2.
   const int bid = blockIdx.x;
   const int xcd = bid & 7;
-  const int per = 8 / a.S;
-  const int split = xcd % a.S;
-  const int rb = (bid >> 3) * per + xcd / a.S;
+  const int local = bid >> 3;
+  const int split = local % a.S;
+  const int rb = (local / a.S) * 8 + xcd;
   const int nq = a.nq_dev ? *a.nq_dev : a.nq;
   if (rb >= a.n_row_blocks || static_cast<int64_t>(rb) * BM >= nq) return;
 
@@ -488,6 +492,7 @@ struct RerankArgs {
   const int32_t* nq_dev;       // nullable: device-side row count (re-screen)
   const int32_t* row_map;      // nullable: original query id of row i (re-screen)
   int fail_stat;               // stats slot counting rows without a certificate
+  int vec4;                    // Q and X rows are 16-byte aligned (d % 4 == 0)
   int euclid;
   int32_t* out_idx;
   float* out_dist;
@@ -577,9 +582,26 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
     float acc = 0.f;
     if (c < nvalid) {
       const float* xr = a.X + static_cast<int64_t>(j) * a.d;
-      for (int64_t t = sub; t < a.d; t += 16) {
-        const float df = qrow[t] - xr[t];
-        acc = fmaf(df, df, acc);
+      if (a.vec4) {
+        // 16 lanes x 16-byte loads cover 64 contiguous floats per step.
+        const float4* q4 = reinterpret_cast<const float4*>(qrow);
+        const float4* x4 = reinterpret_cast<const float4*>(xr);
+        for (int64_t t = sub; t < a.d / 4; t += 16) {
+          const float4 qa = q4[t], xb = x4[t];
+          float df = qa.x - xb.x;
+          acc = fmaf(df, df, acc);
+          df = qa.y - xb.y;
+          acc = fmaf(df, df, acc);
+          df = qa.z - xb.z;
+          acc = fmaf(df, df, acc);
+          df = qa.w - xb.w;
+          acc = fmaf(df, df, acc);
+        }
+      } else {
+        for (int64_t t = sub; t < a.d; t += 16) {
+          const float df = qrow[t] - xr[t];
+          acc = fmaf(df, df, acc);
+        }
       }
     }
     acc += __shfl_xor(acc, 8);
@@ -593,7 +615,10 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   // 3. Sort by (FP32 distance, index); flag pairs FP32 cannot separate.
   uint64_t k32 = l < nvalid ? dkey(d32, static_cast<uint32_t>(cidx)) : KEY_NONE;
   k32 = wave_sort_u64(k32);
-  const float bfac = static_cast<float>(a.d + 8) * 5.9604645e-8f * 1.001f;
+  // Each lane sums ceil(d/16) fmaf terms in order, then 4 butterfly adds;
+  // every term carries the rounding of q - x (relative 2u after squaring).
+  const int64_t terms = a.vec4 ? 4 * ((a.d + 63) / 64) : (a.d + 15) / 16;
+  const float bfac = static_cast<float>(terms + 8) * 5.9604645e-8f * 1.01f;
   const float ds = key_dist(k32);
   const float bd = ds * bfac + 1e-35f;
   const float dn = __shfl_down(ds, 1);
@@ -893,25 +918,15 @@ static int cu_count_cached() {
   return cached[dev];
 }
 
-// Column split count: balances whole waves of workgroups (one per CU) against
-// the per-split merge work.
+// Column split count: the largest S <= 8 with at least two column tiles per
+// split and S * C <= 256 (the re-rank merges S lists of C keys per row).
+// Larger S raises L2 reuse of each query slab (see k_screen).
 static int choose_split(int64_t row_blocks, int64_t col_tiles, int C, int cus) {
-  int best = 1;
-  double best_score = -1.0;
-  for (int S = 1; S <= MAX_SPLIT; S <<= 1) {
-    if (S > col_tiles || S * C > 256) break;
-    const double wgs = static_cast<double>(row_blocks * S);
-    const double waves = ceil(wgs / cus);
-    const double wave_eff = wgs / (waves * cus);
-    const int64_t tps = ceil_div(col_tiles, S);
-    const double tile_eff = static_cast<double>(col_tiles) / static_cast<double>(tps * S);
-    const double score = wave_eff * tile_eff;
-    if (score > best_score + 0.02) {
-      best_score = score;
-      best = S;
-    }
-  }
-  return best;
+  (void)row_blocks;
+  (void)cus;
+  int S = 1;
+  while (S < MAX_SPLIT && 2 * S * 2 <= col_tiles && 2 * S * C <= 256) S *= 2;
+  return S;
 }
 
 template <int DT>
@@ -1035,8 +1050,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   const int64_t row_blocks = L.nq_pad / BM;
   const int64_t col_tiles = L.n_pad / BN;
   const int S = choose_split(row_blocks, col_tiles, L.C, cu_count_cached());
-  const int per = 8 / S;
-  const int64_t grid = ceil_div(row_blocks, per) * 8;
+  const int64_t grid = ceil_div(row_blocks, 8) * 8 * S;
   ScreenArgs sa;
   sa.q_self = q_self;
   sa.self_base = self_b;
@@ -1094,6 +1108,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   ra.nq_dev = nullptr;
   ra.row_map = nullptr;
   ra.fail_stat = 2;
+  ra.vec4 = (d % 4 == 0) && (reinterpret_cast<uintptr_t>(Q) % 16 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
   ra.euclid = euclid;
   ra.out_idx = out_idx;
   ra.out_dist = out_dist;

@@ -204,7 +204,11 @@ __device__ __forceinline__ uint32_t uniform_below(uint32_t r, uint32_t n) {
 // One synchronous epoch over the dense slot array. Lanes are grouped per
 // head row (G lanes, G = next power of two >= k) so the head's attractive and
 // repulsive updates are reduced in registers and written with one atomic per
-// component per row; tails get one atomic per active edge.
+// component per row; tails get one atomic per active edge. An active edge
+// first issues all its gathers (head, tail and up to 8 negatives drawn from
+// Philox-4x32-10 counter (slot, epoch, group of 4)), then computes.
+constexpr int NEG_CHUNK = 8;
+
 template <int D, int G>
 __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ dY,
                                                const int32_t* __restrict__ tail, const float* __restrict__ eps,
@@ -231,12 +235,50 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   }
   if (active) {
     const int64_t j = tail[e];
-    float yi[D], diff[D];
-    float d2 = 0.f;
+    float yi[D], yj[D];
 #pragma unroll
     for (int q = 0; q < D; ++q) {
       yi[q] = Y[i * D + q];
-      diff[q] = yi[q] - Y[j * D + q];
+      yj[q] = Y[j * D + q];
+    }
+    for (int base = 0; base < neg_rate; base += NEG_CHUNK) {
+      const uint4 r0 = Philox::gen(make_uint4(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32),
+                                              static_cast<uint32_t>(epoch), static_cast<uint32_t>(base >> 2)),
+                                   seed_lo, seed_hi);
+      const uint4 r1 = Philox::gen(make_uint4(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32),
+                                              static_cast<uint32_t>(epoch), static_cast<uint32_t>((base >> 2) + 1)),
+                                   seed_lo, seed_hi);
+      const uint32_t words[NEG_CHUNK] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+      float yt[NEG_CHUNK][D];
+#pragma unroll
+      for (int s2 = 0; s2 < NEG_CHUNK; ++s2) {
+        if (base + s2 < neg_rate) {
+          const int64_t t = uniform_below(words[s2], static_cast<uint32_t>(n));
+#pragma unroll
+          for (int q = 0; q < D; ++q) yt[s2][q] = Y[t * D + q];
+        }
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < NEG_CHUNK; ++s2) {
+        if (base + s2 < neg_rate) {
+          float nd[D];
+          float n2 = 0.f;
+#pragma unroll
+          for (int q = 0; q < D; ++q) {
+            nd[q] = yi[q] - yt[s2][q];
+            n2 += nd[q] * nd[q];
+          }
+          const float cr = law.repulse(n2);
+#pragma unroll
+          for (int q = 0; q < D; ++q) acc[q] += clip4(cr * nd[q]) * law.alpha;
+        }
+      }
+    }
+    float diff[D];
+    float d2 = 0.f;
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      diff[q] = yi[q] - yj[q];
       d2 += diff[q] * diff[q];
     }
     const float ca = law.attract(d2);
@@ -245,25 +287,6 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
       const float g = clip4(ca * diff[q]) * law.alpha;
       acc[q] += g;
       atomicAdd(&dY[j * D + q], -g);
-    }
-    uint4 r = make_uint4(0, 0, 0, 0);
-    for (int s2 = 0; s2 < neg_rate; ++s2) {
-      if ((s2 & 3) == 0)
-        r = Philox::gen(make_uint4(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32),
-                                   static_cast<uint32_t>(epoch), static_cast<uint32_t>(s2 >> 2)),
-                        seed_lo, seed_hi);
-      const uint32_t word = (s2 & 3) == 0 ? r.x : (s2 & 3) == 1 ? r.y : (s2 & 3) == 2 ? r.z : r.w;
-      const int64_t t = uniform_below(word, static_cast<uint32_t>(n));
-      float nd[D];
-      float n2 = 0.f;
-#pragma unroll
-      for (int q = 0; q < D; ++q) {
-        nd[q] = yi[q] - Y[t * D + q];
-        n2 += nd[q] * nd[q];
-      }
-      const float cr = law.repulse(n2);
-#pragma unroll
-      for (int q = 0; q < D; ++q) acc[q] += clip4(cr * nd[q]) * law.alpha;
     }
   }
   // Reduce the head's updates over its G lanes.
@@ -371,11 +394,64 @@ __global__ __launch_bounds__(256) void k_spec_wnorm(const float* __restrict__ ws
   wn[e] = eps[e] > 0.f ? wsym[e] * dis[i] * dis[tail[e]] : 0.f;
 }
 
+// R^-1 of the Cholesky factor (G = R^T R, R upper, pivots sqrt(. + 1e-10) as
+// the reference's norms, umap.py:335) of a packed Gram matrix; identity if g
+// is null.
 template <int NC>
-__global__ __launch_bounds__(256) void k_spec_spmv(const float* __restrict__ U, const float* __restrict__ Rinv,
+__device__ void chol_inverse(const double* __restrict__ g, float* __restrict__ out) {
+  double R[NC][NC], Ri[NC][NC];
+  for (int p = 0; p < NC; ++p)
+    for (int q = 0; q < NC; ++q) {
+      R[p][q] = 0.0;
+      Ri[p][q] = p == q ? 1.0 : 0.0;
+    }
+  if (g) {
+    double G[NC][NC];
+    int idx = 0;
+    for (int p = 0; p < NC; ++p)
+      for (int q = p; q < NC; ++q) {
+        G[p][q] = g[idx];
+        G[q][p] = g[idx];
+        ++idx;
+      }
+    for (int j = 0; j < NC; ++j) {
+      double s = G[j][j];
+      for (int i = 0; i < j; ++i) s -= R[i][j] * R[i][j];
+      const double rjj = sqrt(fmax(s, 0.0) + 1e-10);
+      R[j][j] = rjj;
+      for (int q = j + 1; q < NC; ++q) {
+        double v = G[j][q];
+        for (int i = 0; i < j; ++i) v -= R[i][j] * R[i][q];
+        R[j][q] = v / rjj;
+      }
+    }
+    for (int q = 0; q < NC; ++q) {
+      Ri[q][q] = 1.0 / R[q][q];
+      for (int p = q - 1; p >= 0; --p) {
+        double v = 0.0;
+        for (int m = p + 1; m <= q; ++m) v += R[p][m] * Ri[m][q];
+        Ri[p][q] = -v / R[p][p];
+      }
+    }
+  }
+  for (int p = 0; p < NC; ++p)
+    for (int q = 0; q < NC; ++q) out[p * NC + q] = static_cast<float>(Ri[p][q]);
+}
+
+// One power-iteration step: Unext = (W_norm U) R^-1(gram_prev), accumulating
+// the Gram matrix of Unext into gram_cur (one atomic per block and entry) and
+// clearing gram_zero for the next step.
+template <int NC>
+__global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, const double* __restrict__ gram_prev,
                                                    const float* __restrict__ wn, const int32_t* __restrict__ tail,
                                                    int64_t n, int k, float* __restrict__ Unext,
-                                                   double* __restrict__ gram) {
+                                                   double* __restrict__ gram_cur, double* __restrict__ gram_zero) {
+  constexpr int NP = NC * (NC + 1) / 2;
+  __shared__ float s_R[NC * NC];
+  __shared__ double s_part[4][NP];
+  if (threadIdx.x == 0) chol_inverse<NC>(gram_prev, s_R);
+  if (blockIdx.x == 0 && threadIdx.x < NP) gram_zero[threadIdx.x] = 0.0;
+  __syncthreads();
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   float t[NC];
 #pragma unroll
@@ -396,13 +472,14 @@ __global__ __launch_bounds__(256) void k_spec_spmv(const float* __restrict__ U, 
   for (int q = 0; q < NC; ++q) {
     float acc = 0.f;
 #pragma unroll
-    for (int p = 0; p <= q; ++p) acc += t[p] * Rinv[p * NC + q];   // Rinv is upper triangular
+    for (int p = 0; p <= q; ++p) acc += t[p] * s_R[p * NC + q];
     u[q] = acc;
   }
   if (i < n) {
 #pragma unroll
     for (int q = 0; q < NC; ++q) Unext[i * NC + q] = u[q];
   }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   int idx = 0;
 #pragma unroll
   for (int p = 0; p < NC; ++p) {
@@ -411,76 +488,34 @@ __global__ __launch_bounds__(256) void k_spec_spmv(const float* __restrict__ U, 
       double v = static_cast<double>(u[p]) * static_cast<double>(u[q]);
 #pragma unroll
       for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-      if ((threadIdx.x & 63) == 0) atomicAdd(&gram[idx], v);
+      if (l == 0) s_part[w][idx] = v;
       ++idx;
     }
   }
+  __syncthreads();
+  if (threadIdx.x < NP) {
+    const double v = s_part[0][threadIdx.x] + s_part[1][threadIdx.x] + s_part[2][threadIdx.x] +
+                     s_part[3][threadIdx.x];
+    atomicAdd(&gram_cur[threadIdx.x], v);
+  }
 }
 
-// Cholesky of the packed Gram matrix (G = R^T R, R upper, pivots
-// sqrt(. + 1e-10) as the reference's norms, umap.py:335) and R^-1. Clears gram.
+// V = U R^-1(gram).
 template <int NC>
-__global__ void k_spec_chol(double* __restrict__ gram, float* __restrict__ Rinv, float* __restrict__ Rinv_prev) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double G[NC][NC];
-  int idx = 0;
-  for (int p = 0; p < NC; ++p)
-    for (int q = p; q < NC; ++q) {
-      G[p][q] = gram[idx];
-      G[q][p] = gram[idx];
-      gram[idx] = 0.0;
-      ++idx;
-    }
-  double R[NC][NC];
-  for (int p = 0; p < NC; ++p)
-    for (int q = 0; q < NC; ++q) R[p][q] = 0.0;
-  for (int j = 0; j < NC; ++j) {
-    double s = G[j][j];
-    for (int i = 0; i < j; ++i) s -= R[i][j] * R[i][j];
-    const double rjj = sqrt(fmax(s, 0.0) + 1e-10);
-    R[j][j] = rjj;
-    for (int q = j + 1; q < NC; ++q) {
-      double v = G[j][q];
-      for (int i = 0; i < j; ++i) v -= R[i][j] * R[i][q];
-      R[j][q] = v / rjj;
-    }
-  }
-  // Inverse of an upper-triangular matrix by back substitution.
-  double Ri[NC][NC];
-  for (int p = 0; p < NC; ++p)
-    for (int q = 0; q < NC; ++q) Ri[p][q] = 0.0;
-  for (int q = 0; q < NC; ++q) {
-    Ri[q][q] = 1.0 / R[q][q];
-    for (int p = q - 1; p >= 0; --p) {
-      double v = 0.0;
-      for (int m = p + 1; m <= q; ++m) v += R[p][m] * Ri[m][q];
-      Ri[p][q] = -v / R[p][p];
-    }
-  }
-  for (int p = 0; p < NC; ++p)
-    for (int q = 0; q < NC; ++q) {
-      if (Rinv_prev) Rinv_prev[p * NC + q] = Rinv[p * NC + q];
-      Rinv[p * NC + q] = static_cast<float>(Ri[p][q]);
-    }
-}
-
-template <int NC>
-__global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U, const float* __restrict__ Rinv,
+__global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U, const double* __restrict__ gram,
                                                     int64_t n, float* __restrict__ V) {
+  __shared__ float s_R[NC * NC];
+  if (threadIdx.x == 0) chol_inverse<NC>(gram, s_R);
+  __syncthreads();
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     float acc = 0.f;
 #pragma unroll
-    for (int p = 0; p <= q; ++p) acc += U[i * NC + p] * Rinv[p * NC + q];
+    for (int p = 0; p <= q; ++p) acc += U[i * NC + p] * s_R[p * NC + q];
     V[i * NC + q] = acc;
   }
-}
-
-__global__ void k_eye_f32(float* __restrict__ M, int nc) {
-  const int t = threadIdx.x;
-  if (t < nc * nc) M[t] = (t / nc == t % nc) ? 1.f : 0.f;
 }
 
 template <int NC>
@@ -491,42 +526,39 @@ static int spectral_run(const float* wsym, const float* eps, const int32_t* tail
   float* wn = cv.take<float>(static_cast<size_t>(n) * k);
   float* U0 = cv.take<float>(static_cast<size_t>(n) * NC);
   float* U1 = cv.take<float>(static_cast<size_t>(n) * NC);
-  float* Rinv = cv.take<float>(NC * NC);
-  double* gram = cv.take<double>(NC * NC);
+  double* gram = cv.take<double>(3 * 16);   // three rotating packed Gram buffers
   const unsigned rb = static_cast<unsigned>(ceil_div(n, 256));
   hipLaunchKernelGGL(k_spec_prep, dim3(rb), dim3(256), 0, s, wsym, eps, n, k, dis);
   MV_LAUNCHED("k_spec_prep");
   hipLaunchKernelGGL(k_spec_wnorm, dim3(static_cast<unsigned>(ceil_div(n * k, 256))), dim3(256), 0, s, wsym, eps,
                      tail, dis, n, k, wn);
   MV_LAUNCHED("k_spec_wnorm");
-  hipLaunchKernelGGL(k_eye_f32, dim3(1), dim3(64), 0, s, Rinv, NC);
-  MV_LAUNCHED("k_eye_f32");
-  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s, reinterpret_cast<float*>(gram),
-                     static_cast<int64_t>(2 * NC * NC));
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(128), 0, s, reinterpret_cast<float*>(gram),
+                     static_cast<int64_t>(2 * 3 * 16));
   MV_LAUNCHED("k_zero_f32");
   hipLaunchKernelGGL(k_copy_f32, dim3(static_cast<unsigned>(ceil_div(n * NC, 256))), dim3(256), 0, s, U0, V,
                      n * NC);
   MV_LAUNCHED("k_copy_f32");
+  if (iters == 0) return MV_OK;
   float* cur = U0;
   float* nxt = U1;
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL((k_spec_spmv<NC>), dim3(rb), dim3(256), 0, s, cur, Rinv, wn, tail, n, k, nxt, gram);
-    MV_LAUNCHED("k_spec_spmv");
-    hipLaunchKernelGGL((k_spec_chol<NC>), dim3(1), dim3(64), 0, s, gram, Rinv, static_cast<float*>(nullptr));
-    MV_LAUNCHED("k_spec_chol");
+    const double* prev = it == 0 ? nullptr : gram + 16 * ((it + 2) % 3);
+    hipLaunchKernelGGL((k_spec_iter<NC>), dim3(rb), dim3(256), 0, s, cur, prev, wn, tail, n, k, nxt,
+                       gram + 16 * (it % 3), gram + 16 * ((it + 1) % 3));
+    MV_LAUNCHED("k_spec_iter");
     float* t = cur;
     cur = nxt;
     nxt = t;
   }
-  hipLaunchKernelGGL((k_spec_apply<NC>), dim3(rb), dim3(256), 0, s, cur, Rinv, n, V);
+  hipLaunchKernelGGL((k_spec_apply<NC>), dim3(rb), dim3(256), 0, s, cur, gram + 16 * ((iters - 1) % 3), n, V);
   MV_LAUNCHED("k_spec_apply");
   return MV_OK;
 }
 
 static size_t spectral_ws(int64_t n, int k, int nc) {
   return align_up(n * 4, 256) + align_up(static_cast<size_t>(n) * k * 4, 256) +
-         2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + 2 * align_up(static_cast<size_t>(nc) * nc * 8, 256) +
-         1024;
+         2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + align_up(3 * 16 * 8, 256) + 1024;
 }
 
 template <int D, int G>
