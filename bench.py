@@ -54,7 +54,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def knn_phase_timing(Xd, reps, dtype="fp32"):
+def knn_phase_timing(Xd, reps, dtype="fp32", rerank=2):
     """Times mv_knn_brute's phases with HIP events on its stream."""
     import ctypes
 
@@ -68,7 +68,7 @@ def knn_phase_timing(Xd, reps, dtype="fp32"):
     stats = torch.zeros(4, dtype=torch.int32, device=Xd.device)
     try:
         for _ in range(reps):
-            idx, _ = knn_brute_device(Xd, 15, stats=stats, dtype=dtype)
+            idx, _ = knn_brute_device(Xd, 15, stats=stats, dtype=dtype, rerank=rerank)
             out = (ctypes.c_float * 4)()
             _capi.check(lib.mv_knn_last_phase_ms(out), "mv_knn_last_phase_ms")
             phases.append(list(out))
@@ -225,7 +225,7 @@ def main():
     ph["init"] = time.perf_counter() - t
     a, b = find_ab_params(1.0, 0.1)
     t = time.perf_counter()
-    run_epochs(Y0, torch.zeros_like(Y0), g, 1, args.epochs, args.epochs, 5, 1.0, a, b, 1)
+    run_epochs(Y0, None, g, 1, args.epochs, args.epochs, 5, 1.0, a, b, 1)
     torch.cuda.synchronize()
     ph["epochs"] = time.perf_counter() - t
 
@@ -266,11 +266,11 @@ def main():
     }
     if not args.no_extras:
         prec = {}
-        for dt in ("bf16", "fp8"):
-            log(f"kNN {dt} screen")
-            ph_dt, st_dt, idx_dt = knn_phase_timing(Xd, args.knn_reps, dt)
+        for dt, rr in (("bf16", 2), ("bf16", 4), ("fp8", 2), ("fp8", 4)):
+            log(f"kNN {dt} screen, rerank {rr}")
+            ph_dt, st_dt, idx_dt = knn_phase_timing(Xd, args.knn_reps, dt, rr)
             tot = sum(ph_dt)
-            prec[dt] = {"ms_total": round(tot, 3), "ms_phases": [round(v, 3) for v in ph_dt],
+            prec[f"{dt}_c{rr * 15}"] = {"ms_total": round(tot, 3), "ms_phases": [round(v, 3) for v in ph_dt],
                         "speedup_vs_fp32": round(knn_total_ms / tot, 3),
                         "identical_indices": bool(torch.equal(idx_dt, ref_idx)),
                         "rows_uncertified_first_pass": st_dt[2], "rows_uncertified_after_fp32_rescreen": st_dt[3],

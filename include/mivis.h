@@ -145,19 +145,29 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
                         float* w_sym, float* eps, float* epn, uint32_t* counters, float* sigma_out,
                         float* rho_out, void* ws, size_t ws_bytes, void* stream);
 
+/* In-edge lists of the kept slots (eps > 0), grouped by tail node: entries
+ * t < (number of kept slots) hold the slot in_slots[t] and its tail
+ * in_node[t]; nodes appear in ascending order, slots of one node in no fixed
+ * order. Used by mv_umap_epochs to sum tail updates without per-edge atomics.
+ * ws: 3 * align256(4n) + align256(4 * ceil(n / 1024)) bytes. */
+int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, int32_t* in_slots,
+                     int32_t* in_node, void* ws, size_t ws_bytes, void* stream);
+
 /* Layout epochs [e0, e1) — replaces the epoch loop of UMAP._optimize
  * (umap.py:423-447) with UMAP._sgd_step (umap.py:359-387). Y (n, dim) is
- * updated in place; dY (n, dim) is a zero-filled delta buffer (zero again on
- * return). Each epoch e is synchronous (all gradients read Y as of the start
- * of e): edges with eps > 0 and epn <= e are active (then epn += eps); an
- * active edge moves head and tail with the clipped (+-4) attractive gradient
- * and the head with neg_rate clipped repulsive gradients against nodes drawn
- * uniformly from [0, n) by Philox-4x32-10 keyed by seed; the step size is
- * lr * (1 - e / n_epochs). tail = the kNN indices (n, k). dim <= 4, k <= 64.
- */
-int mv_umap_epochs(float* Y, float* dY, int64_t n, int dim, const int32_t* tail, int k, const float* eps,
-                   float* epn, int e0, int e1, int n_epochs, int neg_rate, float lr, float a, float b,
-                   uint64_t seed, void* stream);
+ * updated in place. Each epoch e is synchronous (all gradients read Y as of
+ * the start of e): slots with eps > 0 and epn <= e are active (then
+ * epn += eps); an active edge moves head and tail with the clipped (+-4)
+ * attractive gradient and the head with neg_rate clipped repulsive gradients
+ * against nodes drawn uniformly from [0, n) by Philox-4x32-10 keyed by seed;
+ * the step size is lr * (1 - e / n_epochs). tail = the kNN indices (n, k);
+ * (in_slots, in_node, n_in) from mv_umap_in_edges. ws sized by
+ * mv_umap_epochs_ws_bytes. dim <= 4, k <= 64. */
+int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes);
+int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
+                   const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1, int n_epochs,
+                   int neg_rate, float lr, float a, float b, uint64_t seed, void* ws, size_t ws_bytes,
+                   void* stream);
 
 /* One synchronous step with explicit edge and negative lists, exactly
  * UMAP._sgd_step (umap.py:359-387): Y_out = Y_in + updates, where every update

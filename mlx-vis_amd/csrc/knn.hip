@@ -317,15 +317,14 @@ __global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
   __shared__ int s_self[BM];
   __shared__ float s_inv[BM];
 
-  // Workgroups are dispatched round-robin over the 8 XCDs (bid mod 8). XCD x
-  // owns the row blocks rb = x (mod 8); consecutive workgroups of one XCD take
-  // the S column splits of one row block, so the ~32 co-resident workgroups
-  // of an XCD share each query slab S ways and each data tile 32/S ways in L2.
+  // Workgroups are dispatched round-robin over the 8 XCDs (bid mod 8); XCD x
+  // works on column split x mod S, so the co-resident workgroups of an XCD
+  // stream the same data tiles through its L2.
   const int bid = blockIdx.x;
   const int xcd = bid & 7;
-  const int local = bid >> 3;
-  const int split = local % a.S;
-  const int rb = (local / a.S) * 8 + xcd;
+  const int per = 8 / a.S;
+  const int split = xcd % a.S;
+  const int rb = (bid >> 3) * per + xcd / a.S;
   const int nq = a.nq_dev ? *a.nq_dev : a.nq;
   if (rb >= a.n_row_blocks || static_cast<int64_t>(rb) * BM >= nq) return;
 
@@ -367,50 +366,55 @@ __global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    float4 ra[4], rv[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      ra[i] = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32 * i) * rbytes + lkk * 4);
-      rv[i] = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32 * i) * rbytes + lkk * 4);
-    }
-    {
-      float* As = s_stage;
-      float* Bs = s_stage + BM * LDSA;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        *reinterpret_cast<float4*>(As + (lr + 32 * i) * LDSA + lkk) = ra[i];
-        *reinterpret_cast<float4*>(Bs + (lr + 32 * i) * LDSA + lkk) = rv[i];
-      }
-    }
+    // Two-deep register prefetch: while chunk kc is multiplied out of one LDS
+    // buffer, chunk kc + 1 waits in registers for the other buffer and chunk
+    // kc + 2 is in flight from memory. Chunk indices past the end are clamped
+    // (re-read, never used), keeping every load and store unconditional.
+    float4 a0_0, a0_1, a0_2, a0_3, b0_0, b0_1, b0_2, b0_3;   // register set 0
+    float4 a1_0, a1_1, a1_2, a1_3, b1_0, b1_1, b1_2, b1_3;   // register set 1
+#define MV_GLOAD(S, KCC)                                                                   \
+  do {                                                                                   \
+    const int kq_ = (KCC) < nk ? (KCC) : nk - 1;                                         \
+    const int64_t ko_ = static_cast<int64_t>(kq_) * 128 + lkk * 4;                       \
+    a##S##_0 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr) * rbytes + ko_);       \
+    a##S##_1 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32) * rbytes + ko_);  \
+    a##S##_2 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 64) * rbytes + ko_);  \
+    a##S##_3 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 96) * rbytes + ko_);  \
+    b##S##_0 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr) * rbytes + ko_);       \
+    b##S##_1 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32) * rbytes + ko_);  \
+    b##S##_2 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 64) * rbytes + ko_);  \
+    b##S##_3 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 96) * rbytes + ko_);  \
+  } while (0)
+#define MV_LSTORE(S, BUF)                                                                  \
+  do {                                                                                   \
+    float* As2_ = s_stage + (BUF) * (2 * BM * LDSA);                                     \
+    float* Bs2_ = As2_ + BM * LDSA;                                                      \
+    *reinterpret_cast<float4*>(As2_ + (lr) * LDSA + lkk) = a##S##_0;                     \
+    *reinterpret_cast<float4*>(As2_ + (lr + 32) * LDSA + lkk) = a##S##_1;                \
+    *reinterpret_cast<float4*>(As2_ + (lr + 64) * LDSA + lkk) = a##S##_2;                \
+    *reinterpret_cast<float4*>(As2_ + (lr + 96) * LDSA + lkk) = a##S##_3;                \
+    *reinterpret_cast<float4*>(Bs2_ + (lr) * LDSA + lkk) = b##S##_0;                     \
+    *reinterpret_cast<float4*>(Bs2_ + (lr + 32) * LDSA + lkk) = b##S##_1;                \
+    *reinterpret_cast<float4*>(Bs2_ + (lr + 64) * LDSA + lkk) = b##S##_2;                \
+    *reinterpret_cast<float4*>(Bs2_ + (lr + 96) * LDSA + lkk) = b##S##_3;                \
+  } while (0)
+    MV_GLOAD(0, 0);
+    MV_GLOAD(1, 1);
+    MV_LSTORE(0, 0);
     __syncthreads();
-
-    for (int kc = 0; kc < nk; ++kc) {
-      const int cur = kc & 1;
-      // Prefetch of the next chunk; the last iteration re-reads its own
-      // (in-bounds) chunk so that loads and stores stay unconditional.
-      const int kn = kc + 1 < nk ? kc + 1 : kc;
-      {
-        const int64_t ko = static_cast<int64_t>(kn) * 128 + lkk * 4;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          ra[i] = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32 * i) * rbytes + ko);
-          rv[i] = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32 * i) * rbytes + ko);
-        }
-      }
-      const float* As = s_stage + cur * (2 * BM * LDSA);
-      const float* Bs = As + BM * LDSA;
-      mfma_chunk<DT>(As, Bs, wr, wc, l, acc);
-      {
-        float* As2 = s_stage + (cur ^ 1) * (2 * BM * LDSA);
-        float* Bs2 = As2 + BM * LDSA;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          *reinterpret_cast<float4*>(As2 + (lr + 32 * i) * LDSA + lkk) = ra[i];
-          *reinterpret_cast<float4*>(Bs2 + (lr + 32 * i) * LDSA + lkk) = rv[i];
-        }
-      }
+    for (int kc = 0; kc < nk; kc += 2) {
+      MV_GLOAD(0, kc + 2);
+      mfma_chunk<DT>(s_stage, s_stage + BM * LDSA, wr, wc, l, acc);
+      MV_LSTORE(1, 1);
+      __syncthreads();
+      if (kc + 1 >= nk) break;
+      MV_GLOAD(1, kc + 3);
+      mfma_chunk<DT>(s_stage + 2 * BM * LDSA, s_stage + 3 * BM * LDSA, wr, wc, l, acc);
+      MV_LSTORE(0, 0);
       __syncthreads();
     }
+#undef MV_GLOAD
+#undef MV_LSTORE
 
     // Epilogue: D = |q|^2 + |x|^2 - 2 q.x, clamped at 0; excluded columns -> +inf.
     float* Dt = s_stage;
@@ -918,15 +922,25 @@ static int cu_count_cached() {
   return cached[dev];
 }
 
-// Column split count: the largest S <= 8 with at least two column tiles per
-// split and S * C <= 256 (the re-rank merges S lists of C keys per row).
-// Larger S raises L2 reuse of each query slab (see k_screen).
+// Column split count: balances whole waves of workgroups (one per CU) against
+// the per-split merge work (S * C <= 256 keys per row in the re-rank).
 static int choose_split(int64_t row_blocks, int64_t col_tiles, int C, int cus) {
-  (void)row_blocks;
-  (void)cus;
-  int S = 1;
-  while (S < MAX_SPLIT && 2 * S * 2 <= col_tiles && 2 * S * C <= 256) S *= 2;
-  return S;
+  int best = 1;
+  double best_score = -1.0;
+  for (int S = 1; S <= MAX_SPLIT; S <<= 1) {
+    if (S > col_tiles || S * C > 256) break;
+    const double wgs = static_cast<double>(row_blocks * S);
+    const double waves = ceil(wgs / cus);
+    const double wave_eff = wgs / (waves * cus);
+    const int64_t tps = ceil_div(col_tiles, S);
+    const double tile_eff = static_cast<double>(col_tiles) / static_cast<double>(tps * S);
+    const double score = wave_eff * tile_eff;
+    if (score > best_score + 0.02) {
+      best_score = score;
+      best = S;
+    }
+  }
+  return best;
 }
 
 template <int DT>
@@ -1050,7 +1064,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   const int64_t row_blocks = L.nq_pad / BM;
   const int64_t col_tiles = L.n_pad / BN;
   const int S = choose_split(row_blocks, col_tiles, L.C, cu_count_cached());
-  const int64_t grid = ceil_div(row_blocks, 8) * 8 * S;
+  const int64_t grid = ceil_div(row_blocks, 8 / S) * 8;
   ScreenArgs sa;
   sa.q_self = q_self;
   sa.self_base = self_b;

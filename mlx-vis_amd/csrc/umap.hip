@@ -154,18 +154,23 @@ __global__ __launch_bounds__(256) void k_fuzzy_sched(const float* __restrict__ w
 
 // -------------------------------------------------------------- gradients --
 
+// x^y for x >= 1e-6 as exp2(y * log2(x)) with the hardware v_log_f32 /
+// v_exp_f32 (about 1 ulp each); relative error ~|y log2 x| * 2^-23, below
+// 1e-6 over the clamped range used here.
+__device__ __forceinline__ float pow_pos(float x, float y) { return exp2f(y * __log2f(x)); }
+
 struct Law {
   float a, b, alpha;
   // Attractive coefficient of umap.py:367-370 (FP32, same operation order).
   __device__ __forceinline__ float attract(float d2) const {
     d2 = fmaxf(d2, 1e-6f);
-    const float pw = powf(d2, b);
-    return -2.0f * a * b * powf(d2, b - 1.0f) / (1.0f + a * pw);
+    const float pw = pow_pos(d2, b);
+    return -2.0f * a * b * pow_pos(d2, b - 1.0f) / (1.0f + a * pw);
   }
   // Repulsive coefficient of umap.py:377-380.
   __device__ __forceinline__ float repulse(float d2) const {
     d2 = fmaxf(d2, 1e-6f);
-    const float pw = powf(d2, b);
+    const float pw = pow_pos(d2, b);
     return 2.0f * b / ((0.001f + d2) * (1.0f + a * pw));
   }
 };
@@ -201,27 +206,33 @@ __device__ __forceinline__ uint32_t uniform_below(uint32_t r, uint32_t n) {
   return static_cast<uint32_t>((static_cast<uint64_t>(r) * n) >> 32);
 }
 
-// One synchronous epoch over the dense slot array. Lanes are grouped per
-// head row (G lanes, G = next power of two >= k) so the head's attractive and
-// repulsive updates are reduced in registers and written with one atomic per
-// component per row; tails get one atomic per active edge. An active edge
-// first issues all its gathers (head, tail and up to 8 negatives drawn from
-// Philox-4x32-10 counter (slot, epoch, group of 4)), then computes.
+// One synchronous epoch, phase 1, over the dense slot array. Lanes are
+// grouped per head row (G lanes, G = next power of two >= k). An active edge
+// issues all its gathers (head, tail and up to 8 negatives drawn from
+// Philox-4x32-10 counter (slot, epoch, group of 4)) before computing. The
+// head's attractive and repulsive updates are reduced over the row's lanes and
+// stored to dHead[i]; the tail's update (minus the attractive gradient) is
+// stored per slot to gtail[e] (zero for inactive slots). No atomics: a point
+// that is the tail of many edges (a hub of the kNN graph) is summed in phase 2.
 constexpr int NEG_CHUNK = 8;
 
 template <int D, int G>
-__global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ dY,
-                                               const int32_t* __restrict__ tail, const float* __restrict__ eps,
-                                               float* __restrict__ epn, int64_t n, int k, int epoch,
-                                               int neg_rate, Law law, uint32_t seed_lo, uint32_t seed_hi) {
+__global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ gtail,
+                                               float* __restrict__ dHead, const int32_t* __restrict__ tail,
+                                               const float* __restrict__ eps, float* __restrict__ epn, int64_t n,
+                                               int k, int epoch, int neg_rate, Law law, uint32_t seed_lo,
+                                               uint32_t seed_hi) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;      // head row
   const int c = static_cast<int>(gid % G);
   const bool in_row = i < n && c < k;
   const int64_t e = i * k + c;
-  float acc[D];
+  float acc[D], gt[D];
 #pragma unroll
-  for (int q = 0; q < D; ++q) acc[q] = 0.f;
+  for (int q = 0; q < D; ++q) {
+    acc[q] = 0.f;
+    gt[q] = 0.f;
+  }
   bool active = false;
   if (in_row) {
     const float s = eps[e];
@@ -286,25 +297,152 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     for (int q = 0; q < D; ++q) {
       const float g = clip4(ca * diff[q]) * law.alpha;
       acc[q] += g;
-      atomicAdd(&dY[j * D + q], -g);
+      gt[q] = -g;
     }
   }
-  // Reduce the head's updates over its G lanes.
+  if (in_row) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) gtail[e * D + q] = gt[q];
+  }
+  // Reduce the head's updates over its G lanes (contiguous in the wavefront).
 #pragma unroll
   for (int q = 0; q < D; ++q) {
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) acc[q] += __shfl_xor(acc[q], m);
   }
-  const uint64_t bal = ballot(active);
   if (c == 0 && i < n) {
-    // a row's G lanes are contiguous within the wavefront
-    const int base = (threadIdx.x & 63) & ~(G - 1);
-    const uint64_t mask = (G == 64) ? ~0ull : (((1ull << G) - 1) << base);
-    if (bal & mask) {
 #pragma unroll
-      for (int q = 0; q < D; ++q) atomicAdd(&dY[i * D + q], acc[q]);
+    for (int q = 0; q < D; ++q) dHead[i * D + q] = acc[q];
+  }
+}
+
+// Phase 2: tail sums over the in-edge list (slots grouped by tail node,
+// in_node[t] the node of entry t). A wavefront-segmented inclusive scan adds
+// each node's consecutive entries; the last lane of a segment adds the
+// segment's total to dTail with one atomic, so a hub with m in-edges costs
+// about m / 64 atomics per epoch instead of m.
+template <int D>
+__global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtail,
+                                                  const int32_t* __restrict__ in_slots,
+                                                  const int32_t* __restrict__ in_node, int64_t m,
+                                                  float* __restrict__ dTail) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int l = threadIdx.x & 63;
+  const bool valid = t < m;
+  const int node = valid ? in_node[t] : -1;
+  float v[D];
+  if (valid) {
+    const int64_t e = in_slots[t];
+#pragma unroll
+    for (int q = 0; q < D; ++q) v[q] = gtail[e * D + q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < D; ++q) v[q] = 0.f;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int on = __shfl_up(node, off);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const float ov = __shfl_up(v[q], off);
+      if (l >= off && on == node) v[q] += ov;
     }
   }
+  const int next = __shfl_down(node, 1);
+  const bool last = valid && (l == 63 || next != node);
+  if (last) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) atomicAdd(&dTail[static_cast<int64_t>(node) * D + q], v[q]);
+  }
+}
+
+// Phase 3: Y += dHead + dTail; dTail = 0.
+__global__ __launch_bounds__(256) void k_apply2(float* __restrict__ Y, const float* __restrict__ dHead,
+                                                float* __restrict__ dTail, int64_t count) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) {
+    Y[t] += dHead[t] + dTail[t];
+    dTail[t] = 0.f;
+  }
+}
+
+// In-edge lists of the kept slots: counts per tail, exclusive scan, atomic
+// fill. Order within a node's list is not fixed (summation order only).
+__global__ __launch_bounds__(256) void k_in_count(const int32_t* __restrict__ tail, const float* __restrict__ eps,
+                                                  int64_t m, uint32_t* __restrict__ cnt) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < m && eps[e] > 0.f) atomicAdd(&cnt[tail[e]], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_in_fill(const int32_t* __restrict__ tail, const float* __restrict__ eps,
+                                                 int64_t m, const uint32_t* __restrict__ off,
+                                                 uint32_t* __restrict__ cursor, int32_t* __restrict__ in_slots,
+                                                 int32_t* __restrict__ in_node) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < m && eps[e] > 0.f) {
+    const int32_t j = tail[e];
+    const uint32_t p = off[j] + atomicAdd(&cursor[j], 1u);
+    in_slots[p] = static_cast<int32_t>(e);
+    in_node[p] = j;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_block_u32(const uint32_t* __restrict__ in, int64_t count,
+                                                        uint32_t* __restrict__ out, uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sh[256];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * 1024 + threadIdx.x * 4;
+  uint32_t v[4];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v[j] = base + j < count ? in[base + j] : 0u;
+    tot += v[j];
+  }
+  sh[threadIdx.x] = tot;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const uint32_t add = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  uint32_t run = sh[threadIdx.x] - tot;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (base + j < count) out[base + j] = run;
+    run += v[j];
+  }
+  if (threadIdx.x == 255) sums[blockIdx.x] = sh[255];
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums_u32(uint32_t* __restrict__ sums, int64_t nb) {
+  __shared__ uint32_t sh[256];
+  const int64_t per = (nb + 255) / 256;
+  const int64_t b0 = threadIdx.x * per;
+  uint32_t tot = 0;
+  for (int64_t j = 0; j < per; ++j)
+    if (b0 + j < nb) tot += sums[b0 + j];
+  sh[threadIdx.x] = tot;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const uint32_t add = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  uint32_t run = sh[threadIdx.x] - tot;
+  for (int64_t j = 0; j < per; ++j)
+    if (b0 + j < nb) {
+      const uint32_t v = sums[b0 + j];
+      sums[b0 + j] = run;
+      run += v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_add_u32(uint32_t* __restrict__ out, int64_t count,
+                                                      const uint32_t* __restrict__ sums) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) out[t] += sums[t / 1024];
 }
 
 // Y += dY; dY = 0.
@@ -561,24 +699,56 @@ static size_t spectral_ws(int64_t n, int k, int nc) {
          2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + align_up(3 * 16 * 8, 256) + 1024;
 }
 
+struct EpochBufs {
+  float* gtail;    // n*k*dim
+  float* dHead;    // n*dim
+  float* dTail;    // n*dim
+};
+
+static size_t epochs_ws(int64_t n, int k, int dim) {
+  return align_up(static_cast<size_t>(n) * k * dim * 4, 256) + 2 * align_up(static_cast<size_t>(n) * dim * 4, 256) +
+         256;
+}
+
+static EpochBufs carve_epochs(void* ws, int64_t n, int k, int dim) {
+  Carve c(ws);
+  EpochBufs b;
+  b.gtail = c.take<float>(static_cast<size_t>(n) * k * dim);
+  b.dHead = c.take<float>(static_cast<size_t>(n) * dim);
+  b.dTail = c.take<float>(static_cast<size_t>(n) * dim);
+  return b;
+}
+
 template <int D, int G>
-static int launch_epoch(const float* Y, float* dY, const int32_t* tail, const float* eps, float* epn, int64_t n,
-                        int k, int epoch, int neg_rate, Law law, uint64_t seed, hipStream_t s) {
+static int launch_epoch(const float* Y, const EpochBufs& B, const int32_t* tail, const float* eps, float* epn,
+                        int64_t n, int k, int epoch, int neg_rate, Law law, uint64_t seed, hipStream_t s) {
   const int64_t threads = n * G;
   const unsigned blocks = static_cast<unsigned>(ceil_div(threads, 256));
-  hipLaunchKernelGGL((k_epoch<D, G>), dim3(blocks), dim3(256), 0, s, Y, dY, tail, eps, epn, n, k, epoch, neg_rate,
-                     law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  hipLaunchKernelGGL((k_epoch<D, G>), dim3(blocks), dim3(256), 0, s, Y, B.gtail, B.dHead, tail, eps, epn, n, k,
+                     epoch, neg_rate, law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
   MV_LAUNCHED("k_epoch");
   return MV_OK;
 }
 
 template <int D>
-static int launch_epoch_g(const float* Y, float* dY, const int32_t* tail, const float* eps, float* epn,
-                          int64_t n, int k, int epoch, int neg_rate, Law law, uint64_t seed, hipStream_t s) {
-  if (k <= 8) return launch_epoch<D, 8>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
-  if (k <= 16) return launch_epoch<D, 16>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
-  if (k <= 32) return launch_epoch<D, 32>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
-  return launch_epoch<D, 64>(Y, dY, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+static int run_epoch(float* Y, const EpochBufs& B, const int32_t* tail, const float* eps, float* epn,
+                     const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int64_t n, int k, int epoch,
+                     int neg_rate, Law law, uint64_t seed, hipStream_t s) {
+  int rc;
+  if (k <= 8) rc = launch_epoch<D, 8>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else if (k <= 16) rc = launch_epoch<D, 16>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else if (k <= 32) rc = launch_epoch<D, 32>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else rc = launch_epoch<D, 64>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  if (rc) return rc;
+  if (n_in > 0) {
+    hipLaunchKernelGGL((k_tail_sum<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, B.gtail,
+                       in_slots, in_node, n_in, B.dTail);
+    MV_LAUNCHED("k_tail_sum");
+  }
+  hipLaunchKernelGGL(k_apply2, dim3(static_cast<unsigned>(ceil_div(n * D, 256))), dim3(256), 0, s, Y, B.dHead,
+                     B.dTail, n * D);
+  MV_LAUNCHED("k_apply2");
+  return MV_OK;
 }
 
 }  // namespace umap
@@ -627,20 +797,73 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
   return MV_OK;
 }
 
-int mv_umap_epochs(float* Y, float* dY, int64_t n, int dim, const int32_t* tail, int k, const float* eps,
-                   float* epn, int e0, int e1, int n_epochs, int neg_rate, float lr, float a, float b,
-                   uint64_t seed, void* stream) {
-  MV_REQUIRE(Y && dY && tail && eps && epn, "mv_umap_epochs: null pointer");
+int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, int32_t* in_slots,
+                     int32_t* in_node, void* ws, size_t ws_bytes, void* stream) {
+  MV_REQUIRE(tail && eps && in_slots && in_node, "mv_umap_in_edges: null pointer");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX, "mv_umap_in_edges: bad n=%lld k=%d", (long long)n, k);
+  const int64_t nb = ceil_div(n, 1024);
+  const size_t need = 3 * align_up(static_cast<size_t>(n) * 4, 256) + align_up(static_cast<size_t>(nb) * 4, 256);
+  if (!ws || ws_bytes < need) {
+    mv::set_error("mv_umap_in_edges: workspace of %zu bytes < required %zu", ws_bytes, need);
+    return MV_EWORKSPACE;
+  }
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Carve c(ws);
+  uint32_t* cnt = c.take<uint32_t>(n);
+  uint32_t* off = c.take<uint32_t>(n);
+  uint32_t* cur = c.take<uint32_t>(n);
+  uint32_t* sums = c.take<uint32_t>(nb);
+  const int64_t m = n * k;
+  const unsigned bn = static_cast<unsigned>(ceil_div(n, 256));
+  const unsigned bm = static_cast<unsigned>(ceil_div(m, 256));
+  hipLaunchKernelGGL(k_zero_u32, dim3(bn), dim3(256), 0, s, cnt, static_cast<int>(n));
+  MV_LAUNCHED("k_zero_u32");
+  hipLaunchKernelGGL(k_zero_u32, dim3(bn), dim3(256), 0, s, cur, static_cast<int>(n));
+  MV_LAUNCHED("k_zero_u32");
+  hipLaunchKernelGGL(k_in_count, dim3(bm), dim3(256), 0, s, tail, eps, m, cnt);
+  MV_LAUNCHED("k_in_count");
+  hipLaunchKernelGGL(k_scan_block_u32, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, cnt, n, off, sums);
+  MV_LAUNCHED("k_scan_block_u32");
+  hipLaunchKernelGGL(k_scan_sums_u32, dim3(1), dim3(256), 0, s, sums, nb);
+  MV_LAUNCHED("k_scan_sums_u32");
+  hipLaunchKernelGGL(k_scan_add_u32, dim3(bn), dim3(256), 0, s, off, n, sums);
+  MV_LAUNCHED("k_scan_add_u32");
+  hipLaunchKernelGGL(k_in_fill, dim3(bm), dim3(256), 0, s, tail, eps, m, off, cur, in_slots, in_node);
+  MV_LAUNCHED("k_in_fill");
+  return MV_OK;
+}
+
+int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes) {
+  MV_REQUIRE(out_bytes, "mv_umap_epochs_ws_bytes: out_bytes is NULL");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= 64 && dim >= 1 && dim <= 4, "mv_umap_epochs_ws_bytes: bad sizes");
+  *out_bytes = epochs_ws(n, k, dim);
+  return MV_OK;
+}
+
+int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
+                   const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1, int n_epochs,
+                   int neg_rate, float lr, float a, float b, uint64_t seed, void* ws, size_t ws_bytes,
+                   void* stream) {
+  MV_REQUIRE(Y && tail && eps && epn, "mv_umap_epochs: null pointer");
+  MV_REQUIRE(n_in == 0 || (in_slots && in_node), "mv_umap_epochs: null in-edge list");
   MV_REQUIRE(n >= 1 && k >= 1 && k <= 64, "mv_umap_epochs: bad n=%lld k=%d", (long long)n, k);
+  MV_REQUIRE(n_in >= 0 && n_in <= n * k, "mv_umap_epochs: bad in-edge count %lld", (long long)n_in);
   MV_REQUIRE(dim >= 1 && dim <= 4, "mv_umap_epochs: dim=%d outside [1, 4]", dim);
-  MV_REQUIRE(n < (1ll << 32), "mv_umap_epochs: n too large");
+  MV_REQUIRE(n < (1ll << 31), "mv_umap_epochs: n too large");
   MV_REQUIRE(0 <= e0 && e0 <= e1 && e1 <= n_epochs, "mv_umap_epochs: bad epoch range [%d, %d) of %d", e0, e1,
              n_epochs);
   MV_REQUIRE(neg_rate >= 0, "mv_umap_epochs: negative_sample_rate < 0");
+  if (!ws || ws_bytes < epochs_ws(n, k, dim)) {
+    mv::set_error("mv_umap_epochs: workspace of %zu bytes < required %zu", ws_bytes, epochs_ws(n, k, dim));
+    return MV_EWORKSPACE;
+  }
   MV_FAULT_POINT("launch");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  EpochBufs B = carve_epochs(ws, n, k, dim);
   const int64_t count = n * dim;
-  const unsigned ablocks = static_cast<unsigned>(ceil_div(count, 256));
+  hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, B.dTail, count);
+  MV_LAUNCHED("k_zero_f32");
   for (int ep = e0; ep < e1; ++ep) {
     Law law;
     law.a = a;
@@ -648,14 +871,12 @@ int mv_umap_epochs(float* Y, float* dY, int64_t n, int dim, const int32_t* tail,
     law.alpha = static_cast<float>(static_cast<double>(lr) * (1.0 - static_cast<double>(ep) / n_epochs));
     int rc = MV_OK;
     switch (dim) {
-      case 1: rc = launch_epoch_g<1>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
-      case 2: rc = launch_epoch_g<2>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
-      case 3: rc = launch_epoch_g<3>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
-      default: rc = launch_epoch_g<4>(Y, dY, tail, eps, epn, n, k, ep, neg_rate, law, seed, s); break;
+      case 1: rc = run_epoch<1>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
+      case 2: rc = run_epoch<2>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
+      case 3: rc = run_epoch<3>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
+      default: rc = run_epoch<4>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
     }
     if (rc != MV_OK) return rc;
-    hipLaunchKernelGGL(k_apply, dim3(ablocks), dim3(256), 0, s, Y, dY, count);
-    MV_LAUNCHED("k_apply");
   }
   return MV_OK;
 }

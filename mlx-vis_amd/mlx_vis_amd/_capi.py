@@ -54,8 +54,10 @@ _SIGNATURES: dict[str, list] = {
     "mv_knn_last_phase_ms": [_p],
     "mv_umap_graph_ws_bytes": [_i64, _i32, C.POINTER(_sz)],
     "mv_umap_fuzzy_graph": [_p, _p, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _sz, _p],
-    "mv_umap_epochs": [_p, _p, _i64, _i32, _p, _i32, _p, _p, _i32, _i32, _i32, _i32, _f32, _f32, _f32,
-                       _u64, _p],
+    "mv_umap_in_edges": [_p, _p, _i64, _i32, _p, _p, _p, _sz, _p],
+    "mv_umap_epochs_ws_bytes": [_i64, _i32, _i32, C.POINTER(_sz)],
+    "mv_umap_epochs": [_p, _i64, _i32, _p, _i32, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32, _f32, _f32, _f32,
+                       _u64, _p, _sz, _p],
     "mv_umap_sgd_step": [_p, _p, _i64, _i32, _p, _p, _i64, _p, _p, _i64, _f32, _f32, _f32, _p],
     "mv_zero_f32": [_p, _i64, _p],
     "mv_nnd_ws_bytes": [_i64, _i64, _i32, _i64, _i64, C.POINTER(_sz)],

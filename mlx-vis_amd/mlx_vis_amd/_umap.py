@@ -57,7 +57,7 @@ class FuzzyGraph:
     """Device-resident UMAP graph over the dense (n, k) forward-edge slots."""
 
     def __init__(self, knn_idx: torch.Tensor, w_sym, eps, epn, n_edges: int, w_max: float,
-                 sigma=None, rho=None):
+                 sigma=None, rho=None, in_slots=None, in_node=None):
         self.knn_idx = knn_idx
         self.w_sym = w_sym
         self.eps = eps
@@ -66,6 +66,8 @@ class FuzzyGraph:
         self.w_max = w_max
         self.sigma = sigma
         self.rho = rho
+        self.in_slots = in_slots
+        self.in_node = in_node
 
     @property
     def n(self):
@@ -97,19 +99,36 @@ def fuzzy_graph_device(knn_idx: torch.Tensor, knn_dist: torch.Tensor, n_epochs: 
         knn_idx.data_ptr(), knn_dist.data_ptr(), n, k, int(n_epochs), w_sym.data_ptr(), eps.data_ptr(),
         epn.data_ptr(), counters.data_ptr(), _capi.ptr(sigma), _capi.ptr(rho), ws.data_ptr(), ws.numel(),
         _device.stream()), "mv_umap_fuzzy_graph")
+    in_slots = torch.empty(n * k, dtype=torch.int32, device=dev)
+    in_node = torch.empty(n * k, dtype=torch.int32, device=dev)
+    nb = -(-n // 1024)
+    ws2 = _device.workspace(3 * (-(-4 * n // 256) * 256) + (-(-4 * nb // 256) * 256) + 256)
+    _capi.check(lib.mv_umap_in_edges(knn_idx.data_ptr(), eps.data_ptr(), n, k, in_slots.data_ptr(),
+                                     in_node.data_ptr(), ws2.data_ptr(), ws2.numel(), _device.stream()),
+                "mv_umap_in_edges")
     c = counters.cpu()
     n_edges = int(c[0].item()) & 0xFFFFFFFF
     w_max = float(np.array([int(c[1].item()) & 0xFFFFFFFF], dtype=np.uint32).view(np.float32)[0])
-    return FuzzyGraph(knn_idx, w_sym, eps, epn, n_edges, w_max, sigma, rho)
+    return FuzzyGraph(knn_idx, w_sym, eps, epn, n_edges, w_max, sigma, rho, in_slots, in_node)
 
 
-def run_epochs(Y: torch.Tensor, dY: torch.Tensor, g: FuzzyGraph, e0: int, e1: int, n_epochs: int,
+def epochs_workspace(n: int, k: int, dim: int) -> torch.Tensor:
+    nbytes = ctypes.c_size_t(0)
+    _capi.check(_capi.lib().mv_umap_epochs_ws_bytes(n, k, dim, ctypes.byref(nbytes)), "mv_umap_epochs_ws_bytes")
+    return _device.workspace(nbytes.value)
+
+
+def run_epochs(Y: torch.Tensor, ws: torch.Tensor | None, g: FuzzyGraph, e0: int, e1: int, n_epochs: int,
                neg_rate: int, lr: float, a: float, b: float, seed: int) -> None:
+    """Runs layout epochs [e0, e1) in place on Y (ws from epochs_workspace, or None)."""
     n, dim = Y.shape
+    if ws is None:
+        ws = epochs_workspace(n, g.k, dim)
     _capi.check(_capi.lib().mv_umap_epochs(
-        Y.data_ptr(), dY.data_ptr(), n, dim, g.knn_idx.data_ptr(), g.k, g.eps.data_ptr(), g.epn.data_ptr(),
-        int(e0), int(e1), int(n_epochs), int(neg_rate), float(lr), float(a), float(b),
-        int(seed) & 0xFFFFFFFFFFFFFFFF, _device.stream()), "mv_umap_epochs")
+        Y.data_ptr(), n, dim, g.knn_idx.data_ptr(), g.k, g.eps.data_ptr(), g.epn.data_ptr(),
+        g.in_slots.data_ptr(), g.in_node.data_ptr(), int(g.n_edges), int(e0), int(e1), int(n_epochs),
+        int(neg_rate), float(lr), float(a), float(b), int(seed) & 0xFFFFFFFFFFFFFFFF, ws.data_ptr(), ws.numel(),
+        _device.stream()), "mv_umap_epochs")
 
 
 def sgd_step_device(Y: torch.Tensor, ef, et, neg_from, neg_to, alpha: float, a: float, b: float) -> torch.Tensor:
@@ -263,7 +282,7 @@ class UMAP:
 
     def _optimize(self, g: FuzzyGraph, Y: torch.Tensor, a: float, b: float, epoch_callback=None):
         n_epochs = int(self.n_epochs)
-        dY = torch.zeros_like(Y)
+        ws = epochs_workspace(Y.shape[0], g.k, Y.shape[1])
         seed = self.random_state if self.random_state is not None else int(np.random.randint(0, 2**31 - 1))
         seed = (int(seed) * 0x9E3779B97F4A7C15 + 0x632BE59BD9B4E019) & 0xFFFFFFFFFFFFFFFF
         if epoch_callback is not None:
@@ -276,13 +295,13 @@ class UMAP:
             step = 50 if self.verbose else n_epochs
             for e0 in range(1, n_epochs, step):
                 e1 = min(n_epochs, e0 + step)
-                run_epochs(Y, dY, g, e0, e1, n_epochs, self.negative_sample_rate, self.learning_rate, a, b, seed)
+                run_epochs(Y, ws, g, e0, e1, n_epochs, self.negative_sample_rate, self.learning_rate, a, b, seed)
                 if self.verbose:
                     torch.cuda.synchronize()
                     print(f"Epoch {e1}/{n_epochs}")
         else:
             for ep in range(1, n_epochs):
-                run_epochs(Y, dY, g, ep, ep + 1, n_epochs, self.negative_sample_rate, self.learning_rate, a, b,
+                run_epochs(Y, ws, g, ep, ep + 1, n_epochs, self.negative_sample_rate, self.learning_rate, a, b,
                            seed)
                 epoch_callback(ep + 1, Y.cpu().numpy())
         return Y

@@ -54,7 +54,7 @@ def test_schedule_bit_exact():
     np.testing.assert_array_equal(g.eps.cpu().numpy().reshape(-1)[kept], eps_or)
     Y = torch.zeros((1500, 2), dtype=torch.float32, device="cuda")
     dY = torch.zeros_like(Y)
-    run_epochs(Y, dY, g, 1, n_epochs, n_epochs, 0, 1.0, 1.577, 0.895, 7)
+    run_epochs(Y, None, g, 1, n_epochs, n_epochs, 0, 1.0, 1.577, 0.895, 7)
     epn = g.epn.cpu().numpy().reshape(-1)[kept]
     ref = eps_or.copy()
     for epoch in range(n_epochs):
@@ -94,7 +94,7 @@ def test_layout_quality_matches_oracle_run():
     Y0 = np.random.RandomState(0).uniform(-10, 10, size=(n, 2)).astype(np.float32)
     g = fuzzy_graph_device(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n_epochs)
     Y = torch.from_numpy(Y0).cuda()
-    run_epochs(Y, torch.zeros_like(Y), g, 1, n_epochs, n_epochs, 5, 1.0, a, b, 1234)
+    run_epochs(Y, None, g, 1, n_epochs, n_epochs, 5, 1.0, a, b, 1234)
     Yg = Y.cpu().numpy()
     rows, cols, vals = ou.fuzzy_simplicial_set(idx, dist, n_epochs)
     Yo = ou.optimize(rows, cols, vals, Y0, a, b, n_epochs, seed=0)
