@@ -43,7 +43,10 @@ def parse():
     ap.add_argument("--epochs", type=int, default=500)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--knn-reps", type=int, default=3)
-    ap.add_argument("--only", choices=["knn", "copy"], default=None,
+    ap.add_argument("--learned-n", type=int, default=1_000_000,
+                    help="fresh points mapped by the learned form (default 1,000,000)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-form line")
+    ap.add_argument("--only", choices=["knn", "copy", "learned"], default=None,
                     help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8"])
     ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
@@ -83,14 +86,113 @@ def knn_preservation_gpu(knn_hi, Y):
     """Mean overlap of the 15-NN sets in the input and the embedding."""
     import torch
 
-    from mlx_vis_amd import knn_brute_device
+    from mlx_vis_amd import knn_brute_device, knn_preservation_device
 
     Yd = torch.as_tensor(Y).cuda().contiguous()
     lo, _ = knn_brute_device(Yd, knn_hi.shape[1])
-    hi = knn_hi.long()
-    lo = lo.long()
-    inter = (hi[:, :, None] == lo[:, None, :]).any(dim=2).sum(dim=1).float()
-    return float((inter / hi.shape[1]).mean().item())
+    counts, _ = knn_preservation_device(knn_hi, lo)
+    return float(counts.double().mean().item()) / knn_hi.shape[1]
+
+
+MLP_FLOP_PER_POINT = 2 * (784 * 512 + 512 * 512 + 512 * 2)   # forward of 784 -> 512 -> 512 -> 2
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_form(Xd, um, fresh_n, reps=5):
+    """Trains the MLP on the fitted UMAP (graph + embedding of the 70K set), maps
+    `fresh_n` fresh points of the same mixture, times a plain layout of them,
+    and judges both against the exact 15-NN of the fresh points; points below
+    0.8 preservation get 50 plain epochs."""
+    import numpy as np
+    import torch
+
+    from mlx_vis_amd import UMAP, ParametricUMAP, knn_brute_device, knn_preservation_device
+    from mlx_vis_amd.datasets import mixture_fresh_784
+
+    out = {}
+    pu = ParametricUMAP(hidden=(512, 512), batch_size=4096, learning_rate=1e-3, random_state=42)
+    log("learned form: training the MLP on the 70K graph and embedding")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pu.fit(Xd, umap=um)
+    torch.cuda.synchronize()
+    out["train_s"] = round(time.perf_counter() - t, 4)
+    out["train_steps"] = pu.steps_
+    tr = pu.loss_trace_.cpu().numpy()
+    out["loss_first_last"] = [round(float(tr[:20].sum(axis=1).mean()), 4), round(float(tr[-20:].sum(axis=1).mean()), 4)]
+    z_train = pu.transform_device(Xd)
+    out["train_knn15_preservation_mlp"] = round(knn_preservation_gpu(um.knn_[0], z_train), 4)
+
+    log(f"learned form: generating {fresh_n} fresh points (seed 3)")
+    Xf = mixture_fresh_784(fresh_n, seed=3)
+    Xfd = torch.from_numpy(Xf).cuda()
+    del Xf
+    chunk = 131072
+    ws = pu.forward_workspace(min(chunk, fresh_n))
+    Z = pu.transform_device(Xfd, chunk=chunk, ws=ws)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        Z = pu.transform_device(Xfd, chunk=chunk, ws=ws)
+    ev[1].record()
+    torch.cuda.synchronize()
+    t_inf = ev[0].elapsed_time(ev[1]) / reps / 1e3
+    flops = MLP_FLOP_PER_POINT * fresh_n
+    out["infer_s"] = round(t_inf, 6)
+    out["infer_points_per_s"] = round(fresh_n / t_inf, 1)
+    out["infer_roofline"] = {"bound": "mfma", "achieved": round(flops / t_inf / 1e12, 2),
+                             "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(flops / t_inf / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4)}
+
+    log("learned form: plain layout of the fresh points (UMAP defaults: NNDescent kNN, 200 epochs)")
+    plain = UMAP(n_neighbors=15, min_dist=0.1, negative_sample_rate=5, random_state=42)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    z_plain = plain.fit_transform(Xfd)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t
+    out["plain_layout_s"] = round(t_plain, 4)
+    out["speedup_infer_vs_plain"] = round(t_plain / t_inf, 1)
+
+    log("learned form: exact 15-NN of the fresh points (FP32 brute force)")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    hi_idx, hi_dist = knn_brute_device(Xfd, 15)
+    torch.cuda.synchronize()
+    out["judge_exact_knn_s"] = round(time.perf_counter() - t, 3)
+
+    def pres(Zd):
+        lo, _ = knn_brute_device(Zd.contiguous(), 15)
+        c, _ = knn_preservation_device(hi_idx, lo)
+        return c
+
+    t = time.perf_counter()
+    c_mlp = pres(Z)
+    torch.cuda.synchronize()
+    out["judge_lowd_knn_s"] = round(time.perf_counter() - t, 3)
+    c_plain = pres(torch.from_numpy(z_plain).cuda())
+    p_mlp = float(c_mlp.double().mean().item()) / 15
+    p_plain = float(c_plain.double().mean().item()) / 15
+    log("learned form: refining points below 0.8 preservation (50 plain epochs)")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    z_ref, _ = pu.refine_device(Xfd, Z, threshold=0.8, n_epochs=50, knn=(hi_idx, hi_dist))
+    torch.cuda.synchronize()
+    out["refine_s"] = round(time.perf_counter() - t, 3)
+    c_ref = pres(z_ref)
+    p_ref = float(c_ref.double().mean().item()) / 15
+    flagged = (c_mlp < 12)
+    out["knn15_preservation"] = {"mlp": round(p_mlp, 4), "plain_layout": round(p_plain, 4),
+                                 "mlp_refined": round(p_ref, 4), "gap_mlp_vs_plain": round(p_plain - p_mlp, 4)}
+    out["refine"] = {"flagged_frac": round(float(flagged.float().mean().item()), 4),
+                     "flagged_before": round(float(c_mlp[flagged].double().mean().item()) / 15, 4)
+                     if bool(flagged.any()) else None,
+                     "flagged_after": round(float(c_ref[flagged].double().mean().item()) / 15, 4)
+                     if bool(flagged.any()) else None}
+    out["config"] = {"mlp": "784-512-512-2 GELU, BF16 weights + FP32 master, Adam lr 1e-3, batch 4096 edges",
+                     "fresh_points": fresh_n, "threshold": 0.8, "refine_epochs": 50}
+    return out
 
 
 def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs=8, spec_iters=5):
@@ -184,6 +286,13 @@ def main():
         phases, stats, _ = knn_phase_timing(Xd, args.knn_reps, args.dtype)
         print(json.dumps({"knn_dtype": args.dtype, "ms_phases": phases, "stats": stats,
                           "tflops_screen": 2.0 * n * n * d / (phases[1] * 1e-3) / 1e12}), flush=True)
+        return
+
+    if args.only == "learned":
+        um = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=args.epochs, negative_sample_rate=5,
+                  knn_method="brute", random_state=42)
+        um.fit_transform(Xd)
+        print(json.dumps({"learned": learned_form(Xd, um, args.learned_n)}), flush=True)
         return
 
     def step():
@@ -290,6 +399,8 @@ def main():
         inter = (gi.long()[:, :, None] == hi[:, None, :]).any(dim=2).sum(dim=1).float()
         line["nndescent"] = {"s": round(t_nnd, 4), "recall_vs_exact": round(float((inter / 15).mean().item()), 4),
                              "iterations": nnd.iterations_, "changed": nnd.changed_}
+    if not args.no_extras and not args.no_learned:
+        line["learned"] = learned_form(Xd, um, args.learned_n)
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement, bounded sample)")
         line["cpu_baseline"] = cpu_baseline(X, um.knn_[0].cpu().numpy(), um.knn_[1].cpu().numpy(), args.epochs)

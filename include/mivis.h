@@ -169,6 +169,15 @@ int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, con
                    int neg_rate, float lr, float a, float b, uint64_t seed, void* ws, size_t ws_bytes,
                    void* stream);
 
+/* mv_umap_epochs with a per-node mask (no counterpart in the reference; used
+ * to refine chosen points of an embedding against fixed ones): movable
+ * (device, n bytes, nullable) — nodes with movable[i] == 0 keep their
+ * position, every gradient is still computed from the full embedding. */
+int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
+                          const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1,
+                          int n_epochs, int neg_rate, float lr, float a, float b, uint64_t seed,
+                          const uint8_t* movable, void* ws, size_t ws_bytes, void* stream);
+
 /* One synchronous step with explicit edge and negative lists, exactly
  * UMAP._sgd_step (umap.py:359-387): Y_out = Y_in + updates, where every update
  * is computed from Y_in (Y_out must not alias Y_in). For parity checks. */
@@ -184,6 +193,59 @@ int mv_umap_sgd_step(float* Y_out, const float* Y_in, int64_t n, int dim, const 
 int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes);
 int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, int64_t n, int k, int nc,
                      int iters, float* V, void* ws, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned form: parametric MLP embedding din -> h1 -> h2 -> dout (GELU).
+ * No counterpart in the reference, whose only MLP is the MMAE autoencoder
+ * (mlx_vis/_mmae/mmae.py:22-60, AdamW loop 258-281): that is the precedent
+ * for fitting a network to an embedding and encoding new points with it.
+ *
+ * Parameters live in one device blob (mv_mlp_param_bytes): FP32 master
+ * W1 (h1 x din, stored with rows padded to dp = align64(din)), b1, W2
+ * (h2 x h1), b2, W3 (dout x h2), b3, then Adam m and v, then BF16 copies of
+ * W1, W2 and W2^T. Hidden widths are multiples of 64 up to 1024, dout <= 4.
+ * Forward: H1 = gelu(X W1^T + b1), H2 = gelu(H1 W2^T + b2), Z = H2 W3^T + b3,
+ * with X, W1, W2 and the activations rounded to BF16, FP32 accumulation, exact
+ * erf GELU. ------------------------------------------------------------ */
+int mv_mlp_param_bytes(int din, int h1, int h2, int dout, size_t* out_bytes, int64_t* out_n_params);
+/* Workspace of a batch of `points` rows (training uses 2B points). */
+int mv_mlp_batch_bytes(int din, int h1, int h2, int dout, int points, size_t* out_bytes);
+/* Loads FP32 weights (device; W1 h1 x din row-major) into the blob, zeroes
+ * the Adam moments and writes the BF16 copies. */
+int mv_mlp_init(void* blob, int din, int h1, int h2, int dout, const float* W1, const float* b1, const float* W2,
+                const float* b2, const float* W3, const float* b3, void* stream);
+/* Gradient of one batch of B edges (B a multiple of 32): rows idx[0:B] are
+ * heads and idx[B:2B] tails (rows of X, n x din FP32, and of the target
+ * embedding Yp, n x dout). Loss (1/B) sum_e [log(1 + a d_e^(2b))
+ * - sum_{s<neg} log(1 - 1/(1 + a d_es^(2b)))] + (lam/B) sum_rows ||z - y||^2,
+ * negatives = tails of edges (e + 1 + 97 s) mod B; z-gradient = the layout's
+ * law (umap.py:367-380) without the clip. grad_out (device, n_params FP32 in
+ * blob order, nullable), loss_out (device, 2 floats: edge term, regression
+ * term, nullable). ws: mv_mlp_batch_bytes(2B) + align256(2 n dp) bytes. */
+int mv_mlp_grad(const void* blob, int din, int h1, int h2, int dout, const float* X, int64_t n, const float* Yp,
+                const int32_t* idx, int B, int neg, float a, float b, float lam, float* grad_out, float* loss_out,
+                void* ws, size_t ws_bytes, void* stream);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8, bias-corrected, step
+ * numbers step0 + 1 ...) on the batches idx[t * 2B : (t + 1) * 2B].
+ * loss_trace (device, 2 * steps floats, nullable). Same ws as mv_mlp_grad. */
+int mv_mlp_train(void* blob, int din, int h1, int h2, int dout, const float* X, int64_t n, const float* Yp,
+                 const int32_t* idx, int steps, int B, int neg, float a, float b, float lam, float lr, int step0,
+                 float* loss_trace, void* ws, size_t ws_bytes, void* stream);
+/* Inference Z (m x dout) of X (m x din FP32) in chunks of `chunk` rows;
+ * ws: mv_mlp_batch_bytes(chunk). */
+int mv_mlp_forward(const void* blob, int din, int h1, int h2, int dout, const float* X, int64_t m, float* Z,
+                   int chunk, void* ws, size_t ws_bytes, void* stream);
+/* Edge batches for mv_mlp_train from a fuzzy graph's slot arrays (eps from
+ * mv_umap_fuzzy_graph, tail = kNN indices): `steps` batches of B edges drawn
+ * in proportion to their weights (rejection on 1 / eps, Philox-4x32-10 keyed
+ * by seed, counters from step0), written as heads then tails per batch. */
+int mv_mlp_sample_edges(const float* eps, const int32_t* tail, int64_t n, int k, uint32_t seed, int step0, int steps,
+                        int B, int32_t* out_idx, void* stream);
+/* Per-row overlap of two neighbour lists, the 15-NN preservation count:
+ * out_count[i] = |a[i, 0:ka] & b[i, 0:kb]|, out_flag[i] = out_count[i] <
+ * min_count (each output nullable, not both). */
+int mv_knn_overlap(const int32_t* a, int64_t lda, int ka, const int32_t* b, int64_t ldb, int kb, int64_t n,
+                   int min_count, int32_t* out_count, uint8_t* out_flag, void* stream);
 
 /* p[0:count] = 0 on the stream (a fill kernel, safe under graph capture). */
 int mv_zero_f32(float* p, int64_t count, void* stream);

@@ -140,4 +140,35 @@ __device__ __forceinline__ float key_dist(uint64_t k) {
 }
 __device__ __forceinline__ uint32_t key_idx(uint64_t k) { return static_cast<uint32_t>(k); }
 
+// Philox-4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, SC'11):
+// ten rounds of the 4x32 multiply-xor round with the published multipliers
+// and Weyl key increments.
+struct Philox {
+  __device__ static __forceinline__ void round(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                                               uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
+    const uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+  }
+  __device__ static __forceinline__ uint4 gen(uint4 ctr, uint32_t k0, uint32_t k1) {
+    uint32_t c0 = ctr.x, c1 = ctr.y, c2 = ctr.z, c3 = ctr.w;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      round(c0, c1, c2, c3, k0, k1);
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+  }
+};
+
+__device__ __forceinline__ uint32_t uniform_below(uint32_t r, uint32_t n) {
+  return static_cast<uint32_t>((static_cast<uint64_t>(r) * n) >> 32);
+}
+
 }  // namespace mv

@@ -1,0 +1,939 @@
+// Parametric embedding (learned form) for gfx950: an MLP din -> h1 -> h2 -> dout
+// with GELU, BF16 weights and activations, FP32 master weights, FP32
+// accumulation, and Adam. Trained on the UMAP edge loss over a kNN graph plus a
+// regression to a plain-layout embedding; maps new points in one pass.
+//
+// The reference has no parametric UMAP; its only MLP is the MMAE autoencoder
+// (mlx_vis/_mmae/mmae.py:22-60, trained with AdamW at mmae.py:258-281), the
+// precedent for the API shape (fit on a reference embedding, encode new
+// points). The loss head reuses the layout's gradient law: the attractive
+// coefficient -2ab d^(2(b-1)) / (1 + a d^(2b)) and the repulsive coefficient
+// 2b / ((0.001 + d^2)(1 + a d^(2b))) of umap.py:367-380, here without the
+// layout's +-4 clip, so that it is the exact gradient of the edge
+// cross-entropy.
+//
+// Kernels:
+//   k_gemm_nt<EPI>   C = A B^T, A (M x K) and B (N x K) BF16 row-major (K
+//                    contiguous), v_mfma_f32_16x16x32_bf16, 128x128 tiles,
+//                    64-element K chunks double-buffered in LDS; epilogues:
+//                    FP32 store, bias + GELU (stores pre-activation and
+//                    activation in BF16), GELU-gradient product (BF16).
+//   k_transpose_bf16, k_gather_x, k_to_bf16, k_head_fwd, k_loss_grad,
+//   k_head_bwd, k_adam, k_sample_edges.
+#include <hip/hip_bf16.h>
+#include <math.h>
+
+#include "common.hpp"
+
+namespace mv {
+namespace mlp {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint16_t bf16_t;   // storage type: IEEE bfloat16 bit pattern
+
+constexpr int BM = 128, BN = 128, LDSA = 36;   // LDS row stride in floats (128 B data + 16 B pad)
+constexpr int NT = 256;
+
+__device__ __forceinline__ bf16_t f2bf(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7F800000u) == 0x7F800000u) return static_cast<bf16_t>(u >> 16);   // inf / nan
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<bf16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_GRAD = 2, EPI_BF16 = 3 };
+
+struct GemmArgs {
+  const bf16_t* A;   // M x K, row stride lda (elements)
+  const bf16_t* B;   // N x K, row stride ldb
+  int M, N, K;       // K multiple of 64
+  int64_t lda, ldb, ldc, ldx;
+  void* C;           // F32 or BF16 output, row stride ldc
+  const float* bias;       // EPI_BIAS_GELU: bias[N]
+  bf16_t* pre_out;         // EPI_BIAS_GELU (nullable): pre-activation output (row stride ldx)
+  const bf16_t* pre_in;    // EPI_GELU_GRAD: pre-activation input (row stride ldx)
+  float* colsum;           // EPI_GELU_GRAD (nullable): colsum[col] += sum over rows of the output
+};
+
+template <int EPI>
+__global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) float s_stage[2 * 2 * BM * LDSA];
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int wr = w >> 1, wc = w & 1;
+  const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
+  const int lr = tid >> 3, lkb = (tid & 7) * 16;   // staging row, byte offset in the 128-byte chunk
+  const int nk = g.K / 64;
+
+  // Rows past M / N are clamped (read in bounds, never stored).
+  const char* Arow[4];
+  const char* Brow[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int ra = min(bm + lr + 32 * i, g.M - 1);
+    const int rbn = min(bn + lr + 32 * i, g.N - 1);
+    Arow[i] = reinterpret_cast<const char*>(g.A + static_cast<int64_t>(ra) * g.lda) + lkb;
+    Brow[i] = reinterpret_cast<const char*>(g.B + static_cast<int64_t>(rbn) * g.ldb) + lkb;
+  }
+  v4f acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  float4 ra[4], rv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    ra[i] = *reinterpret_cast<const float4*>(Arow[i]);
+    rv[i] = *reinterpret_cast<const float4*>(Brow[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    *reinterpret_cast<float4*>(s_stage + (lr + 32 * i) * LDSA + lkb / 4) = ra[i];
+    *reinterpret_cast<float4*>(s_stage + BM * LDSA + (lr + 32 * i) * LDSA + lkb / 4) = rv[i];
+  }
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int cur = kc & 1;
+    const int kn = kc + 1 < nk ? kc + 1 : kc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ra[i] = *reinterpret_cast<const float4*>(Arow[i] + static_cast<int64_t>(kn) * 128);
+      rv[i] = *reinterpret_cast<const float4*>(Brow[i] + static_cast<int64_t>(kn) * 128);
+    }
+    const float* As = s_stage + cur * (2 * BM * LDSA);
+    const float* Bs = As + BM * LDSA;
+#pragma unroll
+    for (int gk = 0; gk < 2; ++gk) {
+      v8bf af[4], bf[4];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+        af[mi] = __builtin_bit_cast(
+            v8bf, *reinterpret_cast<const v4f*>(As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + gk * 16 + (l >> 4) * 4));
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+        bf[ni] = __builtin_bit_cast(
+            v8bf, *reinterpret_cast<const v4f*>(Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + gk * 16 + (l >> 4) * 4));
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+    }
+    {
+      float* As2 = s_stage + (cur ^ 1) * (2 * BM * LDSA);
+      float* Bs2 = As2 + BM * LDSA;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        *reinterpret_cast<float4*>(As2 + (lr + 32 * i) * LDSA + lkb / 4) = ra[i];
+        *reinterpret_cast<float4*>(Bs2 + (lr + 32 * i) * LDSA + lkb / 4) = rv[i];
+      }
+    }
+    __syncthreads();
+  }
+
+  float csum[4] = {0.f, 0.f, 0.f, 0.f};   // EPI_GELU_GRAD: column sums of this lane's outputs
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = bm + wr * 64 + mi * 16 + (l >> 4) * 4 + r;
+      if (row >= g.M) continue;
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        const int col = bn + wc * 64 + ni * 16 + (l & 15);
+        if (col >= g.N) continue;
+        const float v = acc[mi][ni][r];
+        if (EPI == EPI_F32) {
+          static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = v;
+        } else if (EPI == EPI_BF16) {
+          static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = f2bf(v);
+        } else if (EPI == EPI_BIAS_GELU) {
+          const bf16_t pb = f2bf(v + g.bias[col]);
+          if (g.pre_out) g.pre_out[static_cast<int64_t>(row) * g.ldx + col] = pb;
+          static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = f2bf(gelu(bf2f(pb)));
+        } else {
+          const float pre = bf2f(g.pre_in[static_cast<int64_t>(row) * g.ldx + col]);
+          const bf16_t o = f2bf(v * gelu_grad(pre));
+          static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = o;
+          csum[ni] += bf2f(o);
+        }
+      }
+    }
+  if (EPI == EPI_GELU_GRAD && g.colsum) {
+    // lanes l, l^16, l^32, l^48 hold the same columns
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      float v = csum[ni];
+      v += __shfl_xor(v, 16);
+      v += __shfl_xor(v, 32);
+      const int col = bn + wc * 64 + ni * 16 + (l & 15);
+      if (l < 16 && col < g.N) atomicAdd(&g.colsum[col], v);
+    }
+  }
+}
+
+// out[c, r] = in[r, c] for a rows x cols BF16 matrix (64 x 64 tiles through LDS).
+__global__ __launch_bounds__(256) void k_transpose_bf16(const bf16_t* __restrict__ in, int rows, int cols,
+                                                        int64_t ldi, bf16_t* __restrict__ out, int64_t ldo) {
+  __shared__ bf16_t tile[64][66];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int y = ty; y < 64; y += 4) {
+    const int r = r0 + y, c = c0 + tx;
+    tile[y][tx] = (r < rows && c < cols) ? in[static_cast<int64_t>(r) * ldi + c] : static_cast<bf16_t>(0);
+  }
+  __syncthreads();
+  for (int y = ty; y < 64; y += 4) {
+    const int c = c0 + y, r = r0 + tx;
+    if (c < cols && r < rows) out[static_cast<int64_t>(c) * ldo + r] = tile[tx][y];
+  }
+}
+
+// BF16, zero-padded copy of FP32 rows: out[r, 0:dp] from X[r, 0:d].
+__global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ X, int64_t n, int d, int dp,
+                                                 bf16_t* __restrict__ out) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (r >= n) return;
+  for (int t = l; t < dp; t += 64) out[r * dp + t] = t < d ? f2bf(X[r * d + t]) : static_cast<bf16_t>(0);
+}
+
+// out[p] = src[idx[p]] (rows of dp BF16), and yb[p] = Y[idx[p]] (dout FP32).
+__global__ __launch_bounds__(256) void k_gather_x(const bf16_t* __restrict__ src, int dp, const int32_t* __restrict__ idx,
+                                                  int P, bf16_t* __restrict__ out, const float* __restrict__ Y,
+                                                  int dout, float* __restrict__ yb) {
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (p >= P) return;
+  const int64_t s = idx[p];
+  const uint4* in = reinterpret_cast<const uint4*>(src + s * dp);
+  uint4* o = reinterpret_cast<uint4*>(out + static_cast<int64_t>(p) * dp);
+  for (int t = l; t < dp / 8; t += 64) o[t] = in[t];
+  if (Y && l < dout) yb[static_cast<int64_t>(p) * dout + l] = Y[s * dout + l];
+}
+
+// Z[p] = H2[p] W3^T + b3 (one wavefront per point; dout <= 4; W3 FP32).
+__global__ __launch_bounds__(256) void k_head_fwd(const bf16_t* __restrict__ H2, int P, int h2,
+                                                  const float* __restrict__ W3, const float* __restrict__ b3, int dout,
+                                                  float* __restrict__ Z) {
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (p >= P) return;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int c = l; c < h2; c += 64) {
+    const float h = bf2f(H2[static_cast<int64_t>(p) * h2 + c]);
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+      if (o < dout) acc[o] = fmaf(h, W3[o * h2 + c], acc[o]);
+  }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) acc[o] = wave_sum_f32(acc[o]);
+  if (l < dout) {
+    float v = acc[0];
+    if (l == 1) v = acc[1];
+    if (l == 2) v = acc[2];
+    if (l == 3) v = acc[3];
+    Z[static_cast<int64_t>(p) * dout + l] = v + b3[l];
+  }
+}
+
+// Loss gradient dZ for a batch of B edges: rows [0, B) are heads, [B, 2B)
+// tails. The loss is the UMAP edge cross-entropy plus a regression term,
+//   L = (1/B) sum_e [ log(1 + a d_e^(2b))
+//                     - sum_s log(1 - 1/(1 + a d_es^(2b))) ]
+//     + (lam/B) sum_rows ||z - y||^2,
+// with d_e = ||z_head - z_tail|| (attraction) and d_es = ||z_head - z_t(s)||
+// for the negatives t(s) = tails of edges (e + 1 + 97 s) mod B, s < neg (the
+// in-batch tails, which are drawn in proportion to the edge weights). Its
+// z-gradient is the layout's law (umap.py:367-380) without the +-4 clip:
+// attraction -2ab d^(2(b-1)) / (1 + a d^(2b)) (z_i - z_j) on the head and the
+// opposite on the tail; repulsion 2b / ((0.001 + d^2)(1 + a d^(2b))) (z_i - z_t)
+// on the head only (negatives move the head only, as the layout). Squared
+// distances are floored at 1e-6 as the layout does. loss_out (nullable)
+// accumulates the two terms.
+__global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ Z, const float* __restrict__ Yb, int B,
+                                                   int dout, int neg, float a, float b, float lam,
+                                                   float* __restrict__ dZ, float* __restrict__ loss_out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  float ce = 0.f, reg = 0.f;
+  if (e < B) {
+    const float inv_b = 1.f / static_cast<float>(B);
+    float zi[4] = {0, 0, 0, 0}, zj[4] = {0, 0, 0, 0}, gi[4] = {0, 0, 0, 0}, gj[4] = {0, 0, 0, 0};
+    float d2 = 0.f;
+    for (int q = 0; q < dout; ++q) {
+      zi[q] = Z[static_cast<int64_t>(e) * dout + q];
+      zj[q] = Z[static_cast<int64_t>(B + e) * dout + q];
+      d2 += (zi[q] - zj[q]) * (zi[q] - zj[q]);
+    }
+    {
+      const float dd = fmaxf(d2, 1e-6f);
+      const float pw = exp2f(b * __log2f(dd));
+      const float coef = -2.0f * a * b * exp2f((b - 1.0f) * __log2f(dd)) / (1.0f + a * pw);
+      ce += logf(1.0f + a * pw);
+      for (int q = 0; q < dout; ++q) {
+        const float gq = coef * (zi[q] - zj[q]) * inv_b;   // minus the head's gradient
+        gi[q] -= gq;
+        gj[q] += gq;
+      }
+    }
+    for (int s = 0; s < neg; ++s) {
+      const int te = (e + 1 + s * 97) % B;
+      if (te == e) continue;
+      const int t = B + te;
+      float n2 = 0.f, zt[4] = {0, 0, 0, 0};
+      for (int q = 0; q < dout; ++q) {
+        zt[q] = Z[static_cast<int64_t>(t) * dout + q];
+        n2 += (zi[q] - zt[q]) * (zi[q] - zt[q]);
+      }
+      const float dd = fmaxf(n2, 1e-6f);
+      const float pw = exp2f(b * __log2f(dd));
+      const float coef = 2.0f * b / ((0.001f + dd) * (1.0f + a * pw));
+      ce -= logf(fmaxf(1.0f - 1.0f / (1.0f + a * pw), 1e-30f));
+      for (int q = 0; q < dout; ++q) gi[q] -= coef * (zi[q] - zt[q]) * inv_b;
+    }
+    for (int q = 0; q < dout; ++q) {
+      const float ri = zi[q] - Yb[static_cast<int64_t>(e) * dout + q];
+      const float rj = zj[q] - Yb[static_cast<int64_t>(B + e) * dout + q];
+      reg += ri * ri + rj * rj;
+      dZ[static_cast<int64_t>(e) * dout + q] = gi[q] + 2.f * lam * ri * inv_b;
+      dZ[static_cast<int64_t>(B + e) * dout + q] = gj[q] + 2.f * lam * rj * inv_b;
+    }
+  }
+  if (loss_out) {
+    ce = wave_sum_f32(ce);
+    reg = wave_sum_f32(reg);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&loss_out[0], ce / static_cast<float>(B));
+      atomicAdd(&loss_out[1], lam * reg / static_cast<float>(B));
+    }
+  }
+}
+
+constexpr int HMAX = 1024;        // widest hidden layer
+constexpr int HEAD_PPW = 8;       // points per wavefront in k_head_bwd
+
+// Output-layer backward for DOUT outputs. Per point p:
+//   dA2[p, c] = (sum_o dZ[p, o] W3[o, c]) * gelu'(A2pre[p, c])   (stored BF16)
+// and the block's partial sums of dW3 = dZ^T H2, db3 = sum dZ and
+// db2 = sum dA2 (of the BF16-rounded values, as the GEMMs see them), reduced
+// in LDS and added with one atomic per block and parameter.
+template <int DOUT>
+__global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, const bf16_t* __restrict__ H2,
+                                                  const bf16_t* __restrict__ A2pre, int P, int h2,
+                                                  const float* __restrict__ W3, bf16_t* __restrict__ dA2,
+                                                  float* __restrict__ gW3, float* __restrict__ gb3,
+                                                  float* __restrict__ gb2) {
+  __shared__ float s_w3[DOUT * HMAX];
+  __shared__ float s_b2[HMAX];
+  __shared__ float s_b3[DOUT];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  for (int t = tid; t < DOUT * h2; t += 256) s_w3[t] = 0.f;
+  for (int t = tid; t < h2; t += 256) s_b2[t] = 0.f;
+  if (tid < DOUT) s_b3[tid] = 0.f;
+  __syncthreads();
+  constexpr int JM = HMAX / 64;
+  float pw3[DOUT][JM], pb2[JM], pb3[DOUT];
+#pragma unroll
+  for (int j = 0; j < JM; ++j) {
+    pb2[j] = 0.f;
+#pragma unroll
+    for (int o = 0; o < DOUT; ++o) pw3[o][j] = 0.f;
+  }
+#pragma unroll
+  for (int o = 0; o < DOUT; ++o) pb3[o] = 0.f;
+  const int p0 = (blockIdx.x * 4 + w) * HEAD_PPW;
+  for (int i = 0; i < HEAD_PPW; ++i) {
+    const int p = p0 + i;
+    if (p >= P) break;
+    float dz[DOUT];
+#pragma unroll
+    for (int o = 0; o < DOUT; ++o) {
+      dz[o] = dZ[static_cast<int64_t>(p) * DOUT + o];
+      pb3[o] += dz[o];
+    }
+#pragma unroll
+    for (int j = 0; j < JM; ++j) {
+      const int c = l + 64 * j;
+      if (c < h2) {
+        float s = 0.f;
+#pragma unroll
+        for (int o = 0; o < DOUT; ++o) s = fmaf(dz[o], W3[o * h2 + c], s);
+        const int64_t at = static_cast<int64_t>(p) * h2 + c;
+        const bf16_t g = f2bf(s * gelu_grad(bf2f(A2pre[at])));
+        dA2[at] = g;
+        pb2[j] += bf2f(g);
+        const float h = bf2f(H2[at]);
+#pragma unroll
+        for (int o = 0; o < DOUT; ++o) pw3[o][j] = fmaf(dz[o], h, pw3[o][j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < JM; ++j) {
+    const int c = l + 64 * j;
+    if (c < h2) {
+      atomicAdd(&s_b2[c], pb2[j]);
+#pragma unroll
+      for (int o = 0; o < DOUT; ++o) atomicAdd(&s_w3[o * h2 + c], pw3[o][j]);
+    }
+  }
+  if (l == 0) {
+#pragma unroll
+    for (int o = 0; o < DOUT; ++o) atomicAdd(&s_b3[o], pb3[o]);
+  }
+  __syncthreads();
+  for (int t = tid; t < DOUT * h2; t += 256) atomicAdd(&gW3[t], s_w3[t]);
+  for (int t = tid; t < h2; t += 256) atomicAdd(&gb2[t], s_b2[t]);
+  if (tid < DOUT) atomicAdd(&gb3[tid], s_b3[tid]);
+}
+
+// Adam on FP32 master weights (bias-corrected), writing a BF16 copy and,
+// when `bt` is set, the BF16 transpose (rows x cols -> cols x rows).
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ gsum,
+                                              float* __restrict__ m, float* __restrict__ v, int64_t count, float lr,
+                                              float b1, float b2, float eps, float c1, float c2,
+                                              bf16_t* __restrict__ bcopy, bf16_t* __restrict__ bt, int rows,
+                                              int cols) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const float g = gsum[t];
+  const float mt = b1 * m[t] + (1.f - b1) * g;
+  const float vt = b2 * v[t] + (1.f - b2) * g * g;
+  m[t] = mt;
+  v[t] = vt;
+  const float nv = p[t] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+  p[t] = nv;
+  if (bcopy) bcopy[t] = f2bf(nv);
+  if (bt) {
+    const int r = static_cast<int>(t / cols), c = static_cast<int>(t % cols);
+    bt[static_cast<int64_t>(c) * rows + r] = f2bf(nv);
+  }
+}
+
+// Edge batches drawn from a fuzzy graph in its dense slot layout (slot e =
+// i * k + c, head i, tail tail[e], eps[e] = epochs per sample = w_max / w,
+// -1 for a pruned slot), in proportion to the weights, by rejection: a slot
+// uniform over n * k is accepted with probability 1 / eps. Draw j of step t
+// writes the head to out[t * 2B + j] and the tail to out[t * 2B + B + j].
+// Randomness is Philox-4x32-10 on the counter (j, step0 + t, try, tag) under
+// key (seed, ~seed), two candidates per call; after 64 rejected candidates the
+// highest-weight kept candidate seen is taken (probability (1 - mean w)^64).
+__global__ __launch_bounds__(256) void k_sample_edges(const float* __restrict__ eps, const int32_t* __restrict__ tail,
+                                                      int64_t nk, int k, uint32_t seed, uint32_t step0, int steps,
+                                                      int B, int32_t* __restrict__ out) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= static_cast<int64_t>(steps) * B) return;
+  const int t = static_cast<int>(g / B), j = static_cast<int>(g % B);
+  int64_t pick = -1, best = -1;
+  float best_eps = INFINITY;
+  for (int tr = 0; tr < 32 && pick < 0; ++tr) {
+    const uint4 r = Philox::gen(make_uint4(static_cast<uint32_t>(j), step0 + static_cast<uint32_t>(t),
+                                           static_cast<uint32_t>(tr), 0x4D4C5030u),
+                                seed, ~seed);
+    const uint32_t ra[2] = {r.x, r.z}, ru[2] = {r.y, r.w};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (pick >= 0) break;
+      const int64_t e = uniform_below(ra[h], static_cast<uint32_t>(nk));
+      const float ep = eps[e];
+      if (ep >= 1.f) {
+        const float u = static_cast<float>(ru[h] >> 8) * (1.f / 16777216.f);
+        if (u * ep < 1.f) pick = e;
+        if (ep < best_eps) {
+          best_eps = ep;
+          best = e;
+        }
+      }
+      if (best < 0) best = e;
+    }
+  }
+  if (pick < 0) pick = best;
+  int32_t* o = out + static_cast<int64_t>(t) * 2 * B;
+  o[j] = static_cast<int32_t>(pick / k);
+  o[B + j] = tail[pick];
+}
+
+// Per-row overlap of two k-NN lists (the 15-NN preservation count):
+// count[i] = |{a[i, 0:ka]} & {b[i, 0:kb]}|, flag[i] = count[i] < min_count.
+__global__ __launch_bounds__(256) void k_knn_overlap(const int32_t* __restrict__ A, int64_t lda, int ka,
+                                                     const int32_t* __restrict__ Bm, int64_t ldb, int kb, int64_t n,
+                                                     int min_count, int32_t* __restrict__ count,
+                                                     uint8_t* __restrict__ flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t* a = A + i * lda;
+  const int32_t* b = Bm + i * ldb;
+  int c = 0;
+  for (int x = 0; x < ka; ++x) {
+    const int32_t v = a[x];
+    bool hit = false;
+    for (int y = 0; y < kb; ++y) hit |= (b[y] == v);
+    c += hit ? 1 : 0;
+  }
+  if (count) count[i] = c;
+  if (flag) flag[i] = c < min_count ? 1 : 0;
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, int64_t count, float v) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = v;
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ in, bf16_t* __restrict__ out, int64_t count) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) out[t] = f2bf(in[t]);
+}
+
+__global__ void k_f32_to_bf16_t(const float* __restrict__ in, int rows, int cols, bf16_t* __restrict__ out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < static_cast<int64_t>(rows) * cols) {
+    const int r = static_cast<int>(t / cols), c = static_cast<int>(t % cols);
+    out[static_cast<int64_t>(c) * rows + r] = f2bf(in[t]);
+  }
+}
+
+// ------------------------------------------------------------------ host --
+
+struct Dims {
+  int din, dp, h1, h2, dout;
+};
+
+// Parameter blob: FP32 master (W1 h1 x dp, b1, W2 h2 x h1, b2, W3 dout x h2, b3),
+// Adam m and v of the same shape, then BF16 copies W1b (h1 x dp), W2b (h2 x h1),
+// W2tb (h1 x h2).
+struct Params {
+  float *W1, *b1, *W2, *b2, *W3, *b3;
+  float *m, *v;
+  bf16_t *W1b, *W2b, *W2tb;
+  int64_t n_f32;   // FP32 parameter count (master region)
+};
+
+static int64_t n_params(const Dims& D) {
+  return static_cast<int64_t>(D.h1) * D.dp + D.h1 + static_cast<int64_t>(D.h2) * D.h1 + D.h2 +
+         static_cast<int64_t>(D.dout) * D.h2 + D.dout;
+}
+
+static Params carve_params(void* blob, const Dims& D, size_t* total) {
+  Carve c(blob);
+  Params P{};
+  const int64_t np = n_params(D);
+  P.n_f32 = np;
+  float* base = c.take<float>(np);
+  P.W1 = base;
+  P.b1 = P.W1 + static_cast<int64_t>(D.h1) * D.dp;
+  P.W2 = P.b1 + D.h1;
+  P.b2 = P.W2 + static_cast<int64_t>(D.h2) * D.h1;
+  P.W3 = P.b2 + D.h2;
+  P.b3 = P.W3 + static_cast<int64_t>(D.dout) * D.h2;
+  P.m = c.take<float>(np);
+  P.v = c.take<float>(np);
+  P.W1b = c.take<bf16_t>(static_cast<int64_t>(D.h1) * D.dp);
+  P.W2b = c.take<bf16_t>(static_cast<int64_t>(D.h2) * D.h1);
+  P.W2tb = c.take<bf16_t>(static_cast<int64_t>(D.h1) * D.h2);
+  if (total) *total = c.off + 256;
+  return P;
+}
+
+// Per-batch buffers for P points.
+struct Batch {
+  bf16_t *Xb, *A1, *H1, *A2, *H2, *dA2, *dA2t, *H1t, *dA1, *dA1t, *Xbt;
+  float *Z, *Yb, *dZ, *grad, *loss;
+  int32_t* idx;
+};
+
+static Batch carve_batch(void* ws, const Dims& D, int P, int64_t np, size_t* total) {
+  Carve c(ws);
+  Batch b{};
+  b.Xb = c.take<bf16_t>(static_cast<int64_t>(P) * D.dp);
+  b.A1 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
+  b.H1 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
+  b.A2 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h2);
+  b.H2 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h2);
+  b.dA2 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h2);
+  b.dA2t = c.take<bf16_t>(static_cast<int64_t>(P) * D.h2);
+  b.H1t = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
+  b.dA1 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
+  b.dA1t = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
+  b.Xbt = c.take<bf16_t>(static_cast<int64_t>(P) * D.dp);
+  b.Z = c.take<float>(static_cast<int64_t>(P) * D.dout);
+  b.Yb = c.take<float>(static_cast<int64_t>(P) * D.dout);
+  b.dZ = c.take<float>(static_cast<int64_t>(P) * D.dout);
+  b.grad = c.take<float>(np);
+  b.loss = c.take<float>(2);
+  b.idx = c.take<int32_t>(P);
+  if (total) *total = align_up(c.off, 256) + 256;   // a multiple of 256: data placed after it stays aligned
+  return b;
+}
+
+template <int EPI>
+static int gemm(const GemmArgs& g, hipStream_t s) {
+  dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
+  hipLaunchKernelGGL((k_gemm_nt<EPI>), grid, dim3(NT), 0, s, g);
+  MV_LAUNCHED("k_gemm_nt");
+  return MV_OK;
+}
+
+static int transpose(const bf16_t* in, int rows, int cols, int64_t ldi, bf16_t* out, int64_t ldo, hipStream_t s) {
+  dim3 grid(static_cast<unsigned>(ceil_div(cols, 64)), static_cast<unsigned>(ceil_div(rows, 64)));
+  hipLaunchKernelGGL(k_transpose_bf16, grid, dim3(256), 0, s, in, rows, cols, ldi, out, ldo);
+  MV_LAUNCHED("k_transpose_bf16");
+  return MV_OK;
+}
+
+// Forward of P rows already in Xb (BF16, stride dp): A1/H1, A2/H2, Z.
+static int forward(const Params& W, const Dims& D, const bf16_t* Xb, int P, bf16_t* A1, bf16_t* H1, bf16_t* A2,
+                   bf16_t* H2, float* Z, hipStream_t s) {
+  GemmArgs g{};
+  g.A = Xb;
+  g.B = W.W1b;
+  g.M = P;
+  g.N = D.h1;
+  g.K = D.dp;
+  g.lda = D.dp;
+  g.ldb = D.dp;
+  g.C = H1;
+  g.ldc = D.h1;
+  g.bias = W.b1;
+  g.pre_out = A1;
+  g.ldx = D.h1;
+  if (int rc = gemm<EPI_BIAS_GELU>(g, s)) return rc;
+  g.A = H1;
+  g.B = W.W2b;
+  g.N = D.h2;
+  g.K = D.h1;
+  g.lda = D.h1;
+  g.ldb = D.h1;
+  g.C = H2;
+  g.ldc = D.h2;
+  g.bias = W.b2;
+  g.pre_out = A2;
+  g.ldx = D.h2;
+  if (int rc = gemm<EPI_BIAS_GELU>(g, s)) return rc;
+  hipLaunchKernelGGL(k_head_fwd, dim3(static_cast<unsigned>(ceil_div(P, 4))), dim3(256), 0, s, H2, P, D.h2, W.W3,
+                     W.b3, D.dout, Z);
+  MV_LAUNCHED("k_head_fwd");
+  return MV_OK;
+}
+
+// Backward from dZ into the gradient buffer (zeroed here).
+static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64_t np, hipStream_t s) {
+  float* gW1 = b.grad;
+  float* gb1 = gW1 + static_cast<int64_t>(D.h1) * D.dp;
+  float* gW2 = gb1 + D.h1;
+  float* gb2 = gW2 + static_cast<int64_t>(D.h2) * D.h1;
+  float* gW3 = gb2 + D.h2;
+  float* gb3 = gW3 + static_cast<int64_t>(D.dout) * D.h2;
+  hipLaunchKernelGGL(k_fill_f32, dim3(static_cast<unsigned>(ceil_div(np, 256))), dim3(256), 0, s, b.grad, np, 0.f);
+  MV_LAUNCHED("k_fill_f32");
+  {
+    const dim3 grid(static_cast<unsigned>(ceil_div(P, 4 * HEAD_PPW)));
+    switch (D.dout) {
+      case 1: hipLaunchKernelGGL(k_head_bwd<1>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
+      case 2: hipLaunchKernelGGL(k_head_bwd<2>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
+      case 3: hipLaunchKernelGGL(k_head_bwd<3>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
+      default: hipLaunchKernelGGL(k_head_bwd<4>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
+    }
+    MV_LAUNCHED("k_head_bwd");
+  }
+  if (int rc = transpose(b.dA2, P, D.h2, D.h2, b.dA2t, P, s)) return rc;
+  if (int rc = transpose(b.H1, P, D.h1, D.h1, b.H1t, P, s)) return rc;
+  GemmArgs g{};
+  // gW2 (h2 x h1) = dA2^T H1
+  g.A = b.dA2t;
+  g.B = b.H1t;
+  g.M = D.h2;
+  g.N = D.h1;
+  g.K = P;
+  g.lda = P;
+  g.ldb = P;
+  g.C = gW2;
+  g.ldc = D.h1;
+  if (int rc = gemm<EPI_F32>(g, s)) return rc;
+  // dA1 (P x h1) = (dA2 W2) * gelu'(A1) = dA2 (W2^T)^T
+  g = GemmArgs{};
+  g.A = b.dA2;
+  g.B = W.W2tb;
+  g.M = P;
+  g.N = D.h1;
+  g.K = D.h2;
+  g.lda = D.h2;
+  g.ldb = D.h2;
+  g.C = b.dA1;
+  g.ldc = D.h1;
+  g.pre_in = b.A1;
+  g.ldx = D.h1;
+  g.colsum = gb1;
+  if (int rc = gemm<EPI_GELU_GRAD>(g, s)) return rc;
+  if (int rc = transpose(b.dA1, P, D.h1, D.h1, b.dA1t, P, s)) return rc;
+  if (int rc = transpose(b.Xb, P, D.dp, D.dp, b.Xbt, P, s)) return rc;
+  // gW1 (h1 x dp) = dA1^T Xb
+  g = GemmArgs{};
+  g.A = b.dA1t;
+  g.B = b.Xbt;
+  g.M = D.h1;
+  g.N = D.dp;
+  g.K = P;
+  g.lda = P;
+  g.ldb = P;
+  g.C = gW1;
+  g.ldc = D.dp;
+  return gemm<EPI_F32>(g, s);
+}
+
+static int adam_all(const Params& W, const Dims& D, const float* grad, int64_t np, float lr, int step, hipStream_t s) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.f - powf(b1, static_cast<float>(step));
+  const float c2 = 1.f - powf(b2, static_cast<float>(step));
+  // W1, b1 (BF16 copy of W1)
+  const int64_t nW1 = static_cast<int64_t>(D.h1) * D.dp;
+  const int64_t nW2 = static_cast<int64_t>(D.h2) * D.h1;
+  const int64_t o_b1 = nW1, o_W2 = o_b1 + D.h1, o_b2 = o_W2 + nW2, o_W3 = o_b2 + D.h2;
+  auto launch = [&](int64_t off, int64_t cnt, bf16_t* bc, bf16_t* bt, int rows, int cols) -> int {
+    hipLaunchKernelGGL(k_adam, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0, s, W.W1 + off,
+                       grad + off, W.m + off, W.v + off, cnt, lr, b1, b2, eps, c1, c2, bc, bt, rows, cols);
+    MV_LAUNCHED("k_adam");
+    return MV_OK;
+  };
+  if (int rc = launch(0, nW1, W.W1b, nullptr, D.h1, D.dp)) return rc;
+  if (int rc = launch(o_b1, D.h1, nullptr, nullptr, 1, D.h1)) return rc;
+  if (int rc = launch(o_W2, nW2, W.W2b, W.W2tb, D.h2, D.h1)) return rc;
+  if (int rc = launch(o_b2, np - o_b2, nullptr, nullptr, 1, static_cast<int>(np - o_b2))) return rc;
+  (void)o_W3;
+  return MV_OK;
+}
+
+static int check_dims(int din, int h1, int h2, int dout) {
+  MV_REQUIRE(din >= 1 && din <= 65536 && h1 >= 64 && h2 >= 64 && h1 % 64 == 0 && h2 % 64 == 0 && h1 <= HMAX &&
+                 h2 <= HMAX && dout >= 1 && dout <= 4,
+             "mlp: unsupported dims din=%d h1=%d h2=%d dout=%d (hidden: multiples of 64 up to %d; dout <= 4)", din,
+             h1, h2, dout, HMAX);
+  return MV_OK;
+}
+
+static Dims make_dims(int din, int h1, int h2, int dout) {
+  Dims D;
+  D.din = din;
+  D.dp = static_cast<int>(align_up(din, 64));
+  D.h1 = h1;
+  D.h2 = h2;
+  D.dout = dout;
+  return D;
+}
+
+}  // namespace mlp
+}  // namespace mv
+
+using namespace mv;
+using namespace mv::mlp;
+
+extern "C" {
+
+int mv_mlp_param_bytes(int din, int h1, int h2, int dout, size_t* out_bytes, int64_t* out_n_params) {
+  MV_REQUIRE(out_bytes, "mv_mlp_param_bytes: null output");
+  if (int rc = check_dims(din, h1, h2, dout)) return rc;
+  Dims D = make_dims(din, h1, h2, dout);
+  size_t total = 0;
+  carve_params(nullptr, D, &total);
+  *out_bytes = total;
+  if (out_n_params) *out_n_params = n_params(D);
+  return MV_OK;
+}
+
+int mv_mlp_batch_bytes(int din, int h1, int h2, int dout, int points, size_t* out_bytes) {
+  MV_REQUIRE(out_bytes, "mv_mlp_batch_bytes: null output");
+  if (int rc = check_dims(din, h1, h2, dout)) return rc;
+  MV_REQUIRE(points >= 1, "mv_mlp_batch_bytes: points < 1");
+  Dims D = make_dims(din, h1, h2, dout);
+  size_t total = 0;
+  carve_batch(nullptr, D, points, n_params(D), &total);
+  *out_bytes = total;
+  return MV_OK;
+}
+
+int mv_mlp_init(void* blob, int din, int h1, int h2, int dout, const float* W1, const float* b1, const float* W2,
+                const float* b2, const float* W3, const float* b3, void* stream) {
+  MV_REQUIRE(blob && W1 && b1 && W2 && b2 && W3 && b3, "mv_mlp_init: null pointer");
+  if (int rc = check_dims(din, h1, h2, dout)) return rc;
+  MV_FAULT_POINT("launch");
+  Dims D = make_dims(din, h1, h2, dout);
+  Params W = carve_params(blob, D, nullptr);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t np = W.n_f32;
+  for (float* region : {W.W1, W.m, W.v}) {
+    hipLaunchKernelGGL(k_fill_f32, dim3(static_cast<unsigned>(ceil_div(np, 256))), dim3(256), 0, s, region, np, 0.f);
+    MV_LAUNCHED("k_fill_f32");
+  }
+  // W1 rows are padded from din to dp columns (zeros).
+  MV_TRY_HIP(hipMemcpy2DAsync(W.W1, sizeof(float) * D.dp, W1, sizeof(float) * din, sizeof(float) * din, h1,
+                              hipMemcpyDeviceToDevice, s));
+  MV_TRY_HIP(hipMemcpyAsync(W.b1, b1, sizeof(float) * h1, hipMemcpyDeviceToDevice, s));
+  MV_TRY_HIP(hipMemcpyAsync(W.W2, W2, sizeof(float) * h2 * h1, hipMemcpyDeviceToDevice, s));
+  MV_TRY_HIP(hipMemcpyAsync(W.b2, b2, sizeof(float) * h2, hipMemcpyDeviceToDevice, s));
+  MV_TRY_HIP(hipMemcpyAsync(W.W3, W3, sizeof(float) * dout * h2, hipMemcpyDeviceToDevice, s));
+  MV_TRY_HIP(hipMemcpyAsync(W.b3, b3, sizeof(float) * dout, hipMemcpyDeviceToDevice, s));
+  const int64_t nW1 = static_cast<int64_t>(h1) * D.dp, nW2 = static_cast<int64_t>(h2) * h1;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(static_cast<unsigned>(ceil_div(nW1, 256))), dim3(256), 0, s, W.W1, W.W1b, nW1);
+  MV_LAUNCHED("k_f32_to_bf16");
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(static_cast<unsigned>(ceil_div(nW2, 256))), dim3(256), 0, s, W.W2, W.W2b, nW2);
+  MV_LAUNCHED("k_f32_to_bf16");
+  hipLaunchKernelGGL(k_f32_to_bf16_t, dim3(static_cast<unsigned>(ceil_div(nW2, 256))), dim3(256), 0, s, W.W2, h2, h1,
+                     W.W2tb);
+  MV_LAUNCHED("k_f32_to_bf16_t");
+  return MV_OK;
+}
+
+// One forward/backward on a batch of B edges (rows idx[0:2B] of X, heads
+// then tails), writing the FP32 gradient (parameter layout of the blob) into
+// grad_out and the two loss terms into loss_out (both device, nullable).
+// Used to check gradients; mv_mlp_train runs the same step with Adam.
+int mv_mlp_grad(const void* blob, int din, int h1, int h2, int dout, const float* X, int64_t n, const float* Yp,
+                const int32_t* idx, int B, int neg, float a, float b, float lam, float* grad_out, float* loss_out,
+                void* ws, size_t ws_bytes, void* stream) {
+  MV_REQUIRE(blob && X && Yp && idx, "mv_mlp_grad: null pointer");
+  if (int rc = check_dims(din, h1, h2, dout)) return rc;
+  MV_REQUIRE(B >= 32 && B % 32 == 0 && neg >= 0 && n >= 1 && n <= INT32_MAX,
+             "mv_mlp_grad: batch B=%d must be a positive multiple of 32 (n=%lld)", B, (long long)n);
+  Dims D = make_dims(din, h1, h2, dout);
+  const int P = 2 * B;
+  const int64_t np = n_params(D);
+  size_t need = 0;
+  carve_batch(nullptr, D, P, np, &need);
+  const size_t xbytes = align_up(static_cast<size_t>(n) * D.dp * 2, 256);
+  if (!ws || ws_bytes < need + xbytes) {
+    mv::set_error("mv_mlp_grad: workspace of %zu bytes < required %zu", ws_bytes, need + xbytes);
+    return MV_EWORKSPACE;
+  }
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Params W = carve_params(const_cast<void*>(blob), D, nullptr);
+  Batch bt = carve_batch(ws, D, P, np, nullptr);
+  bf16_t* Xbf = reinterpret_cast<bf16_t*>(static_cast<char*>(ws) + need);
+  hipLaunchKernelGGL(k_to_bf16, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, X, n, din, D.dp, Xbf);
+  MV_LAUNCHED("k_to_bf16");
+  hipLaunchKernelGGL(k_gather_x, dim3(static_cast<unsigned>(ceil_div(P, 4))), dim3(256), 0, s, Xbf, D.dp, idx, P,
+                     bt.Xb, Yp, dout, bt.Yb);
+  MV_LAUNCHED("k_gather_x");
+  if (int rc = forward(W, D, bt.Xb, P, bt.A1, bt.H1, bt.A2, bt.H2, bt.Z, s)) return rc;
+  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, bt.loss, static_cast<int64_t>(2), 0.f);
+  MV_LAUNCHED("k_fill_f32");
+  hipLaunchKernelGGL(k_loss_grad, dim3(static_cast<unsigned>(ceil_div(B, 256))), dim3(256), 0, s, bt.Z, bt.Yb, B, dout,
+                     neg, a, b, lam, bt.dZ, bt.loss);
+  MV_LAUNCHED("k_loss_grad");
+  if (int rc = backward(W, D, bt, P, np, s)) return rc;
+  if (grad_out) MV_TRY_HIP(hipMemcpyAsync(grad_out, bt.grad, sizeof(float) * np, hipMemcpyDeviceToDevice, s));
+  if (loss_out) MV_TRY_HIP(hipMemcpyAsync(loss_out, bt.loss, sizeof(float) * 2, hipMemcpyDeviceToDevice, s));
+  return MV_OK;
+}
+
+// Trains for `steps` Adam steps; batch t uses idx[t * 2B : (t + 1) * 2B]
+// (heads then tails, rows of X / Yp). loss_trace (device, nullable, 2 per
+// step) receives the loss terms of every step.
+int mv_mlp_train(void* blob, int din, int h1, int h2, int dout, const float* X, int64_t n, const float* Yp,
+                 const int32_t* idx, int steps, int B, int neg, float a, float b, float lam, float lr, int step0,
+                 float* loss_trace, void* ws, size_t ws_bytes, void* stream) {
+  MV_REQUIRE(blob && X && Yp && idx, "mv_mlp_train: null pointer");
+  if (int rc = check_dims(din, h1, h2, dout)) return rc;
+  MV_REQUIRE(B >= 32 && B % 32 == 0 && neg >= 0 && n >= 1 && n <= INT32_MAX && steps >= 0 && step0 >= 0,
+             "mv_mlp_train: batch B=%d must be a positive multiple of 32 (n=%lld, steps=%d)", B, (long long)n, steps);
+  Dims D = make_dims(din, h1, h2, dout);
+  const int P = 2 * B;
+  const int64_t np = n_params(D);
+  size_t need = 0;
+  carve_batch(nullptr, D, P, np, &need);
+  const size_t xbytes = align_up(static_cast<size_t>(n) * D.dp * 2, 256);
+  if (!ws || ws_bytes < need + xbytes) {
+    mv::set_error("mv_mlp_train: workspace of %zu bytes < required %zu", ws_bytes, need + xbytes);
+    return MV_EWORKSPACE;
+  }
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Params W = carve_params(blob, D, nullptr);
+  Batch bt = carve_batch(ws, D, P, np, nullptr);
+  bf16_t* Xbf = reinterpret_cast<bf16_t*>(static_cast<char*>(ws) + need);
+  hipLaunchKernelGGL(k_to_bf16, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, X, n, din, D.dp, Xbf);
+  MV_LAUNCHED("k_to_bf16");
+  for (int t = 0; t < steps; ++t) {
+    hipLaunchKernelGGL(k_gather_x, dim3(static_cast<unsigned>(ceil_div(P, 4))), dim3(256), 0, s, Xbf, D.dp,
+                       idx + static_cast<int64_t>(t) * P, P, bt.Xb, Yp, dout, bt.Yb);
+    MV_LAUNCHED("k_gather_x");
+    if (int rc = forward(W, D, bt.Xb, P, bt.A1, bt.H1, bt.A2, bt.H2, bt.Z, s)) return rc;
+    float* lo = loss_trace ? loss_trace + 2 * static_cast<int64_t>(t) : nullptr;
+    if (lo) {
+      hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, lo, static_cast<int64_t>(2), 0.f);
+      MV_LAUNCHED("k_fill_f32");
+    }
+    hipLaunchKernelGGL(k_loss_grad, dim3(static_cast<unsigned>(ceil_div(B, 256))), dim3(256), 0, s, bt.Z, bt.Yb, B,
+                       dout, neg, a, b, lam, bt.dZ, lo);
+    MV_LAUNCHED("k_loss_grad");
+    if (int rc = backward(W, D, bt, P, np, s)) return rc;
+    if (int rc = adam_all(W, D, bt.grad, np, lr, step0 + t + 1, s)) return rc;
+  }
+  return MV_OK;
+}
+
+// Inference: Z (m x dout) for rows of X (m x din FP32), in chunks of `chunk`
+// rows (ws sized by mv_mlp_batch_bytes(..., chunk) plus chunk * dp * 2 bytes).
+int mv_mlp_forward(const void* blob, int din, int h1, int h2, int dout, const float* X, int64_t m, float* Z,
+                   int chunk, void* ws, size_t ws_bytes, void* stream) {
+  MV_REQUIRE(blob && X && Z, "mv_mlp_forward: null pointer");
+  if (int rc = check_dims(din, h1, h2, dout)) return rc;
+  MV_REQUIRE(m >= 0 && chunk >= 1, "mv_mlp_forward: bad sizes");
+  Dims D = make_dims(din, h1, h2, dout);
+  const int64_t np = n_params(D);
+  size_t need = 0;
+  carve_batch(nullptr, D, chunk, np, &need);
+  if (!ws || ws_bytes < need) {
+    mv::set_error("mv_mlp_forward: workspace of %zu bytes < required %zu", ws_bytes, need);
+    return MV_EWORKSPACE;
+  }
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Params W = carve_params(const_cast<void*>(blob), D, nullptr);
+  Batch bt = carve_batch(ws, D, chunk, np, nullptr);
+  for (int64_t r0 = 0; r0 < m; r0 += chunk) {
+    const int rows = static_cast<int>(m - r0 < chunk ? m - r0 : chunk);
+    hipLaunchKernelGGL(k_to_bf16, dim3(static_cast<unsigned>(ceil_div(rows, 4))), dim3(256), 0, s, X + r0 * din,
+                       static_cast<int64_t>(rows), din, D.dp, bt.Xb);
+    MV_LAUNCHED("k_to_bf16");
+    // inference keeps no pre-activations
+    if (int rc = forward(W, D, bt.Xb, rows, nullptr, bt.H1, nullptr, bt.H2, Z + r0 * dout, s)) return rc;
+  }
+  return MV_OK;
+}
+
+int mv_mlp_sample_edges(const float* eps, const int32_t* tail, int64_t n, int k, uint32_t seed, int step0, int steps,
+                        int B, int32_t* out_idx, void* stream) {
+  MV_REQUIRE(eps && tail && out_idx, "mv_mlp_sample_edges: null pointer");
+  MV_REQUIRE(n >= 1 && k >= 1 && n * k < (int64_t(1) << 32) && steps >= 0 && B >= 1 && step0 >= 0,
+             "mv_mlp_sample_edges: bad sizes n=%lld k=%d steps=%d B=%d", (long long)n, k, steps, B);
+  MV_FAULT_POINT("launch");
+  const int64_t count = static_cast<int64_t>(steps) * B;
+  if (count == 0) return MV_OK;
+  hipLaunchKernelGGL(k_sample_edges, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), eps, tail, n * k, k, seed, static_cast<uint32_t>(step0), steps,
+                     B, out_idx);
+  MV_LAUNCHED("k_sample_edges");
+  return MV_OK;
+}
+
+int mv_knn_overlap(const int32_t* a, int64_t lda, int ka, const int32_t* b, int64_t ldb, int kb, int64_t n,
+                   int min_count, int32_t* out_count, uint8_t* out_flag, void* stream) {
+  MV_REQUIRE(a && b && (out_count || out_flag), "mv_knn_overlap: null pointer");
+  MV_REQUIRE(n >= 0 && ka >= 0 && kb >= 0 && lda >= ka && ldb >= kb, "mv_knn_overlap: bad sizes");
+  MV_FAULT_POINT("launch");
+  if (n == 0) return MV_OK;
+  hipLaunchKernelGGL(k_knn_overlap, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), a, lda, ka, b, ldb, kb, n, min_count, out_count, out_flag);
+  MV_LAUNCHED("k_knn_overlap");
+  return MV_OK;
+}
+
+}  // extern "C"

@@ -177,35 +177,6 @@ struct Law {
 
 __device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 4.0f); }
 
-// Philox-4x32-10.
-struct Philox {
-  __device__ static __forceinline__ void round(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
-                                               uint32_t k0, uint32_t k1) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    const uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
-    const uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
-    const uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
-    c0 = n0;
-    c1 = n1;
-    c2 = n2;
-    c3 = n3;
-  }
-  __device__ static __forceinline__ uint4 gen(uint4 ctr, uint32_t k0, uint32_t k1) {
-    uint32_t c0 = ctr.x, c1 = ctr.y, c2 = ctr.z, c3 = ctr.w;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-      round(c0, c1, c2, c3, k0, k1);
-      k0 += 0x9E3779B9u;
-      k1 += 0xBB67AE85u;
-    }
-    return make_uint4(c0, c1, c2, c3);
-  }
-};
-
-__device__ __forceinline__ uint32_t uniform_below(uint32_t r, uint32_t n) {
-  return static_cast<uint32_t>((static_cast<uint64_t>(r) * n) >> 32);
-}
-
 // One synchronous epoch, phase 1, over the dense slot array. Lanes are
 // grouped per head row (G lanes, G = next power of two >= k). An active edge
 // issues all its gathers (head, tail and up to 8 negatives drawn from
@@ -357,11 +328,14 @@ __global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtai
 }
 
 // Phase 3: Y += dHead + dTail; dTail = 0.
+// Y += dHead + dTail, then dTail = 0. With `movable` set, rows whose flag is
+// 0 keep their position (their gradients are dropped).
 __global__ __launch_bounds__(256) void k_apply2(float* __restrict__ Y, const float* __restrict__ dHead,
-                                                float* __restrict__ dTail, int64_t count) {
+                                                float* __restrict__ dTail, int64_t count, int dim,
+                                                const uint8_t* __restrict__ movable) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t < count) {
-    Y[t] += dHead[t] + dTail[t];
+    if (!movable || movable[t / dim]) Y[t] += dHead[t] + dTail[t];
     dTail[t] = 0.f;
   }
 }
@@ -733,7 +707,7 @@ static int launch_epoch(const float* Y, const EpochBufs& B, const int32_t* tail,
 template <int D>
 static int run_epoch(float* Y, const EpochBufs& B, const int32_t* tail, const float* eps, float* epn,
                      const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int64_t n, int k, int epoch,
-                     int neg_rate, Law law, uint64_t seed, hipStream_t s) {
+                     int neg_rate, Law law, uint64_t seed, const uint8_t* movable, hipStream_t s) {
   int rc;
   if (k <= 8) rc = launch_epoch<D, 8>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
   else if (k <= 16) rc = launch_epoch<D, 16>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
@@ -746,7 +720,7 @@ static int run_epoch(float* Y, const EpochBufs& B, const int32_t* tail, const fl
     MV_LAUNCHED("k_tail_sum");
   }
   hipLaunchKernelGGL(k_apply2, dim3(static_cast<unsigned>(ceil_div(n * D, 256))), dim3(256), 0, s, Y, B.dHead,
-                     B.dTail, n * D);
+                     B.dTail, n * D, D, movable);
   MV_LAUNCHED("k_apply2");
   return MV_OK;
 }
@@ -841,10 +815,10 @@ int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes) {
   return MV_OK;
 }
 
-int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
-                   const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1, int n_epochs,
-                   int neg_rate, float lr, float a, float b, uint64_t seed, void* ws, size_t ws_bytes,
-                   void* stream) {
+int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
+                          const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1,
+                          int n_epochs, int neg_rate, float lr, float a, float b, uint64_t seed,
+                          const uint8_t* movable, void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(Y && tail && eps && epn, "mv_umap_epochs: null pointer");
   MV_REQUIRE(n_in == 0 || (in_slots && in_node), "mv_umap_epochs: null in-edge list");
   MV_REQUIRE(n >= 1 && k >= 1 && k <= 64, "mv_umap_epochs: bad n=%lld k=%d", (long long)n, k);
@@ -871,14 +845,22 @@ int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, con
     law.alpha = static_cast<float>(static_cast<double>(lr) * (1.0 - static_cast<double>(ep) / n_epochs));
     int rc = MV_OK;
     switch (dim) {
-      case 1: rc = run_epoch<1>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
-      case 2: rc = run_epoch<2>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
-      case 3: rc = run_epoch<3>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
-      default: rc = run_epoch<4>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, s); break;
+      case 1: rc = run_epoch<1>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 2: rc = run_epoch<2>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 3: rc = run_epoch<3>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      default: rc = run_epoch<4>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
     }
     if (rc != MV_OK) return rc;
   }
   return MV_OK;
+}
+
+int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
+                   const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1, int n_epochs,
+                   int neg_rate, float lr, float a, float b, uint64_t seed, void* ws, size_t ws_bytes,
+                   void* stream) {
+  return mv_umap_epochs_masked(Y, n, dim, tail, k, eps, epn, in_slots, in_node, n_in, e0, e1, n_epochs, neg_rate, lr,
+                               a, b, seed, nullptr, ws, ws_bytes, stream);
 }
 
 int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes) {

@@ -11,5 +11,7 @@ from ._knn import compute_knn, knn_brute_device
 from ._normalize import normalize_input
 from ._nndescent import NNDescent
 from ._umap import UMAP, find_ab_params
+from ._parametric import ParametricUMAP, knn_preservation_device
 
-__all__ = ["UMAP", "NNDescent", "compute_knn", "knn_brute_device", "normalize_input", "find_ab_params"]
+__all__ = ["UMAP", "NNDescent", "ParametricUMAP", "compute_knn", "knn_brute_device", "knn_preservation_device",
+           "normalize_input", "find_ab_params"]

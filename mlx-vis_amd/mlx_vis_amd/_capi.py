@@ -40,6 +40,7 @@ _i64 = C.c_int64
 _i32 = C.c_int
 _f32 = C.c_float
 _u64 = C.c_uint64
+_u32 = C.c_uint32
 _sz = C.c_size_t
 
 # name -> argtypes (restype is always int unless listed in _RESTYPES)
@@ -67,6 +68,17 @@ _SIGNATURES: dict[str, list] = {
     "mv_nnd_gather_dists": [_p, _i64, _i64, _p, _i32, _i32, _p, _p, _sz, _p],
     "mv_umap_spectral_ws_bytes": [_i64, _i32, _i32, C.POINTER(_sz)],
     "mv_umap_spectral": [_p, _p, _p, _i64, _i32, _i32, _i32, _p, _p, _sz, _p],
+    "mv_umap_epochs_masked": [_p, _i64, _i32, _p, _i32, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _i32, _f32, _f32,
+                              _f32, _u64, _p, _p, _sz, _p],
+    "mv_mlp_param_bytes": [_i32, _i32, _i32, _i32, C.POINTER(_sz), C.POINTER(_i64)],
+    "mv_mlp_batch_bytes": [_i32, _i32, _i32, _i32, _i32, C.POINTER(_sz)],
+    "mv_mlp_init": [_p, _i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p],
+    "mv_mlp_grad": [_p, _i32, _i32, _i32, _i32, _p, _i64, _p, _p, _i32, _i32, _f32, _f32, _f32, _p, _p, _p, _sz, _p],
+    "mv_mlp_train": [_p, _i32, _i32, _i32, _i32, _p, _i64, _p, _p, _i32, _i32, _i32, _f32, _f32, _f32, _f32, _i32,
+                     _p, _p, _sz, _p],
+    "mv_mlp_forward": [_p, _i32, _i32, _i32, _i32, _p, _i64, _p, _i32, _p, _sz, _p],
+    "mv_mlp_sample_edges": [_p, _p, _i64, _i32, _u32, _i32, _i32, _i32, _p, _p],
+    "mv_knn_overlap": [_p, _i64, _i32, _p, _i64, _i32, _i64, _i32, _p, _p, _p],
 }
 _RESTYPES = {"mv_last_error": C.c_char_p}
 

@@ -119,16 +119,26 @@ def epochs_workspace(n: int, k: int, dim: int) -> torch.Tensor:
 
 
 def run_epochs(Y: torch.Tensor, ws: torch.Tensor | None, g: FuzzyGraph, e0: int, e1: int, n_epochs: int,
-               neg_rate: int, lr: float, a: float, b: float, seed: int) -> None:
-    """Runs layout epochs [e0, e1) in place on Y (ws from epochs_workspace, or None)."""
+               neg_rate: int, lr: float, a: float, b: float, seed: int,
+               movable: torch.Tensor | None = None) -> None:
+    """Runs layout epochs [e0, e1) in place on Y (ws from epochs_workspace, or None).
+
+    movable: optional (n,) uint8 device mask; nodes with 0 keep their position.
+    """
     n, dim = Y.shape
     if ws is None:
         ws = epochs_workspace(n, g.k, dim)
-    _capi.check(_capi.lib().mv_umap_epochs(
-        Y.data_ptr(), n, dim, g.knn_idx.data_ptr(), g.k, g.eps.data_ptr(), g.epn.data_ptr(),
-        g.in_slots.data_ptr(), g.in_node.data_ptr(), int(g.n_edges), int(e0), int(e1), int(n_epochs),
-        int(neg_rate), float(lr), float(a), float(b), int(seed) & 0xFFFFFFFFFFFFFFFF, ws.data_ptr(), ws.numel(),
-        _device.stream()), "mv_umap_epochs")
+    args = (Y.data_ptr(), n, dim, g.knn_idx.data_ptr(), g.k, g.eps.data_ptr(), g.epn.data_ptr(),
+            g.in_slots.data_ptr(), g.in_node.data_ptr(), int(g.n_edges), int(e0), int(e1), int(n_epochs),
+            int(neg_rate), float(lr), float(a), float(b), int(seed) & 0xFFFFFFFFFFFFFFFF)
+    lib = _capi.lib()
+    if movable is None:
+        _capi.check(lib.mv_umap_epochs(*args, ws.data_ptr(), ws.numel(), _device.stream()), "mv_umap_epochs")
+    else:
+        if movable.dtype != torch.uint8 or movable.numel() != n or not movable.is_cuda:
+            raise ValueError("movable must be a (n,) uint8 device tensor")
+        _capi.check(lib.mv_umap_epochs_masked(*args, movable.contiguous().data_ptr(), ws.data_ptr(), ws.numel(),
+                                              _device.stream()), "mv_umap_epochs_masked")
 
 
 def sgd_step_device(Y: torch.Tensor, ef, et, neg_from, neg_to, alpha: float, a: float, b: float) -> torch.Tensor:

@@ -36,8 +36,8 @@ def test_reverse_candidates_bit_exact():
     idx[:50] = 7  # a hub with far more than k in-edges
     ws = _ws(n, 1, k)
     rev = torch.empty((n, k), dtype=torch.int32, device="cuda")
-    _capi.check(_capi.lib().mv_nnd_reverse(torch.from_numpy(idx).cuda().data_ptr(), n, k, rev.data_ptr(),
-                                           ws.data_ptr(), ws.numel(), None))
+    idx_d = torch.from_numpy(idx).cuda()
+    _capi.check(_capi.lib().mv_nnd_reverse(idx_d.data_ptr(), n, k, rev.data_ptr(), ws.data_ptr(), ws.numel(), None))
     np.testing.assert_array_equal(rev.cpu().numpy(), onn.reverse_candidates(idx))
 
 
@@ -51,8 +51,8 @@ def test_gather_dists(half):
     idx = rng.integers(0, n, size=(n, k)).astype(np.int32)
     ws = _ws(n, d, k)
     out = torch.empty((n, k), dtype=torch.float32, device="cuda")
-    _capi.check(_capi.lib().mv_nnd_gather_dists(torch.from_numpy(X).cuda().data_ptr(), n, d,
-                                                torch.from_numpy(idx).cuda().data_ptr(), k, half, out.data_ptr(),
+    X_d, idx_d = torch.from_numpy(X).cuda(), torch.from_numpy(idx).cuda()   # kept alive across the call
+    _capi.check(_capi.lib().mv_nnd_gather_dists(X_d.data_ptr(), n, d, idx_d.data_ptr(), k, half, out.data_ptr(),
                                                 ws.data_ptr(), ws.numel(), None))
     got = out.cpu().numpy()
     sq = np.einsum("ij,ij->i", X, X).astype(np.float32)
