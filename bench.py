@@ -32,6 +32,20 @@ for _p in (ROOT, ROOT / "mlx-vis_amd"):
 
 MFMA_FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+SCREEN_PROFILE = ROOT / "profiles" / "k_screen_f32_70000x784.md"
+
+
+def screen_traffic(n, d):
+    """HBM bytes per k_screen launch at 70,000 x 784, as measured by two
+    rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1, factors from a 1 GiB
+    copy) and recorded with their commands in the profile summary; None for
+    other shapes or when the summary is absent."""
+    if (n, d) != (70_000, 784) or not SCREEN_PROFILE.exists():
+        return None
+    for line in SCREEN_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(float(line.split("=")[1]))
+    return None
 
 
 def parse():
@@ -46,7 +60,8 @@ def parse():
     ap.add_argument("--learned-n", type=int, default=1_000_000,
                     help="fresh points mapped by the learned form (default 1,000,000)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form line")
-    ap.add_argument("--only", choices=["knn", "copy", "learned"], default=None,
+    ap.add_argument("--no-scale", action="store_true", help="skip the 1M x 128 NNDescent + UMAP line")
+    ap.add_argument("--only", choices=["knn", "copy", "learned", "scale"], default=None,
                     help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8"])
     ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
@@ -195,6 +210,69 @@ def learned_form(Xd, um, fresh_n, reps=5):
     return out
 
 
+def scale_form(n=1_000_000, sample=10_000):
+    """1,000,000 x 128 mixture of 50 Gaussians (Dirichlet weights, seed 2):
+    NNDescent k=15, 12 iterations, recall on `sample` rows (drawn by
+    default_rng(0) without replacement) against exact FP32 brute force over all
+    rows; then UMAP 2-D, 200 epochs, 5 negatives, NNDescent kNN."""
+    import numpy as np
+    import torch
+
+    from mlx_vis_amd import UMAP, NNDescent, knn_brute_device, knn_preservation_device
+    from mlx_vis_amd.datasets import gaussian_mixture_128
+
+    log(f"scale: generating the {n} x 128 mixture (seed 2)")
+    X = gaussian_mixture_128(n, seed=2)
+    Xd = torch.from_numpy(X).cuda()
+    del X
+    out = {}
+    nnd = NNDescent(k=15, n_iters=12, random_state=42)
+    nnd.build_device(Xd[:20000].contiguous())   # loads the kernels
+    torch.cuda.synchronize()
+    log("scale: NNDescent k=15, 12 iterations")
+    t = time.perf_counter()
+    gi, _ = nnd.build_device(Xd)
+    torch.cuda.synchronize()
+    out["nndescent_s"] = round(time.perf_counter() - t, 4)
+    out["nndescent_iterations"] = nnd.iterations_
+    rows = np.sort(np.random.default_rng(0).choice(n, sample, replace=False)).astype(np.int64)
+    rows_d = torch.from_numpy(rows).cuda()
+    Q = Xd.index_select(0, rows_d).contiguous()
+    t = time.perf_counter()
+    ex, _ = knn_brute_device(Xd, 15, Q=Q, q_self=rows_d.to(torch.int32))
+    torch.cuda.synchronize()
+    out["exact_sample_knn_s"] = round(time.perf_counter() - t, 4)
+    c, _ = knn_preservation_device(ex, gi.index_select(0, rows_d).contiguous())
+    out["recall_vs_exact"] = round(float(c.double().mean().item()) / 15, 4)
+    out["recall_sample_rows"] = sample
+    log("scale: UMAP 2-D, 200 epochs, NNDescent kNN")
+    um = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=200, negative_sample_rate=5, knn_method="nndescent",
+              random_state=42)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    um.fit_transform(Xd)
+    torch.cuda.synchronize()
+    out["fit_transform_s"] = round(time.perf_counter() - t, 4)
+    g = um.graph_
+    from mlx_vis_amd._umap import find_ab_params, run_epochs
+
+    a, b = find_ab_params(1.0, 0.1)
+    Y = torch.from_numpy(um.embedding_).cuda()
+    g.epn.copy_(g.eps.masked_fill(g.eps <= 0, float("inf")))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    run_epochs(Y, None, g, 1, 200, 200, 5, 1.0, a, b, 7)
+    torch.cuda.synchronize()
+    t_ep = time.perf_counter() - t
+    # every kept edge is active in epoch e for floor(e / eps) - floor((e - 1) / eps) samples
+    eps = g.eps[g.eps > 0].double()
+    samples = float(torch.floor((199.0 / eps) + 1e-9).sum().item())
+    out["layout"] = {"epochs_s": round(t_ep, 4), "kept_edges": g.n_edges,
+                     "edge_samples": int(samples), "edge_samples_per_s": round(samples / t_ep, 1)}
+    out["config"] = {"n": n, "d": 128, "k": 15, "nndescent_iters": 12, "n_epochs": 200, "negatives": 5}
+    return out
+
+
 def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs=8, spec_iters=5):
     """The oracle restatement of fit_transform on a bounded sample, scaled."""
     import numpy as np
@@ -288,6 +366,10 @@ def main():
                           "tflops_screen": 2.0 * n * n * d / (phases[1] * 1e-3) / 1e12}), flush=True)
         return
 
+    if args.only == "scale":
+        print(json.dumps({"scale": scale_form()}), flush=True)
+        return
+
     if args.only == "learned":
         um = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=args.epochs, negative_sample_rate=5,
                   knn_method="brute", random_state=42)
@@ -371,7 +453,7 @@ def main():
         "quality": {"knn15_preservation": round(quality, 4), "n_edges": g.n_edges},
         "roofline": {"kernel": "k_screen_f32", "bound": "mfma", "achieved": round(screen_tflops, 2),
                      "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": None},
+                     "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": screen_traffic(n, d)},
     }
     if not args.no_extras:
         prec = {}
@@ -401,6 +483,8 @@ def main():
                              "iterations": nnd.iterations_, "changed": nnd.changed_}
     if not args.no_extras and not args.no_learned:
         line["learned"] = learned_form(Xd, um, args.learned_n)
+    if not args.no_extras and not args.no_scale:
+        line["scale"] = scale_form()
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement, bounded sample)")
         line["cpu_baseline"] = cpu_baseline(X, um.knn_[0].cpu().numpy(), um.knn_[1].cpu().numpy(), args.epochs)
