@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--no-scale", action="store_true", help="skip the 1M x 128 NNDescent + UMAP line")
     ap.add_argument("--only", choices=["knn", "copy", "learned", "scale"], default=None,
                     help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
-    ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8"])
+    ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8", "bf16x3"])
     ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
     return ap.parse_args()
 
@@ -457,7 +457,7 @@ def main():
     }
     if not args.no_extras:
         prec = {}
-        for dt, rr in (("bf16", 2), ("bf16", 4), ("fp8", 2), ("fp8", 4)):
+        for dt, rr in (("bf16x3", 2), ("bf16", 2), ("bf16", 4), ("fp8", 2), ("fp8", 4)):
             log(f"kNN {dt} screen, rerank {rr}")
             ph_dt, st_dt, idx_dt = knn_phase_timing(Xd, args.knn_reps, dt, rr)
             tot = sum(ph_dt)

@@ -41,7 +41,9 @@ enum mv_status {
 enum mv_dtype {
   MV_F32 = 0,     /* FP32 MFMA distance screen */
   MV_BF16 = 1,    /* BF16 MFMA screen, FP32 accumulate */
-  MV_FP8 = 2      /* FP8 e4m3 MFMA screen, per-row absmax scale, FP32 accumulate */
+  MV_FP8 = 2,     /* FP8 e4m3 MFMA screen, per-row absmax scale, FP32 accumulate */
+  MV_BF16X3 = 3   /* split BF16 screen: x = hi + lo (two BF16 values), hi*hi + hi*lo + lo*hi
+                     on BF16 MFMA with FP32 accumulate (error ~2^-16 relative) */
 };
 
 int mv_abi_version(void);

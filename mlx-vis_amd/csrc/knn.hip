@@ -121,8 +121,12 @@ __global__ __launch_bounds__(256) void k_row_sqnorms(const float* __restrict__ X
   if (l == 0) out[r] = acc;
 }
 
-enum { DT_F32 = 0, DT_BF16 = 1, DT_FP8 = 2 };
-__host__ __device__ constexpr int elem_bytes(int dt) { return dt == DT_F32 ? 4 : dt == DT_BF16 ? 2 : 1; }
+enum { DT_F32 = 0, DT_BF16 = 1, DT_FP8 = 2, DT_BF16X3 = 3 };
+// Bytes per element of the screen operand (the split BF16 form stores two
+// BF16 values per element).
+__host__ __device__ constexpr int elem_bytes(int dt) {
+  return dt == DT_F32 || dt == DT_BF16X3 ? 4 : dt == DT_BF16 ? 2 : 1;
+}
 // Scale operand of the FP8 MFMA that leaves products unscaled.
 constexpr int FP8_UNIT_SCALE = 127;
 
@@ -190,6 +194,23 @@ __global__ __launch_bounds__(256) void k_center_q(const float* __restrict__ X, i
       atomicMax(errmax, __float_as_uint(eb));
     }
   }
+}
+
+// Split BF16 copy of the centred FP32 rows: v = hi + lo + r with hi = bf16(v),
+// lo = bf16(v - hi) (both subtractions exact in FP32), |r| <= 2^-16 |v|. Row
+// layout per 32-element group g (128 bytes): hi[32g .. 32g+31] then
+// lo[32g .. 32g+31], so one 128-byte chunk of the screen holds one K group.
+__global__ __launch_bounds__(256) void k_split_bf16(const float* __restrict__ Xc, int64_t count, int64_t d_pad,
+                                                    uint16_t* __restrict__ Xs) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t r = t / d_pad, c = t % d_pad;
+  const float v = Xc[t];
+  const uint16_t hi = bf16_rne(v);
+  const uint16_t lo = bf16_rne(v - bf16_to_f32(hi));
+  uint16_t* row = Xs + r * 2 * d_pad + (c >> 5) * 64 + (c & 31);
+  row[0] = hi;
+  row[32] = lo;
 }
 
 // Gathers the FP32 centred rows listed in rows[0:*count] (original query ids)
@@ -267,6 +288,29 @@ __device__ __forceinline__ void mfma_chunk(const float* As, const float* Bs, int
           for (int ni = 0; ni < 4; ++ni)
             acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mi][j], bf[ni][j], acc[mi][ni], 0, 0, 0);
     }
+  } else if constexpr (DT == DT_BF16X3) {
+    // one K group of 32: hi in floats [0, 16) of the row, lo in [16, 32)
+    v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const float* p = As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + (l >> 4) * 4;
+      ah[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p));
+      al[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p + 16));
+    }
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const float* p = Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + (l >> 4) * 4;
+      bh[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p));
+      bl[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p + 16));
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+      }
   } else if constexpr (DT == DT_BF16) {
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
@@ -309,10 +353,11 @@ __device__ __forceinline__ void mfma_chunk(const float* As, const float* Bs, int
 }
 
 template <int DT>
-__global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
+__global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
-  __shared__ uint64_t s_list[BM * CMAX];
   __shared__ uint64_t s_tau[BM];
+  __shared__ float s_tauf[BM];          // distance of s_tau (+inf while the list is not full)
+  __shared__ uint64_t s_hit[2][2];      // [column half][row half]: rows with survivors
   __shared__ float s_sqq[BM];
   __shared__ int s_self[BM];
   __shared__ float s_inv[BM];
@@ -345,8 +390,12 @@ __global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
     }
     s_self[r] = static_cast<int>(self);
     s_tau[r] = KEY_NONE;
+    s_tauf[r] = INFINITY;
   }
-  for (int e = tid; e < BM * CMAX; e += NT) s_list[e] = KEY_NONE;
+  // The candidate lists live in the output buffer (this workgroup's rows of
+  // its split, contiguous, L2-resident); only their C-th keys stay in LDS.
+  uint64_t* lists = a.cand + (static_cast<int64_t>(split) * a.nq_pad + static_cast<int64_t>(rb) * BM) * C;
+  for (int e = tid; e < BM * C; e += NT) lists[e] = KEY_NONE;
   __syncthreads();
 
   const int t0 = split * a.tiles_per_split;
@@ -366,57 +415,59 @@ __global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    // Two-deep register prefetch: while chunk kc is multiplied out of one LDS
-    // buffer, chunk kc + 1 waits in registers for the other buffer and chunk
-    // kc + 2 is in flight from memory. Chunk indices past the end are clamped
-    // (re-read, never used), keeping every load and store unconditional.
-    float4 a0_0, a0_1, a0_2, a0_3, b0_0, b0_1, b0_2, b0_3;   // register set 0
-    float4 a1_0, a1_1, a1_2, a1_3, b1_0, b1_1, b1_2, b1_3;   // register set 1
-#define MV_GLOAD(S, KCC)                                                                   \
+    // Register prefetch one chunk deep into a double-buffered LDS stage: while
+    // chunk kc is multiplied out of one buffer, chunk kc + 1 is in flight from
+    // memory and then stored to the other buffer (the co-resident workgroup of
+    // the CU hides the remaining latency). Chunk indices past the end are
+    // clamped (re-read, never used), keeping every load and store unconditional.
+    float4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define MV_GLOAD(KCC)                                                                      \
   do {                                                                                   \
     const int kq_ = (KCC) < nk ? (KCC) : nk - 1;                                         \
     const int64_t ko_ = static_cast<int64_t>(kq_) * 128 + lkk * 4;                       \
-    a##S##_0 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr) * rbytes + ko_);       \
-    a##S##_1 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32) * rbytes + ko_);  \
-    a##S##_2 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 64) * rbytes + ko_);  \
-    a##S##_3 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 96) * rbytes + ko_);  \
-    b##S##_0 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr) * rbytes + ko_);       \
-    b##S##_1 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32) * rbytes + ko_);  \
-    b##S##_2 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 64) * rbytes + ko_);  \
-    b##S##_3 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 96) * rbytes + ko_);  \
+    ra0 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr) * rbytes + ko_);        \
+    ra1 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32) * rbytes + ko_);   \
+    ra2 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 64) * rbytes + ko_);   \
+    ra3 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 96) * rbytes + ko_);   \
+    rb0 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr) * rbytes + ko_);        \
+    rb1 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32) * rbytes + ko_);   \
+    rb2 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 64) * rbytes + ko_);   \
+    rb3 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 96) * rbytes + ko_);   \
   } while (0)
-#define MV_LSTORE(S, BUF)                                                                  \
+#define MV_LSTORE(BUF)                                                                     \
   do {                                                                                   \
     float* As2_ = s_stage + (BUF) * (2 * BM * LDSA);                                     \
     float* Bs2_ = As2_ + BM * LDSA;                                                      \
-    *reinterpret_cast<float4*>(As2_ + (lr) * LDSA + lkk) = a##S##_0;                     \
-    *reinterpret_cast<float4*>(As2_ + (lr + 32) * LDSA + lkk) = a##S##_1;                \
-    *reinterpret_cast<float4*>(As2_ + (lr + 64) * LDSA + lkk) = a##S##_2;                \
-    *reinterpret_cast<float4*>(As2_ + (lr + 96) * LDSA + lkk) = a##S##_3;                \
-    *reinterpret_cast<float4*>(Bs2_ + (lr) * LDSA + lkk) = b##S##_0;                     \
-    *reinterpret_cast<float4*>(Bs2_ + (lr + 32) * LDSA + lkk) = b##S##_1;                \
-    *reinterpret_cast<float4*>(Bs2_ + (lr + 64) * LDSA + lkk) = b##S##_2;                \
-    *reinterpret_cast<float4*>(Bs2_ + (lr + 96) * LDSA + lkk) = b##S##_3;                \
+    *reinterpret_cast<float4*>(As2_ + (lr) * LDSA + lkk) = ra0;                          \
+    *reinterpret_cast<float4*>(As2_ + (lr + 32) * LDSA + lkk) = ra1;                     \
+    *reinterpret_cast<float4*>(As2_ + (lr + 64) * LDSA + lkk) = ra2;                     \
+    *reinterpret_cast<float4*>(As2_ + (lr + 96) * LDSA + lkk) = ra3;                     \
+    *reinterpret_cast<float4*>(Bs2_ + (lr) * LDSA + lkk) = rb0;                          \
+    *reinterpret_cast<float4*>(Bs2_ + (lr + 32) * LDSA + lkk) = rb1;                     \
+    *reinterpret_cast<float4*>(Bs2_ + (lr + 64) * LDSA + lkk) = rb2;                     \
+    *reinterpret_cast<float4*>(Bs2_ + (lr + 96) * LDSA + lkk) = rb3;                     \
   } while (0)
-    MV_GLOAD(0, 0);
-    MV_GLOAD(1, 1);
-    MV_LSTORE(0, 0);
+    MV_GLOAD(0);
+    MV_LSTORE(0);
     __syncthreads();
-    for (int kc = 0; kc < nk; kc += 2) {
-      MV_GLOAD(0, kc + 2);
-      mfma_chunk<DT>(s_stage, s_stage + BM * LDSA, wr, wc, l, acc);
-      MV_LSTORE(1, 1);
-      __syncthreads();
-      if (kc + 1 >= nk) break;
-      MV_GLOAD(1, kc + 3);
-      mfma_chunk<DT>(s_stage + 2 * BM * LDSA, s_stage + 3 * BM * LDSA, wr, wc, l, acc);
-      MV_LSTORE(0, 0);
+    for (int kc = 0; kc < nk; ++kc) {
+      const int cur = kc & 1;
+      MV_GLOAD(kc + 1);
+      // keep the loads ahead of the multiplies (the scheduler would otherwise
+      // sink them past the MFMAs to reuse operand registers, exposing their latency)
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_chunk<DT>(s_stage + cur * (2 * BM * LDSA), s_stage + cur * (2 * BM * LDSA) + BM * LDSA, wr, wc, l, acc);
+      MV_LSTORE(cur ^ 1);
       __syncthreads();
     }
 #undef MV_GLOAD
 #undef MV_LSTORE
 
-    // Epilogue: D = |q|^2 + |x|^2 - 2 q.x, clamped at 0; excluded columns -> +inf.
+    // Epilogue, phase 1: screen values D = |q|^2 + |x|^2 - 2 q.x (clamped at 0;
+    // excluded columns +inf) stay in registers and are compared with each row's
+    // current C-th distance (a superset test: equal distances pass). A ballot
+    // per (mi, r) gives the rows of this wavefront's quadrant with survivors;
+    // only quadrants with survivors are written to the D tile.
     float* Dt = s_stage;
     float sqxv[4], invc[4];
     int colv[4];
@@ -426,6 +477,7 @@ __global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
       sqxv[ni] = a.sqx[colv[ni]];
       invc[ni] = DT == DT_FP8 ? a.invx[colv[ni]] : 1.f;
     }
+    uint64_t hits = 0;   // bit mi*16 + g*4 + r: row wr*64 + mi*16 + g*4 + r has a survivor
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
 #pragma unroll
@@ -434,50 +486,75 @@ __global__ __launch_bounds__(NT, 1) void k_screen(ScreenArgs a) {
         const float sr = s_sqq[row];
         const int self = s_self[row];
         const float ir = s_inv[row];
+        const float tf = s_tauf[row];
+        bool pass = false;
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
           const float dot = DT == DT_FP8 ? acc[mi][ni][r] * ir * invc[ni] : acc[mi][ni][r];
           float v = fmaxf(sr + sqxv[ni] - 2.f * dot, 0.f);
           if (colv[ni] >= a.n || colv[ni] == self) v = INFINITY;
-          Dt[row * DTS + wc * 64 + ni * 16 + (l & 15)] = v;
+          acc[mi][ni][r] = v;
+          pass |= (v <= tf) && (v < INFINITY);
+        }
+        const uint64_t b = ballot(pass);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          if ((b >> (16 * g)) & 0xFFFFull) hits |= 1ull << (mi * 16 + g * 4 + r);
+      }
+    }
+    if (hits) {
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wr * 64 + mi * 16 + (l >> 4) * 4 + r;
+#pragma unroll
+          for (int ni = 0; ni < 4; ++ni) Dt[row * DTS + wc * 64 + ni * 16 + (l & 15)] = acc[mi][ni][r];
+        }
+    }
+    if (l == 0) s_hit[wc][wr] = hits;
+    __syncthreads();
+
+    // Phase 2: wavefront w owns rows 32w .. 32w + 31 and visits only the rows
+    // with survivors, reading the column halves that have them.
+    {
+      const int hr = w >> 1, sh = (w & 1) * 32;
+      const uint32_t h0 = static_cast<uint32_t>(s_hit[0][hr] >> sh);
+      const uint32_t h1 = static_cast<uint32_t>(s_hit[1][hr] >> sh);
+      uint32_t todo = h0 | h1;
+      while (todo) {
+        const int rr = __ffs(static_cast<int>(todo)) - 1;
+        todo &= todo - 1;
+        const int row = w * 32 + rr;
+        uint64_t tau = s_tau[row];
+        const float v0 = (h0 >> rr) & 1u ? Dt[row * DTS + l] : INFINITY;
+        const float v1 = (h1 >> rr) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;
+        const uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));
+        const uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));
+        uint64_t m0 = ballot(k0 < tau);
+        uint64_t m1 = ballot(k1 < tau);
+        if ((m0 | m1) == 0) continue;
+        uint64_t lk = l < C ? lists[row * C + l] : KEY_NONE;
+        while (m0) {
+          const int src = __ffsll(static_cast<long long>(m0)) - 1;
+          m0 &= m0 - 1;
+          list_insert(lk, readlane_u64(k0, src), tau, C, l);
+        }
+        while (m1) {
+          const int src = __ffsll(static_cast<long long>(m1)) - 1;
+          m1 &= m1 - 1;
+          list_insert(lk, readlane_u64(k1, src), tau, C, l);
+        }
+        if (l < C) lists[row * C + l] = lk;
+        if (l == 0) {
+          s_tau[row] = tau;
+          s_tauf[row] = tau == KEY_NONE ? INFINITY : key_dist(tau);
         }
       }
     }
     __syncthreads();
-
-    for (int rr = 0; rr < BM / 4; ++rr) {
-      const int row = w * (BM / 4) + rr;
-      uint64_t tau = s_tau[row];
-      const float v0 = Dt[row * DTS + l];
-      const float v1 = Dt[row * DTS + 64 + l];
-      const uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));
-      const uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));
-      uint64_t m0 = ballot(k0 < tau);
-      uint64_t m1 = ballot(k1 < tau);
-      if ((m0 | m1) == 0) continue;
-      uint64_t lk = l < C ? s_list[row * CMAX + l] : KEY_NONE;
-      while (m0) {
-        const int src = __ffsll(static_cast<long long>(m0)) - 1;
-        m0 &= m0 - 1;
-        list_insert(lk, readlane_u64(k0, src), tau, C, l);
-      }
-      while (m1) {
-        const int src = __ffsll(static_cast<long long>(m1)) - 1;
-        m1 &= m1 - 1;
-        list_insert(lk, readlane_u64(k1, src), tau, C, l);
-      }
-      if (l < C) s_list[row * CMAX + l] = lk;
-      if (l == 0) s_tau[row] = tau;
-    }
-    __syncthreads();
   }
 
-  for (int rr = 0; rr < BM / 4; ++rr) {
-    const int row = w * (BM / 4) + rr;
-    const int64_t grow = static_cast<int64_t>(rb) * BM + row;
-    if (grow < nq && l < C)
-      a.cand[(static_cast<int64_t>(split) * a.nq_pad + grow) * C + l] = s_list[row * CMAX + l];
-  }
 }
 
 // ----------------------------------------------------------------- rerank --
@@ -683,6 +760,164 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
 
 // ------------------------------------------------------------- exact scan --
 
+constexpr int SCAN_CHUNKS = 64;    // column chunks per failing row in the split scan
+constexpr int CMAX_K = 32;         // largest k
+constexpr int SCAN_ROWS = 256;     // failing rows handled by the split scan (the rest: one block per row)
+
+// Split exhaustive scan, part 1: block (chunk c, slot y) handles failing rows
+// f = y, y + SCAN_ROWS', ... < min(count, SCAN_ROWS) over the columns of chunk c.
+// Every lane computes canonical FP64 distances (sequential, unfused) for its
+// columns; each wavefront keeps a sorted top-k list; the block's four lists are
+// merged and its k best (distance bits, index) pairs stored for row f.
+struct ScanPartArgs {
+  const float* Q;
+  const float* X;
+  int64_t d;
+  int n, k;
+  const int32_t* q_self;
+  int64_t self_base;
+  const int32_t* fb_count;
+  const int32_t* fb_rows;
+  uint64_t* part_d;   // [SCAN_ROWS, SCAN_CHUNKS, k]
+  uint32_t* part_i;
+};
+
+__device__ __forceinline__ void wave_list_insert64(uint64_t& ld, uint32_t& li, uint64_t cd, uint32_t ci, int k,
+                                                   int l) {
+  const int pos = __popcll(ballot(ld < cd || (ld == cd && li < ci)));
+  const uint64_t upd = shfl_up_u64(ld, 1);
+  const uint32_t upi = static_cast<uint32_t>(__shfl_up(static_cast<int>(li), 1));
+  if (l > pos) {
+    ld = upd;
+    li = upi;
+  } else if (l == pos) {
+    ld = cd;
+    li = ci;
+  }
+  if (l >= k) {
+    ld = KEY_NONE;
+    li = 0xFFFFFFFFu;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_part(ScanPartArgs a) {
+  __shared__ uint64_t s_d[4 * 64];
+  __shared__ uint32_t s_i[4 * 64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int count = min(*a.fb_count, SCAN_ROWS);
+  const int k = a.k;
+  const int64_t per = (a.n + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+  const int c0 = static_cast<int>(blockIdx.x * per);
+  const int c1 = static_cast<int>(min(static_cast<int64_t>(a.n), c0 + per));
+  for (int f = blockIdx.y; f < count; f += gridDim.y) {
+    const int i = a.fb_rows[f];
+    int64_t self = -1;
+    if (a.q_self) self = a.q_self[i];
+    else if (a.self_base >= 0) self = a.self_base + i;
+    const float* qrow = a.Q + static_cast<int64_t>(i) * a.d;
+    uint64_t ld = KEY_NONE, td = KEY_NONE;
+    uint32_t li = 0xFFFFFFFFu, ti = 0xFFFFFFFFu;
+    for (int j0 = c0 + w * 64; j0 < c1; j0 += 256) {
+      const int j = j0 + l;
+      const bool valid = j < c1 && j != self;
+      uint64_t vb = KEY_NONE;
+      if (valid) vb = static_cast<uint64_t>(__double_as_longlong(dist64_seq(qrow, a.X + static_cast<int64_t>(j) * a.d, a.d)));
+      uint64_t m = ballot(valid && pair_less(vb, static_cast<uint32_t>(j), td, ti));
+      while (m) {
+        const int src = __ffsll(static_cast<long long>(m)) - 1;
+        m &= m - 1;
+        const uint64_t cd = readlane_u64(vb, src);
+        const uint32_t ci = static_cast<uint32_t>(j0 + src);
+        if (!pair_less(cd, ci, td, ti)) continue;
+        wave_list_insert64(ld, li, cd, ci, k, l);
+        td = readlane_u64(ld, k - 1);
+        ti = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(li), k - 1));
+      }
+    }
+    s_d[w * 64 + l] = ld;
+    s_i[w * 64 + l] = li;
+    __syncthreads();
+    if (w == 0) {
+      // merge: insert the other three lists into wave 0's list
+      for (int q = 1; q < 4; ++q)
+        for (int e = 0; e < k; ++e) {
+          const uint64_t cd = s_d[q * 64 + e];
+          const uint32_t ci = s_i[q * 64 + e];
+          if (cd == KEY_NONE) break;
+          if (!pair_less(cd, ci, td, ti)) break;   // lists are sorted: the rest cannot enter
+          wave_list_insert64(ld, li, cd, ci, k, l);
+          td = readlane_u64(ld, k - 1);
+          ti = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(li), k - 1));
+        }
+      if (l < k) {
+        const int64_t o = (static_cast<int64_t>(f) * SCAN_CHUNKS + blockIdx.x) * k + l;
+        a.part_d[o] = ld;
+        a.part_i[o] = li;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Split exhaustive scan, part 2: one wavefront per failing row f <
+// min(count, SCAN_ROWS) merges the SCAN_CHUNKS partial lists into the top-k.
+struct ScanMergeArgs {
+  const int32_t* fb_count;
+  const int32_t* fb_rows;
+  const uint64_t* part_d;
+  const uint32_t* part_i;
+  int k, euclid;
+  int32_t* out_idx;
+  float* out_dist;
+  int32_t* stats;
+};
+
+__global__ __launch_bounds__(256) void k_scan_merge(ScanMergeArgs a) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int f = blockIdx.x * 4 + w;
+  const int count = min(*a.fb_count, SCAN_ROWS);
+  if (f >= count) return;
+  const int k = a.k;
+  const int M = SCAN_CHUNKS * k;   // <= 64 * 32 entries, at most 32 per lane
+  uint64_t outd = KEY_NONE;
+  uint32_t outi = 0xFFFFFFFFu;
+  int taken = 0;   // per lane: entries of its column consumed (lists are sorted per chunk)
+  // lane l < SCAN_CHUNKS walks chunk l's sorted list: a k-way merge by wave minima
+  const int64_t base = static_cast<int64_t>(f) * SCAN_CHUNKS * k;
+  for (int j = 0; j < k; ++j) {
+    uint64_t hd = KEY_NONE;
+    uint32_t hi = 0xFFFFFFFFu;
+    if (l < SCAN_CHUNKS && taken < k) {
+      hd = a.part_d[base + static_cast<int64_t>(l) * k + taken];
+      hi = a.part_i[base + static_cast<int64_t>(l) * k + taken];
+    }
+    uint64_t bd = hd;
+    uint32_t bi = hi;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const uint64_t od = shfl_xor_u64(bd, m);
+      const uint32_t oi = static_cast<uint32_t>(__shfl_xor(static_cast<int>(bi), m));
+      if (pair_less(od, oi, bd, bi)) {
+        bd = od;
+        bi = oi;
+      }
+    }
+    if (hd == bd && hi == bi && hd != KEY_NONE) ++taken;
+    if (l == j) {
+      outd = bd;
+      outi = bi;
+    }
+  }
+  (void)M;
+  const int i = a.fb_rows[f];
+  if (l < k) {
+    const double dv = __longlong_as_double(static_cast<long long>(outd));
+    a.out_idx[static_cast<int64_t>(i) * k + l] = static_cast<int32_t>(outi);
+    a.out_dist[static_cast<int64_t>(i) * k + l] = a.euclid ? static_cast<float>(sqrt(dv)) : static_cast<float>(dv);
+  }
+  if (l == 0) atomicAdd(&a.stats[0], 1);
+}
+
 struct ScanArgs {
   const float* Q;
   const float* X;
@@ -698,7 +933,8 @@ struct ScanArgs {
   int32_t* stats;
 };
 
-// Exhaustive canonical scan of one query row per workgroup iteration: every
+// Exhaustive canonical scan of one failing row (index >= SCAN_ROWS in the
+// failure list) per workgroup iteration: every
 // lane computes FP64 sequential distances for its columns; each wavefront
 // keeps a sorted top-k list; the four lists are merged at the end.
 __global__ __launch_bounds__(256) void k_exact_scan(ScanArgs a) {
@@ -707,7 +943,8 @@ __global__ __launch_bounds__(256) void k_exact_scan(ScanArgs a) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int count = *a.fb_count;
   const int k = a.k;
-  for (int f = blockIdx.x; f < count; f += gridDim.x) {
+  // rows below SCAN_ROWS are handled by the split scan (k_scan_part / k_scan_merge)
+  for (int f = SCAN_ROWS + blockIdx.x; f < count; f += gridDim.x) {
     const int i = a.fb_rows[f];
     int64_t self = -1;
     if (a.q_self) self = a.q_self[i];
@@ -824,6 +1061,8 @@ struct Buffers {
   uint64_t* cand;
   int32_t* ctr;   // [0] fb1 count, [1] sqmax32, [2] sqmaxq, [3] emax, [4] fb2 count, [8..11] stats
   int32_t *fb1, *fb2;
+  uint64_t* scan_d;                             // split exhaustive scan partial lists
+  uint32_t* scan_i;
   float *Qc2, *sqq2;                            // FP32 re-screen rows
   int32_t* self2;
 };
@@ -842,7 +1081,11 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
     b.Qc = c.take<float>(static_cast<size_t>(L.nq_pad) * L.d_pad);
     b.sqq = c.take<float>(L.nq_pad);
   }
-  if (L.dtype != MV_F32) {
+  if (L.dtype == MV_BF16X3) {
+    const size_t rb = static_cast<size_t>(L.dq_pad) * elem_bytes(L.dtype);
+    b.Xq = c.take<char>(static_cast<size_t>(L.n_pad) * rb);
+    b.Qq = L.same ? b.Xq : c.take<char>(static_cast<size_t>(L.nq_pad) * rb);
+  } else if (L.dtype != MV_F32) {
     const size_t rb = static_cast<size_t>(L.dq_pad) * elem_bytes(L.dtype);
     b.Xq = c.take<char>(static_cast<size_t>(L.n_pad) * rb);
     b.sqxq = c.take<float>(L.n_pad);
@@ -867,6 +1110,9 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
   b.ctr = c.take<int32_t>(16);
   b.fb1 = c.take<int32_t>(nq > 0 ? nq : 1);
   b.fb2 = c.take<int32_t>(nq > 0 ? nq : 1);
+  const int64_t srows = nq < SCAN_ROWS ? (nq > 0 ? nq : 1) : SCAN_ROWS;
+  b.scan_d = c.take<uint64_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
+  b.scan_i = c.take<uint32_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
   *total = c.off + 256;
   return b;
 }
@@ -965,7 +1211,7 @@ int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rera
              (long long)nq, (long long)n, (long long)d);
   MV_REQUIRE(k >= 1 && k <= 32, "mv_knn_ws_bytes: k=%d outside [1, 32]", k);
   MV_REQUIRE(rerank >= 1, "mv_knn_ws_bytes: rerank=%d < 1", rerank);
-  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8) {
+  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
     mv::set_error("mv_knn_ws_bytes: dtype %d not supported", dtype);
     return MV_EUNSUPPORTED;
   }
@@ -998,7 +1244,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   const bool excl = q_self != nullptr || self_base >= 0;
   MV_REQUIRE(k <= n - (excl ? 1 : 0), "mv_knn_brute: k=%d exceeds the %lld selectable rows", k,
              (long long)(n - (excl ? 1 : 0)));
-  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8) {
+  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
     mv::set_error("mv_knn_brute: dtype %d not supported", dtype);
     return MV_EUNSUPPORTED;
   }
@@ -1048,7 +1294,18 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
                        L.d_pad, L.nq_pad, B.mu, B.Qc, B.sqq, static_cast<uint32_t*>(nullptr));
     MV_LAUNCHED("k_center_f32");
   }
-  if (dtype != MV_F32) {
+  if (dtype == MV_BF16X3) {
+    const int64_t cx = L.n_pad * L.d_pad;
+    hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cx, 256))), dim3(256), 0, s, B.Xc, cx,
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xq));
+    MV_LAUNCHED("k_split_bf16");
+    if (!same) {
+      const int64_t cq = L.nq_pad * L.d_pad;
+      hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cq, 256))), dim3(256), 0, s, B.Qc, cq,
+                         L.d_pad, reinterpret_cast<uint16_t*>(B.Qq));
+      MV_LAUNCHED("k_split_bf16");
+    }
+  } else if (dtype != MV_F32) {
     auto kq = dtype == MV_BF16 ? k_center_q<DT_BF16> : k_center_q<DT_FP8>;
     hipLaunchKernelGGL(kq, dim3(static_cast<unsigned>(L.n_pad / 4)), dim3(256), 0, s, X, n, d, L.dq_pad, L.n_pad,
                        B.mu, B.Xq, B.sqxq, B.ex, B.invx, sqmaxq, emax);
@@ -1063,7 +1320,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
 
   const int64_t row_blocks = L.nq_pad / BM;
   const int64_t col_tiles = L.n_pad / BN;
-  const int S = choose_split(row_blocks, col_tiles, L.C, cu_count_cached());
+  const int S = choose_split(row_blocks, col_tiles, L.C, 2 * cu_count_cached());   // two workgroups per CU
   const int64_t grid = ceil_div(row_blocks, 8 / S) * 8;
   ScreenArgs sa;
   sa.q_self = q_self;
@@ -1078,9 +1335,9 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   sa.n_row_blocks = static_cast<int>(row_blocks);
   sa.nq_pad = static_cast<int>(L.nq_pad);
   sa.cand = B.cand;
-  if (dtype == MV_F32) {
-    sa.Qq = reinterpret_cast<const char*>(B.Qc);
-    sa.Xq = reinterpret_cast<const char*>(B.Xc);
+  if (dtype == MV_F32 || dtype == MV_BF16X3) {
+    sa.Qq = dtype == MV_F32 ? reinterpret_cast<const char*>(B.Qc) : B.Qq;
+    sa.Xq = dtype == MV_F32 ? reinterpret_cast<const char*>(B.Xc) : B.Xq;
     sa.sqq = B.sqq;
     sa.sqx = B.sqx;
     sa.invq = sa.invx = nullptr;
@@ -1095,14 +1352,22 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     sa.row_bytes = L.dq_pad * elem_bytes(dtype);
   }
   if (int rc = g_timer.mark(1, s)) return rc;
-  int rc = dtype == MV_F32    ? launch_screen<DT_F32>(sa, grid, s)
-           : dtype == MV_BF16 ? launch_screen<DT_BF16>(sa, grid, s)
-                              : launch_screen<DT_FP8>(sa, grid, s);
+  int rc = dtype == MV_F32      ? launch_screen<DT_F32>(sa, grid, s)
+           : dtype == MV_BF16   ? launch_screen<DT_BF16>(sa, grid, s)
+           : dtype == MV_BF16X3 ? launch_screen<DT_BF16X3>(sa, grid, s)
+                                : launch_screen<DT_FP8>(sa, grid, s);
   if (rc) return rc;
   if (int rc2 = g_timer.mark(2, s)) return rc2;
 
-  const float fac = static_cast<float>((2.0 * static_cast<double>(sa.row_bytes / elem_bytes(dtype)) + 10.0) *
-                                       5.9604644775390625e-8 * 1.01);
+  // Screen arithmetic bound, in units of (|q|^2 + max |x|^2). FP32: two
+  // roundings per product term plus the norms and the final combination.
+  // Split BF16: products are exact, one rounding per each of the 3 d_pad
+  // accumulated terms, plus the dropped lo*lo term and the split residuals
+  // (each <= 2^-16 relative: 3 * 256 units of 2^-24).
+  const double units = dtype == MV_BF16X3
+                           ? 3.0 * static_cast<double>(L.d_pad) + 3.0 * 256.0 + 40.0
+                           : 2.0 * static_cast<double>(sa.row_bytes / elem_bytes(dtype)) + 10.0;
+  const float fac = static_cast<float>(units * 5.9604644775390625e-8 * 1.01);
   RerankArgs ra;
   ra.Q = Q;
   ra.X = X;
@@ -1115,9 +1380,10 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   ra.nq_pad = static_cast<int>(L.nq_pad);
   ra.cand = B.cand;
   ra.sqq = sa.sqq;
-  ra.sqmax_bits = dtype == MV_F32 ? sqmax32 : sqmaxq;
+  const bool exact_form = dtype == MV_F32 || dtype == MV_BF16X3;   // no quantisation-error term
+  ra.sqmax_bits = exact_form ? sqmax32 : sqmaxq;
   ra.err_fac = fac;
-  ra.eq = dtype == MV_F32 ? nullptr : B.eq;
+  ra.eq = exact_form ? nullptr : B.eq;
   ra.emax_bits = emax;
   ra.nq_dev = nullptr;
   ra.row_map = nullptr;
@@ -1135,7 +1401,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
 
   const int32_t* scan_count = fb1_count;
   const int32_t* scan_rows = B.fb1;
-  if (dtype != MV_F32) {
+  if (!exact_form) {
     // FP32 re-screen of the rows the quantised certificate could not settle.
     hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, B.Qc, B.sqq,
                        L.d_pad, B.fb1, fb1_count, q_self, self_b, B.Qc2, B.sqq2, B.self2);
@@ -1167,23 +1433,53 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     scan_rows = B.fb2;
   }
 
-  ScanArgs xa;
-  xa.Q = Q;
-  xa.X = X;
-  xa.d = d;
-  xa.n = static_cast<int>(n);
-  xa.k = k;
-  xa.q_self = q_self;
-  xa.self_base = self_b;
-  xa.fb_count = scan_count;
-  xa.fb_rows = scan_rows;
-  xa.euclid = euclid;
-  xa.out_idx = out_idx;
-  xa.out_dist = out_dist;
-  xa.stats = stats;
-  const int64_t sgrid = nq < 2048 ? nq : 2048;
-  hipLaunchKernelGGL(k_exact_scan, dim3(static_cast<unsigned>(sgrid)), dim3(256), 0, s, xa);
-  MV_LAUNCHED("k_exact_scan");
+  const int64_t srows = nq < SCAN_ROWS ? nq : SCAN_ROWS;
+  ScanPartArgs pa;
+  pa.Q = Q;
+  pa.X = X;
+  pa.d = d;
+  pa.n = static_cast<int>(n);
+  pa.k = k;
+  pa.q_self = q_self;
+  pa.self_base = self_b;
+  pa.fb_count = scan_count;
+  pa.fb_rows = scan_rows;
+  pa.part_d = B.scan_d;
+  pa.part_i = B.scan_i;
+  hipLaunchKernelGGL(k_scan_part, dim3(SCAN_CHUNKS, static_cast<unsigned>(srows)), dim3(256), 0, s, pa);
+  MV_LAUNCHED("k_scan_part");
+  ScanMergeArgs ma;
+  ma.fb_count = scan_count;
+  ma.fb_rows = scan_rows;
+  ma.part_d = B.scan_d;
+  ma.part_i = B.scan_i;
+  ma.k = k;
+  ma.euclid = euclid;
+  ma.out_idx = out_idx;
+  ma.out_dist = out_dist;
+  ma.stats = stats;
+  hipLaunchKernelGGL(k_scan_merge, dim3(static_cast<unsigned>(ceil_div(srows, 4))), dim3(256), 0, s, ma);
+  MV_LAUNCHED("k_scan_merge");
+  if (nq > SCAN_ROWS) {
+    ScanArgs xa;
+    xa.Q = Q;
+    xa.X = X;
+    xa.d = d;
+    xa.n = static_cast<int>(n);
+    xa.k = k;
+    xa.q_self = q_self;
+    xa.self_base = self_b;
+    xa.fb_count = scan_count;
+    xa.fb_rows = scan_rows;
+    xa.euclid = euclid;
+    xa.out_idx = out_idx;
+    xa.out_dist = out_dist;
+    xa.stats = stats;
+    const int64_t rest = nq - SCAN_ROWS;
+    const int64_t sgrid = rest < 2048 ? rest : 2048;
+    hipLaunchKernelGGL(k_exact_scan, dim3(static_cast<unsigned>(sgrid)), dim3(256), 0, s, xa);
+    MV_LAUNCHED("k_exact_scan");
+  }
   if (int rc2 = g_timer.mark(4, s)) return rc2;
   g_timer.valid = g_timer.on;
   return MV_OK;

@@ -21,7 +21,7 @@ MV_EWORKSPACE = -3
 MV_EUNSUPPORTED = -4
 MV_EINJECTED = -5
 
-DTYPES = {"fp32": 0, "bf16": 1, "fp8": 2}
+DTYPES = {"fp32": 0, "bf16": 1, "fp8": 2, "bf16x3": 3}
 
 
 class MivisError(RuntimeError):

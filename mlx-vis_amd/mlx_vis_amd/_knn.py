@@ -8,7 +8,7 @@ float32 distances, Euclidean unless ``return_euclidean=False``).
 
 The brute path returns the canonical neighbours: ascending exact distance,
 ties broken by the lower index (the reference's argsort order,
-_knn.py:71-72); the screen ``dtype`` ("fp32", "bf16", "fp8") changes speed,
+_knn.py:71-72); the screen ``dtype`` ("fp32", "bf16", "fp8", "bf16x3": split BF16) changes speed,
 never the result.
 """
 
@@ -89,7 +89,7 @@ def compute_knn(X, k, method="auto", verbose=False, random_state=None, return_eu
         method: "auto" (brute for n <= 20000, else NNDescent), "brute", "nndescent".
         random_state: NNDescent seed (default 42).
         return_euclidean: Euclidean distances if True, squared if False.
-        dtype: brute-force screen precision ("fp32", "bf16", "fp8").
+        dtype: brute-force screen precision ("fp32", "bf16", "fp8", "bf16x3").
 
     Returns:
         (indices int32 (n, k), distances float32 (n, k)) numpy arrays.

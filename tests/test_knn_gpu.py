@@ -89,7 +89,7 @@ def test_c0_shape_bit_exact():
     print("stats", st)
 
 
-@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("dtype", ["bf16", "fp8", "bf16x3"])
 @pytest.mark.parametrize("n,d,k", [(1000, 128, 15), (777, 3, 5), (300, 784, 15), (2000, 129, 32)])
 def test_precision_screens_give_identical_indices(dtype, n, d, k):
     print(f"case dtype={dtype} n={n} d={d} k={k}", flush=True)
@@ -101,7 +101,7 @@ def test_precision_screens_give_identical_indices(dtype, n, d, k):
     _check(idx, dist, oi, od)
 
 
-@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("dtype", ["bf16", "fp8", "bf16x3"])
 def test_precision_screens_queries_and_duplicates(dtype):
     rng = np.random.default_rng(8)
     base = rng.integers(0, 3, size=(150, 6)).astype(np.float32)
