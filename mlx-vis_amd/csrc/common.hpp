@@ -90,6 +90,15 @@ __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int delta) {
   int hi = __shfl_up(static_cast<int>(static_cast<uint32_t>(v >> 32)), delta);
   return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
 }
+// Lane i receives lane i - 1's value (lane 0 keeps its own): the DPP
+// wave_shr:1 move, a single VALU operation instead of a permute through LDS.
+__device__ __forceinline__ uint64_t wave_shr1_u64(uint64_t v) {
+  const int lo = static_cast<int>(static_cast<uint32_t>(v));
+  const int hi = static_cast<int>(static_cast<uint32_t>(v >> 32));
+  const int slo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xF, 0xF, false);
+  const int shi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xF, 0xF, false);
+  return (static_cast<uint64_t>(static_cast<uint32_t>(shi)) << 32) | static_cast<uint32_t>(slo);
+}
 __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
   uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), src);
   uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), src);

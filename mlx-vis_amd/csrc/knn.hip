@@ -255,7 +255,7 @@ struct ScreenArgs {
 __device__ __forceinline__ void list_insert(uint64_t& lk, uint64_t kv, uint64_t& tau, int C, int l) {
   if (kv >= tau) return;
   const int pos = __popcll(ballot(lk < kv));
-  const uint64_t up = shfl_up_u64(lk, 1);
+  const uint64_t up = wave_shr1_u64(lk);
   if (l > pos) lk = up;
   else if (l == pos) lk = kv;
   if (l >= C) lk = KEY_NONE;
@@ -405,34 +405,27 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   const int lkk = (tid & 7) * 4;      // staging offset within the chunk (floats)
   const int nk = static_cast<int>(rbytes / 128);
 
-  for (int t = t0; t < t1; ++t) {
-    const int col0 = t * BN;
-    const char* Bb = a.Xq + static_cast<int64_t>(col0) * rbytes;
-
-    v4f acc[4][4];
-#pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
-
-    // Register prefetch one chunk deep into a double-buffered LDS stage: while
-    // chunk kc is multiplied out of one buffer, chunk kc + 1 is in flight from
-    // memory and then stored to the other buffer (the co-resident workgroup of
-    // the CU hides the remaining latency). Chunk indices past the end are
-    // clamped (re-read, never used), keeping every load and store unconditional.
-    float4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define MV_GLOAD(KCC)                                                                      \
+  // Register prefetch one chunk deep into a double-buffered LDS stage: while
+  // chunk kc is multiplied out of one buffer, chunk kc + 1 is in flight from
+  // memory and then stored to the other buffer (the co-resident workgroup of
+  // the CU hides the remaining latency). The last chunk of a tile prefetches
+  // the first chunk of the next tile, which stays in registers through the
+  // epilogue. Chunk indices past the end are clamped (re-read, never used),
+  // keeping every load and store unconditional.
+  float4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define MV_GLOAD(BPTR, KCC)                                                                \
   do {                                                                                   \
     const int kq_ = (KCC) < nk ? (KCC) : nk - 1;                                         \
     const int64_t ko_ = static_cast<int64_t>(kq_) * 128 + lkk * 4;                       \
+    const char* B_ = (BPTR);                                                             \
     ra0 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr) * rbytes + ko_);        \
     ra1 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 32) * rbytes + ko_);   \
     ra2 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 64) * rbytes + ko_);   \
     ra3 = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 96) * rbytes + ko_);   \
-    rb0 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr) * rbytes + ko_);        \
-    rb1 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 32) * rbytes + ko_);   \
-    rb2 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 64) * rbytes + ko_);   \
-    rb3 = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 96) * rbytes + ko_);   \
+    rb0 = *reinterpret_cast<const float4*>(B_ + static_cast<int64_t>(lr) * rbytes + ko_);        \
+    rb1 = *reinterpret_cast<const float4*>(B_ + static_cast<int64_t>(lr + 32) * rbytes + ko_);   \
+    rb2 = *reinterpret_cast<const float4*>(B_ + static_cast<int64_t>(lr + 64) * rbytes + ko_);   \
+    rb3 = *reinterpret_cast<const float4*>(B_ + static_cast<int64_t>(lr + 96) * rbytes + ko_);   \
   } while (0)
 #define MV_LSTORE(BUF)                                                                     \
   do {                                                                                   \
@@ -447,21 +440,33 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
     *reinterpret_cast<float4*>(Bs2_ + (lr + 64) * LDSA + lkk) = rb2;                     \
     *reinterpret_cast<float4*>(Bs2_ + (lr + 96) * LDSA + lkk) = rb3;                     \
   } while (0)
-    MV_GLOAD(0);
+  // (a split past the last tile runs no tile; its prefetch address is clamped)
+  MV_GLOAD(a.Xq + static_cast<int64_t>(t0 < a.n_col_tiles ? t0 : a.n_col_tiles - 1) * BN * rbytes, 0);
+
+  for (int t = t0; t < t1; ++t) {
+    const int col0 = t * BN;
+    const char* Bb = a.Xq + static_cast<int64_t>(col0) * rbytes;
+    const char* Bnext = t + 1 < t1 ? Bb + static_cast<int64_t>(BN) * rbytes : Bb;
+
+    v4f acc[4][4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
+
     MV_LSTORE(0);
     __syncthreads();
     for (int kc = 0; kc < nk; ++kc) {
       const int cur = kc & 1;
-      MV_GLOAD(kc + 1);
+      const bool last = kc + 1 == nk;
+      MV_GLOAD(last ? Bnext : Bb, last ? 0 : kc + 1);
       // keep the loads ahead of the multiplies (the scheduler would otherwise
       // sink them past the MFMAs to reuse operand registers, exposing their latency)
       __builtin_amdgcn_sched_barrier(0);
       mfma_chunk<DT>(s_stage + cur * (2 * BM * LDSA), s_stage + cur * (2 * BM * LDSA) + BM * LDSA, wr, wc, l, acc);
-      MV_LSTORE(cur ^ 1);
+      if (!last) MV_LSTORE(cur ^ 1);
       __syncthreads();
     }
-#undef MV_GLOAD
-#undef MV_LSTORE
 
     // Epilogue, phase 1: screen values D = |q|^2 + |x|^2 - 2 q.x (clamped at 0;
     // excluded columns +inf) stay in registers and are compared with each row's
@@ -516,44 +521,64 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
     __syncthreads();
 
     // Phase 2: wavefront w owns rows 32w .. 32w + 31 and visits only the rows
-    // with survivors, reading the column halves that have them.
+    // with survivors, reading the column halves that have them. The lists of
+    // up to 8 such rows are loaded from L2 together, so their latency is paid
+    // once per batch.
     {
       const int hr = w >> 1, sh = (w & 1) * 32;
       const uint32_t h0 = static_cast<uint32_t>(s_hit[0][hr] >> sh);
       const uint32_t h1 = static_cast<uint32_t>(s_hit[1][hr] >> sh);
       uint32_t todo = h0 | h1;
       while (todo) {
-        const int rr = __ffs(static_cast<int>(todo)) - 1;
-        todo &= todo - 1;
-        const int row = w * 32 + rr;
-        uint64_t tau = s_tau[row];
-        const float v0 = (h0 >> rr) & 1u ? Dt[row * DTS + l] : INFINITY;
-        const float v1 = (h1 >> rr) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;
-        const uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));
-        const uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));
-        uint64_t m0 = ballot(k0 < tau);
-        uint64_t m1 = ballot(k1 < tau);
-        if ((m0 | m1) == 0) continue;
-        uint64_t lk = l < C ? lists[row * C + l] : KEY_NONE;
-        while (m0) {
-          const int src = __ffsll(static_cast<long long>(m0)) - 1;
-          m0 &= m0 - 1;
-          list_insert(lk, readlane_u64(k0, src), tau, C, l);
+        int rrq[8];
+        uint64_t lkq[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          rrq[q] = -1;
+          lkq[q] = KEY_NONE;
+          if (todo) {
+            const int rr = __ffs(static_cast<int>(todo)) - 1;
+            todo &= todo - 1;
+            rrq[q] = rr;
+            if (l < C) lkq[q] = lists[(w * 32 + rr) * C + l];
+          }
         }
-        while (m1) {
-          const int src = __ffsll(static_cast<long long>(m1)) - 1;
-          m1 &= m1 - 1;
-          list_insert(lk, readlane_u64(k1, src), tau, C, l);
-        }
-        if (l < C) lists[row * C + l] = lk;
-        if (l == 0) {
-          s_tau[row] = tau;
-          s_tauf[row] = tau == KEY_NONE ? INFINITY : key_dist(tau);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int rr = rrq[q];
+          if (rr < 0) break;
+          const int row = w * 32 + rr;
+          uint64_t tau = s_tau[row];
+          const float v0 = (h0 >> rr) & 1u ? Dt[row * DTS + l] : INFINITY;
+          const float v1 = (h1 >> rr) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;
+          const uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));
+          const uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));
+          uint64_t m0 = ballot(k0 < tau);
+          uint64_t m1 = ballot(k1 < tau);
+          if ((m0 | m1) == 0) continue;
+          uint64_t lk = lkq[q];
+          while (m0) {
+            const int src = __ffsll(static_cast<long long>(m0)) - 1;
+            m0 &= m0 - 1;
+            list_insert(lk, readlane_u64(k0, src), tau, C, l);
+          }
+          while (m1) {
+            const int src = __ffsll(static_cast<long long>(m1)) - 1;
+            m1 &= m1 - 1;
+            list_insert(lk, readlane_u64(k1, src), tau, C, l);
+          }
+          if (l < C) lists[row * C + l] = lk;
+          if (l == 0) {
+            s_tau[row] = tau;
+            s_tauf[row] = tau == KEY_NONE ? INFINITY : key_dist(tau);
+          }
         }
       }
     }
     __syncthreads();
   }
+#undef MV_GLOAD
+#undef MV_LSTORE
 
 }
 
