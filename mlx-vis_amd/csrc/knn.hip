@@ -54,59 +54,90 @@ __global__ void k_zero_i32(int32_t* p, int count) {
   if (t < count) p[t] = 0;
 }
 
-// part[c, t] = sum over rows of chunk c of X[r, t] (FP64).
-__global__ __launch_bounds__(256) void k_colsum_partial(const float* __restrict__ X, int64_t n,
-                                                        int64_t d, double* __restrict__ part) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+// part[c, t] = sum over the rows of chunk c (rows c*rows_per .. ) of X[r, t]
+// in FP64. A block covers 64 columns x 4 row lanes; the row lanes are summed
+// through LDS.
+__global__ __launch_bounds__(256) void k_colsum_partial(const float* __restrict__ X, int64_t n, int64_t d,
+                                                        int64_t rows_per, double* __restrict__ part) {
+  __shared__ double s_p[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * 64 + tx;
   const int64_t c = blockIdx.y;
-  if (t >= d) return;
-  const int64_t r0 = c * ROW_CHUNK;
-  const int64_t r1 = r0 + ROW_CHUNK < n ? r0 + ROW_CHUNK : n;
-  double s = 0.0;
-  for (int64_t r = r0; r < r1; ++r) s += static_cast<double>(X[r * d + t]);
-  part[c * d + t] = s;
+  const int64_t r0 = c * rows_per;
+  const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+  double acc = 0.0;
+  if (t < d)
+    for (int64_t r = r0 + ty; r < r1; r += 4) acc += static_cast<double>(X[r * d + t]);
+  s_p[ty][tx] = acc;
+  __syncthreads();
+  if (ty == 0 && t < d) part[c * d + t] = s_p[0][tx] + s_p[1][tx] + s_p[2][tx] + s_p[3][tx];
 }
 
+// mu[t] = (sum over chunks of part[., t]) / n, one wavefront per column;
+// padding columns get 0.
 __global__ __launch_bounds__(256) void k_colsum_final(const double* __restrict__ part, int64_t nchunks,
                                                       int64_t n, int64_t d, int64_t d_pad,
                                                       float* __restrict__ mu) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
   if (t >= d_pad) return;
   if (t >= d) {
-    mu[t] = 0.f;
+    if (l == 0) mu[t] = 0.f;
     return;
   }
-  double s = 0.0;
-  for (int64_t c = 0; c < nchunks; ++c) s += part[c * d + t];
-  mu[t] = static_cast<float>(s / static_cast<double>(n));
+  double acc = 0.0;
+  for (int64_t c = l; c < nchunks; c += 64) acc += part[c * d + t];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
+  if (l == 0) mu[t] = static_cast<float>(acc / static_cast<double>(n));
 }
 
-// Centred, zero-padded FP32 copy: one wavefront per row. Rows in [n, n_pad)
-// become zero rows. sqmax holds the bit pattern of the largest row norm.
+// Largest of a non-negative float over a block (bit patterns order like the
+// values), then one atomicMax per block.
+__device__ __forceinline__ void block_max_bits(uint32_t bits, uint32_t* dst) {
+  __shared__ uint32_t s_m[4];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), m));
+    bits = o > bits ? o : bits;
+  }
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t v = s_m[0];
+    for (int w = 1; w < 4; ++w) v = s_m[w] > v ? s_m[w] : v;
+    if (dst) atomicMax(dst, v);
+  }
+}
+
+// Centred, zero-padded FP32 copy: one wavefront per row, rows strided over a
+// bounded grid. Rows in [n, n_pad) become zero rows. sqmax receives the bit
+// pattern of the largest row norm (one atomic per block).
 __global__ __launch_bounds__(256) void k_center_f32(const float* __restrict__ X, int64_t n, int64_t d,
                                                     int64_t d_pad, int64_t n_pad,
                                                     const float* __restrict__ mu, float* __restrict__ Xc,
                                                     float* __restrict__ sq, uint32_t* __restrict__ sqmax) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
-  if (r >= n_pad) return;
-  float acc = 0.f;
-  float* out = Xc + r * d_pad;
-  if (r < n) {
-    const float* in = X + r * d;
-    for (int64_t t = l; t < d_pad; t += 64) {
-      float v = t < d ? in[t] - mu[t] : 0.f;
-      out[t] = v;
-      acc = fmaf(v, v, acc);
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * 4;
+  uint32_t mbits = 0;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6); r < n_pad; r += waves) {
+    float acc = 0.f;
+    float* out = Xc + r * d_pad;
+    if (r < n) {
+      const float* in = X + r * d;
+      for (int64_t t = l; t < d_pad; t += 64) {
+        float v = t < d ? in[t] - mu[t] : 0.f;
+        out[t] = v;
+        acc = fmaf(v, v, acc);
+      }
+    } else {
+      for (int64_t t = l; t < d_pad; t += 64) out[t] = 0.f;
     }
-  } else {
-    for (int64_t t = l; t < d_pad; t += 64) out[t] = 0.f;
+    acc = wave_sum_f32(acc);
+    if (l == 0) sq[r] = acc;
+    if (r < n) mbits = __float_as_uint(acc) > mbits ? __float_as_uint(acc) : mbits;
   }
-  acc = wave_sum_f32(acc);
-  if (l == 0) {
-    sq[r] = acc;
-    if (r < n && sqmax) atomicMax(sqmax, __float_as_uint(acc));
-  }
+  block_max_bits(mbits, sqmax);
 }
 
 __global__ __launch_bounds__(256) void k_row_sqnorms(const float* __restrict__ X, int64_t n, int64_t d,
@@ -148,52 +179,57 @@ __global__ __launch_bounds__(256) void k_center_q(const float* __restrict__ X, i
                                                   float* __restrict__ sq, float* __restrict__ err,
                                                   float* __restrict__ inv, uint32_t* __restrict__ sqmax,
                                                   uint32_t* __restrict__ errmax) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
-  if (r >= n_pad) return;
-  const bool live = r < n;
-  const float* in = X + r * d;
-  float scale = 1.f;
-  if (DT == DT_FP8 && live) {
-    float amax = 0.f;
-    for (int64_t t = l; t < d; t += 64) amax = fmaxf(amax, fabsf(in[t] - mu[t]));
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * 4;
+  uint32_t sbits = 0, ebits = 0;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6); r < n_pad; r += waves) {
+    const bool live = r < n;
+    const float* in = X + r * d;
+    float scale = 1.f;
+    if (DT == DT_FP8 && live) {
+      float amax = 0.f;
+      for (int64_t t = l; t < d; t += 64) amax = fmaxf(amax, fabsf(in[t] - mu[t]));
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) amax = fmaxf(amax, __shfl_xor(amax, m));
-    scale = amax > 0.f ? 448.f / amax : 1.f;
-  }
-  const float iv = 1.f / scale;
-  float acc = 0.f, e2 = 0.f, v2 = 0.f;
-  for (int64_t t = l; t < d_pad; t += 64) {
-    const float v = (live && t < d) ? in[t] - mu[t] : 0.f;
-    float xh;
-    if (DT == DT_BF16) {
-      const uint16_t q = bf16_rne(v);
-      reinterpret_cast<uint16_t*>(Xq + r * d_pad * 2)[t] = q;
-      xh = bf16_to_f32(q);
-    } else {
-      const int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v * scale, 0.f, 0, false);
-      reinterpret_cast<uint8_t*>(Xq + r * d_pad)[t] = static_cast<uint8_t>(pk & 0xFF);
-      xh = __builtin_amdgcn_cvt_f32_fp8(pk, 0) * iv;
+      for (int m = 32; m >= 1; m >>= 1) amax = fmaxf(amax, __shfl_xor(amax, m));
+      scale = amax > 0.f ? 448.f / amax : 1.f;
     }
-    acc = fmaf(xh, xh, acc);
-    const float dv = v - xh;
-    e2 = fmaf(dv, dv, e2);
-    v2 = fmaf(v, v, v2);
-  }
-  acc = wave_sum_f32(acc);
-  e2 = wave_sum_f32(e2);
-  v2 = wave_sum_f32(v2);
-  if (l == 0) {
+    const float iv = 1.f / scale;
+    float acc = 0.f, e2 = 0.f, v2 = 0.f;
+    for (int64_t t = l; t < d_pad; t += 64) {
+      const float v = (live && t < d) ? in[t] - mu[t] : 0.f;
+      float xh;
+      if (DT == DT_BF16) {
+        const uint16_t q = bf16_rne(v);
+        reinterpret_cast<uint16_t*>(Xq + r * d_pad * 2)[t] = q;
+        xh = bf16_to_f32(q);
+      } else {
+        const int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v * scale, 0.f, 0, false);
+        reinterpret_cast<uint8_t*>(Xq + r * d_pad)[t] = static_cast<uint8_t>(pk & 0xFF);
+        xh = __builtin_amdgcn_cvt_f32_fp8(pk, 0) * iv;
+      }
+      acc = fmaf(xh, xh, acc);
+      const float dv = v - xh;
+      e2 = fmaf(dv, dv, e2);
+      v2 = fmaf(v, v, v2);
+    }
+    acc = wave_sum_f32(acc);
+    e2 = wave_sum_f32(e2);
+    v2 = wave_sum_f32(v2);
     // sqrt of the FP32 sum with 0.1 % slack, plus the centring rounding bound
     const float eb = live ? sqrtf(e2) * 1.001f + 1.2e-7f * sqrtf(v2) + 1e-20f : 0.f;
-    sq[r] = acc;
-    err[r] = eb;
-    if (inv) inv[r] = iv;
+    if (l == 0) {
+      sq[r] = acc;
+      err[r] = eb;
+      if (inv) inv[r] = iv;
+    }
     if (live) {
-      atomicMax(sqmax, __float_as_uint(acc));
-      atomicMax(errmax, __float_as_uint(eb));
+      sbits = __float_as_uint(acc) > sbits ? __float_as_uint(acc) : sbits;
+      ebits = __float_as_uint(eb) > ebits ? __float_as_uint(eb) : ebits;
     }
   }
+  block_max_bits(sbits, sqmax);
+  __syncthreads();
+  block_max_bits(ebits, errmax);
 }
 
 // Split BF16 copy of the centred FP32 rows: v = hi + lo + r with hi = bf16(v),
@@ -1071,6 +1107,7 @@ __global__ __launch_bounds__(256) void k_exact_scan(ScanArgs a) {
 
 struct Layout {
   int64_t nq_pad, n_pad, d_pad, nchunks;   // d_pad: FP32 row (multiple of 32)
+  int64_t rows_per;                        // rows per column-sum partial
   int64_t dq_pad;                          // quantised row elements (128-byte rows)
   int C, dtype;
   bool same;
@@ -1148,7 +1185,8 @@ static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, b
   L.n_pad = static_cast<int64_t>(align_up(n, BN));
   L.d_pad = static_cast<int64_t>(align_up(d, KC));
   L.dq_pad = dtype == MV_F32 ? L.d_pad : static_cast<int64_t>(align_up(d, 128 / elem_bytes(dtype)));
-  L.nchunks = ceil_div(n, ROW_CHUNK);
+  L.rows_per = n > 512 * ROW_CHUNK ? ceil_div(n, 512) : ROW_CHUNK;   // at most 512 column-sum partials
+  L.nchunks = ceil_div(n, L.rows_per);
   int C = rerank * k;
   if (C < k) C = k;
   if (C > CMAX) C = CMAX;
@@ -1179,6 +1217,13 @@ struct PhaseTimer {
   }
 };
 static thread_local PhaseTimer g_timer;
+
+// Blocks for the row-strided prepass kernels: one wavefront per row up to 4,096
+// blocks, so that their per-block atomics stay few.
+static int64_t prep_grid(int64_t rows) {
+  const int64_t b = ceil_div(rows, 4);
+  return b < 4096 ? b : 4096;
+}
 
 static int cu_count_cached() {
   static int cached[64] = {0};
@@ -1303,19 +1348,18 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, out_stats, 4);
     MV_LAUNCHED("k_zero_i32");
   }
-  hipLaunchKernelGGL(k_colsum_partial, dim3(static_cast<unsigned>(ceil_div(d, 256)),
-                                            static_cast<unsigned>(L.nchunks)),
-                     dim3(256), 0, s, X, n, d, B.part);
+  hipLaunchKernelGGL(k_colsum_partial, dim3(static_cast<unsigned>(ceil_div(d, 64)), static_cast<unsigned>(L.nchunks)),
+                     dim3(256), 0, s, X, n, d, L.rows_per, B.part);
   MV_LAUNCHED("k_colsum_partial");
   const int64_t mu_len = L.d_pad > L.dq_pad ? L.d_pad : L.dq_pad;
-  hipLaunchKernelGGL(k_colsum_final, dim3(static_cast<unsigned>(ceil_div(mu_len, 256))), dim3(256), 0, s, B.part,
+  hipLaunchKernelGGL(k_colsum_final, dim3(static_cast<unsigned>(ceil_div(mu_len, 4))), dim3(256), 0, s, B.part,
                      L.nchunks, n, d, mu_len, B.mu);
   MV_LAUNCHED("k_colsum_final");
-  hipLaunchKernelGGL(k_center_f32, dim3(static_cast<unsigned>(L.n_pad / 4)), dim3(256), 0, s, X, n, d, L.d_pad,
-                     L.n_pad, B.mu, B.Xc, B.sqx, sqmax32);
+  hipLaunchKernelGGL(k_center_f32, dim3(static_cast<unsigned>(prep_grid(L.n_pad))), dim3(256), 0, s, X, n, d,
+                     L.d_pad, L.n_pad, B.mu, B.Xc, B.sqx, sqmax32);
   MV_LAUNCHED("k_center_f32");
   if (!same) {
-    hipLaunchKernelGGL(k_center_f32, dim3(static_cast<unsigned>(L.nq_pad / 4)), dim3(256), 0, s, Q, nq, d,
+    hipLaunchKernelGGL(k_center_f32, dim3(static_cast<unsigned>(prep_grid(L.nq_pad))), dim3(256), 0, s, Q, nq, d,
                        L.d_pad, L.nq_pad, B.mu, B.Qc, B.sqq, static_cast<uint32_t*>(nullptr));
     MV_LAUNCHED("k_center_f32");
   }
@@ -1332,12 +1376,12 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     }
   } else if (dtype != MV_F32) {
     auto kq = dtype == MV_BF16 ? k_center_q<DT_BF16> : k_center_q<DT_FP8>;
-    hipLaunchKernelGGL(kq, dim3(static_cast<unsigned>(L.n_pad / 4)), dim3(256), 0, s, X, n, d, L.dq_pad, L.n_pad,
+    hipLaunchKernelGGL(kq, dim3(static_cast<unsigned>(prep_grid(L.n_pad))), dim3(256), 0, s, X, n, d, L.dq_pad, L.n_pad,
                        B.mu, B.Xq, B.sqxq, B.ex, B.invx, sqmaxq, emax);
     MV_LAUNCHED("k_center_q");
     if (!same) {
       uint32_t* dummy = reinterpret_cast<uint32_t*>(B.ctr + 5);
-      hipLaunchKernelGGL(kq, dim3(static_cast<unsigned>(L.nq_pad / 4)), dim3(256), 0, s, Q, nq, d, L.dq_pad,
+      hipLaunchKernelGGL(kq, dim3(static_cast<unsigned>(prep_grid(L.nq_pad))), dim3(256), 0, s, Q, nq, d, L.dq_pad,
                          L.nq_pad, B.mu, B.Qq, B.sqqq, B.eq, B.invq, dummy, dummy);
       MV_LAUNCHED("k_center_q");
     }

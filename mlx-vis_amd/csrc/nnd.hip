@@ -486,6 +486,8 @@ __global__ __launch_bounds__(256) void k_rev_fill(const int32_t* __restrict__ id
   }
 }
 
+constexpr uint32_t REV_THREAD_MAX = 64;   // in-degree handled by one thread in k_rev_select
+
 // rev[j, t] = source row of the t-th smallest flat edge position pointing at
 // j (t < min(k, count)); remaining slots hold 0.
 __global__ __launch_bounds__(256) void k_rev_select(const uint32_t* __restrict__ bucket,
@@ -495,6 +497,7 @@ __global__ __launch_bounds__(256) void k_rev_select(const uint32_t* __restrict__
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t c = cnt[j];
+  if (c > REV_THREAD_MAX) return;   // hubs: k_rev_select_hub
   const uint32_t* b = bucket + off[j];
   int64_t prev = -1;
   for (int t = 0; t < k; ++t) {
@@ -509,6 +512,37 @@ __global__ __launch_bounds__(256) void k_rev_select(const uint32_t* __restrict__
       prev = best;
     }
     rev[j * k + t] = best >= 0 ? static_cast<int32_t>(best / k) : 0;
+  }
+}
+
+// Destinations with more than REV_THREAD_MAX in-edges (hubs): one wavefront
+// each, k rounds of a wave-wide minimum over the entries above the previous
+// pick. Wavefronts stride over the destinations of a bounded grid.
+__global__ __launch_bounds__(256) void k_rev_select_hub(const uint32_t* __restrict__ bucket,
+                                                        const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ cnt, int64_t n, int k,
+                                                        int32_t* __restrict__ rev) {
+  const int l = threadIdx.x & 63;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * 4;
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6); j < n; j += waves) {
+    const uint32_t c = cnt[j];
+    if (c <= REV_THREAD_MAX) continue;
+    const uint32_t* b = bucket + off[j];
+    int64_t prev = -1;
+    for (int t = 0; t < k; ++t) {
+      uint32_t lm = 0xFFFFFFFFu;
+      for (uint32_t e = l; e < c; e += 64) {
+        const uint32_t v = b[e];
+        if (static_cast<int64_t>(v) > prev && v < lm) lm = v;
+      }
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(lm), m));
+        lm = o < lm ? o : lm;
+      }
+      prev = lm;
+      if (l == 0) rev[j * k + t] = static_cast<int32_t>(lm / static_cast<uint32_t>(k));
+    }
   }
 }
 
@@ -628,6 +662,10 @@ static int reverse_candidates(const Plan& P, const int32_t* idx, int32_t* rev, h
   hipLaunchKernelGGL(k_rev_select, dim3(bn), dim3(256), 0, s, P.bucket, P.off, P.cnt, n, static_cast<int>(P.k),
                      rev);
   MV_LAUNCHED("k_rev_select");
+  const int64_t hgrid = ceil_div(n, 4) < 2048 ? ceil_div(n, 4) : 2048;
+  hipLaunchKernelGGL(k_rev_select_hub, dim3(static_cast<unsigned>(hgrid)), dim3(256), 0, s, P.bucket, P.off, P.cnt, n,
+                     static_cast<int>(P.k), rev);
+  MV_LAUNCHED("k_rev_select_hub");
   return MV_OK;
 }
 

@@ -98,9 +98,12 @@ __global__ __launch_bounds__(256) void k_fuzzy_rows(const float* __restrict__ di
 __global__ __launch_bounds__(256) void k_fuzzy_sym(const int32_t* __restrict__ knn, const float* __restrict__ w,
                                                    int64_t n, int k, float* __restrict__ wsym,
                                                    uint32_t* __restrict__ wmax_bits) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  float v = 0.f;
-  if (e < n * k) {
+  // slots strided over a bounded grid; the block's maximum is added with one
+  // atomic (weights are >= 0, so their IEEE bit patterns order like the values)
+  __shared__ uint32_t s_m[4];
+  uint32_t bits = 0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n * k; e += stride) {
     const int64_t i = e / k;
     const int64_t j = knn[e];
     const float a = w[e];
@@ -113,43 +116,58 @@ __global__ __launch_bounds__(256) void k_fuzzy_sym(const int32_t* __restrict__ k
           break;
         }
     }
-    v = a + b - a * b;
+    const float v = a + b - a * b;
     wsym[e] = v;
+    bits = __float_as_uint(v) > bits ? __float_as_uint(v) : bits;
   }
-  // wavefront max, then one atomic per wavefront (weights are >= 0, so the
-  // IEEE bit patterns order like the values)
-  uint32_t bits = __float_as_uint(v);
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
-    uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), m));
+    const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), m));
     bits = o > bits ? o : bits;
   }
-  if ((threadIdx.x & 63) == 0) atomicMax(wmax_bits, bits);
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t v = s_m[0];
+    for (int q = 1; q < 4; ++q) v = s_m[q] > v ? s_m[q] : v;
+    atomicMax(wmax_bits, v);
+  }
 }
 
-// Prune and schedule (umap.py:265-270, 400-404) in FP32.
+// Prune and schedule each slot: kept when w_sym >= w_max / n_epochs
+// (umap.py:265), eps = n_epochs / (n_epochs * (w / w_max)) (umap.py:402-403),
+// epn = eps; pruned slots get eps = -1, epn = +inf. n_edges counts kept slots
+// (one atomic per block).
 __global__ __launch_bounds__(256) void k_fuzzy_sched(const float* __restrict__ wsym, int64_t count,
                                                      int n_epochs, const uint32_t* __restrict__ wmax_bits,
                                                      float* __restrict__ eps, float* __restrict__ epn,
                                                      uint32_t* __restrict__ n_edges) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool keep = false;
-  if (e < count) {
-    const float wmax = __uint_as_float(*wmax_bits);
-    const float ne = static_cast<float>(n_epochs);
-    const float thr = __fdiv_rn(wmax, ne);
+  __shared__ uint32_t s_c[4];
+  const float wmax = __uint_as_float(*wmax_bits);
+  const float ne = static_cast<float>(n_epochs);
+  const float thr = __fdiv_rn(wmax, ne);
+  uint32_t kept = 0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < count; e += stride) {
     const float v = wsym[e];
-    keep = v >= thr;
-    float s = -1.f;
+    const bool keep = v >= thr;
+    float sv = -1.f;
     if (keep) {
       const float ns = __fmul_rn(ne, __fdiv_rn(v, wmax));
-      s = ns > 0.f ? __fdiv_rn(ne, ns) : -1.f;
+      sv = ns > 0.f ? __fdiv_rn(ne, ns) : -1.f;
     }
-    eps[e] = keep ? s : -1.f;
-    epn[e] = keep ? s : INFINITY;
+    eps[e] = keep ? sv : -1.f;
+    epn[e] = keep ? sv : INFINITY;
+    kept += keep ? 1u : 0u;
   }
-  const uint64_t m = ballot(keep);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(n_edges, static_cast<uint32_t>(__popcll(m)));
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) kept += static_cast<uint32_t>(__shfl_xor(static_cast<int>(kept), m));
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = kept;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t t = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    if (t) atomicAdd(n_edges, t);
+  }
 }
 
 // -------------------------------------------------------------- gradients --
@@ -762,11 +780,12 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
   hipLaunchKernelGGL(k_fuzzy_rows, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, knn_dist, n, k,
                      static_cast<float>(log2(static_cast<double>(k))), w, sigma_out, rho_out);
   MV_LAUNCHED("k_fuzzy_rows");
-  hipLaunchKernelGGL(k_fuzzy_sym, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, knn_idx, w,
-                     n, k, w_sym, counters + 1);
+  const int64_t fgrid = ceil_div(count, 256) < 4096 ? ceil_div(count, 256) : 4096;
+  hipLaunchKernelGGL(k_fuzzy_sym, dim3(static_cast<unsigned>(fgrid)), dim3(256), 0, s, knn_idx, w, n, k, w_sym,
+                     counters + 1);
   MV_LAUNCHED("k_fuzzy_sym");
-  hipLaunchKernelGGL(k_fuzzy_sched, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, w_sym,
-                     count, n_epochs, counters + 1, eps, epn, counters);
+  hipLaunchKernelGGL(k_fuzzy_sched, dim3(static_cast<unsigned>(fgrid)), dim3(256), 0, s, w_sym, count, n_epochs,
+                     counters + 1, eps, epn, counters);
   MV_LAUNCHED("k_fuzzy_sched");
   return MV_OK;
 }
