@@ -54,6 +54,7 @@ __global__ void k_zero_i32(int32_t* p, int count) {
   if (t < count) p[t] = 0;
 }
 
+
 // part[c, t] = sum over the rows of chunk c (rows c*rows_per .. ) of X[r, t]
 // in FP64. A block covers 64 columns x 4 row lanes; the row lanes are summed
 // through LDS.
@@ -298,6 +299,33 @@ __device__ __forceinline__ void list_insert(uint64_t& lk, uint64_t kv, uint64_t&
   tau = readlane_u64(lk, C - 1);
 }
 
+// Sorts a bitonic sequence of one key per lane ascending (half-cleaners).
+__device__ __forceinline__ uint64_t bitonic_clean_u64(uint64_t v, int l) {
+#pragma unroll
+  for (int j = 32; j >= 1; j >>= 1) {
+    const uint64_t o = shfl_xor_u64(v, j);
+    v = (l & j) ? (o > v ? o : v) : (o < v ? o : v);
+  }
+  return v;
+}
+
+// Survivor count above which a row's candidates are merged into its list by
+// sorting instead of one insertion each.
+constexpr int BULK_MIN = 24;
+
+// The 64 smallest of the sorted list `lk` (one key per lane, ascending,
+// KEY_NONE padded) and the unsorted keys a, b, ascending, one per lane.
+__device__ __forceinline__ uint64_t list_merge128(uint64_t lk, uint64_t a, uint64_t b, int l) {
+  a = wave_sort_u64(a);
+  b = wave_sort_u64(b);
+  const uint64_t br = shfl_u64(b, 63 - l);
+  uint64_t c = a < br ? a : br;   // the 64 smallest of a and b, bitonic
+  c = bitonic_clean_u64(c, l);
+  const uint64_t cr = shfl_u64(c, 63 - l);
+  uint64_t m = lk < cr ? lk : cr;   // the 64 smallest of the list and c, bitonic
+  return bitonic_clean_u64(m, l);
+}
+
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 
@@ -391,8 +419,7 @@ __device__ __forceinline__ void mfma_chunk(const float* As, const float* Bs, int
 template <int DT>
 __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
-  __shared__ uint64_t s_tau[BM];
-  __shared__ float s_tauf[BM];          // distance of s_tau (+inf while the list is not full)
+  __shared__ float s_tauf[BM];          // C-th list distance (+inf while the list is not full)
   __shared__ uint64_t s_hit[2][2];      // [column half][row half]: rows with survivors
   __shared__ float s_sqq[BM];
   __shared__ int s_self[BM];
@@ -425,7 +452,6 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
       else if (a.self_base >= 0) self = a.self_base + grow;
     }
     s_self[r] = static_cast<int>(self);
-    s_tau[r] = KEY_NONE;
     s_tauf[r] = INFINITY;
   }
   // The candidate lists live in the output buffer (this workgroup's rows of
@@ -584,7 +610,7 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
           const int rr = rrq[q];
           if (rr < 0) break;
           const int row = w * 32 + rr;
-          uint64_t tau = s_tau[row];
+          uint64_t tau = readlane_u64(lkq[q], C - 1);   // the list's C-th key (KEY_NONE until full)
           const float v0 = (h0 >> rr) & 1u ? Dt[row * DTS + l] : INFINITY;
           const float v1 = (h1 >> rr) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;
           const uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));
@@ -593,21 +619,26 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
           uint64_t m1 = ballot(k1 < tau);
           if ((m0 | m1) == 0) continue;
           uint64_t lk = lkq[q];
-          while (m0) {
-            const int src = __ffsll(static_cast<long long>(m0)) - 1;
-            m0 &= m0 - 1;
-            list_insert(lk, readlane_u64(k0, src), tau, C, l);
-          }
-          while (m1) {
-            const int src = __ffsll(static_cast<long long>(m1)) - 1;
-            m1 &= m1 - 1;
-            list_insert(lk, readlane_u64(k1, src), tau, C, l);
+          if (__popcll(m0) + __popcll(m1) > BULK_MIN) {
+            // many survivors (a list still filling): sort both halves, keep the
+            // 64 smallest, and merge them with the list in one pass
+            lk = list_merge128(lk, k0 < tau ? k0 : KEY_NONE, k1 < tau ? k1 : KEY_NONE, l);
+            if (l >= C) lk = KEY_NONE;
+            tau = readlane_u64(lk, C - 1);
+          } else {
+            while (m0) {
+              const int src = __ffsll(static_cast<long long>(m0)) - 1;
+              m0 &= m0 - 1;
+              list_insert(lk, readlane_u64(k0, src), tau, C, l);
+            }
+            while (m1) {
+              const int src = __ffsll(static_cast<long long>(m1)) - 1;
+              m1 &= m1 - 1;
+              list_insert(lk, readlane_u64(k1, src), tau, C, l);
+            }
           }
           if (l < C) lists[row * C + l] = lk;
-          if (l == 0) {
-            s_tau[row] = tau;
-            s_tauf[row] = tau == KEY_NONE ? INFINITY : key_dist(tau);
-          }
+          if (l == 0 && tau != KEY_NONE) s_tauf[row] = key_dist(tau);
         }
       }
     }
