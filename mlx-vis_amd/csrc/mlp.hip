@@ -48,39 +48,66 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
 
-enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_GRAD = 2, EPI_BF16 = 3 };
+enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_GRAD = 2, EPI_BF16 = 3, EPI_F32_ADD = 4, EPI_HEAD = 5 };
 
 struct GemmArgs {
-  const bf16_t* A;   // M x K, row stride lda (elements)
+  const void* A;     // M x K, row stride lda (elements): BF16, or FP32 when the kernel's AF32 is set
   const bf16_t* B;   // N x K, row stride ldb
   int M, N, K;       // K multiple of 64
+  int kchunks;       // K chunks (of 64) per split along grid z
+  int kvalid;        // AF32: A elements per row that exist (a multiple of 8); later ones read as 0
   int64_t lda, ldb, ldc, ldx;
   void* C;           // F32 or BF16 output, row stride ldc
-  const float* bias;       // EPI_BIAS_GELU: bias[N]
+  const float* bias;       // EPI_BIAS_GELU / EPI_HEAD: bias[N]
   bf16_t* pre_out;         // EPI_BIAS_GELU (nullable): pre-activation output (row stride ldx)
   const bf16_t* pre_in;    // EPI_GELU_GRAD: pre-activation input (row stride ldx)
   float* colsum;           // EPI_GELU_GRAD (nullable): colsum[col] += sum over rows of the output
+  const float* W3;         // EPI_HEAD: output layer (dout x N, FP32)
+  int dout;                // EPI_HEAD: outputs (<= 4); C is Z (M x dout FP32), pre-filled with b3
 };
 
-template <int EPI>
+// Eight FP32 values rounded to BF16 and packed into 16 bytes.
+__device__ __forceinline__ float4 pack_bf16x8(float4 a, float4 b) {
+  const uint32_t p0 = static_cast<uint32_t>(f2bf(a.x)) | (static_cast<uint32_t>(f2bf(a.y)) << 16);
+  const uint32_t p1 = static_cast<uint32_t>(f2bf(a.z)) | (static_cast<uint32_t>(f2bf(a.w)) << 16);
+  const uint32_t p2 = static_cast<uint32_t>(f2bf(b.x)) | (static_cast<uint32_t>(f2bf(b.y)) << 16);
+  const uint32_t p3 = static_cast<uint32_t>(f2bf(b.z)) | (static_cast<uint32_t>(f2bf(b.w)) << 16);
+  return make_float4(__uint_as_float(p0), __uint_as_float(p1), __uint_as_float(p2), __uint_as_float(p3));
+}
+
+// C = A B^T on v_mfma_f32_16x16x32_bf16: 128 x 128 output tiles, four
+// wavefronts of 64 x 64, 64-element K chunks double-buffered in LDS with a one
+// chunk deep register prefetch. With AF32 the A operand is read as FP32 and
+// rounded to BF16 (ties to even) while it is staged. Grid z splits K
+// (EPI_F32_ADD accumulates the partial products with atomics). Epilogues:
+//   EPI_F32 / EPI_BF16   store C;
+//   EPI_F32_ADD          C += partial (C zeroed beforehand);
+//   EPI_BIAS_GELU        pre = bf16(acc + bias), C = bf16(gelu(pre));
+//   EPI_GELU_GRAD        C = bf16(acc * gelu'(pre_in)), colsum += C;
+//   EPI_HEAD             h = bf16(gelu(bf16(acc + bias))), Z[row, o] += sum_col h W3[o, col].
+template <int EPI, bool AF32>
 __global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) float s_stage[2 * 2 * BM * LDSA];
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63;
   const int wr = w >> 1, wc = w & 1;
   const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
-  const int lr = tid >> 3, lkb = (tid & 7) * 16;   // staging row, byte offset in the 128-byte chunk
-  const int nk = g.K / 64;
+  const int lr = tid >> 3, lq = tid & 7;   // staging row, 16-byte slot in the 128-byte BF16 chunk row
+  const int nk_all = g.K / 64;
+  const int kc0 = blockIdx.z * g.kchunks;
+  const int kc1 = min(nk_all, kc0 + g.kchunks);
+  if (kc0 >= kc1) return;
 
   // Rows past M / N are clamped (read in bounds, never stored).
   const char* Arow[4];
   const char* Brow[4];
+  constexpr int AEB = AF32 ? 4 : 2;   // bytes per A element
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int ra = min(bm + lr + 32 * i, g.M - 1);
     const int rbn = min(bn + lr + 32 * i, g.N - 1);
-    Arow[i] = reinterpret_cast<const char*>(g.A + static_cast<int64_t>(ra) * g.lda) + lkb;
-    Brow[i] = reinterpret_cast<const char*>(g.B + static_cast<int64_t>(rbn) * g.ldb) + lkb;
+    Arow[i] = static_cast<const char*>(g.A) + static_cast<int64_t>(ra) * g.lda * AEB + lq * 8 * AEB;
+    Brow[i] = reinterpret_cast<const char*>(g.B + static_cast<int64_t>(rbn) * g.ldb) + lq * 16;
   }
   v4f acc[4][4];
 #pragma unroll
@@ -88,26 +115,61 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  float4 ra[4], rv[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    ra[i] = *reinterpret_cast<const float4*>(Arow[i]);
-    rv[i] = *reinterpret_cast<const float4*>(Brow[i]);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    *reinterpret_cast<float4*>(s_stage + (lr + 32 * i) * LDSA + lkb / 4) = ra[i];
-    *reinterpret_cast<float4*>(s_stage + BM * LDSA + (lr + 32 * i) * LDSA + lkb / 4) = rv[i];
-  }
+  float4 ra0, ra1, ra2, ra3;   // A slot (BF16 chunk, or the first half of an FP32 slot)
+  float4 rx0, rx1, rx2, rx3;   // FP32 A: second half of the slot
+  float4 rv0, rv1, rv2, rv3;   // B slot
+  int kcur = kc0;              // chunk held in the prefetch registers
+  const char* Ar0 = Arow[0];
+  const char* Ar1 = Arow[1];
+  const char* Ar2 = Arow[2];
+  const char* Ar3 = Arow[3];
+  const char* Br0 = Brow[0];
+  const char* Br1 = Brow[1];
+  const char* Br2 = Brow[2];
+  const char* Br3 = Brow[3];
+  // FP32 A: an 8-element slot past the row's valid length is read at the row
+  // start (in bounds) and zeroed when staged.
+#define MV_GLOAD1(I, KC, AVAL)                                                                     \
+  do {                                                                                           \
+    const char* pa_ = (AVAL) ? Ar##I + static_cast<int64_t>(KC) * 64 * AEB : Ar##I - lq * 8 * AEB; \
+    ra##I = *reinterpret_cast<const float4*>(pa_);                                               \
+    if (AF32) rx##I = *reinterpret_cast<const float4*>(pa_ + 16);                                \
+    rv##I = *reinterpret_cast<const float4*>(Br##I + static_cast<int64_t>(KC) * 128);            \
+  } while (0)
+#define MV_GLOAD(KC)                                                                               \
+  do {                                                                                           \
+    const bool aval_ = !AF32 || (KC) * 64 + lq * 8 < g.kvalid;                                   \
+    MV_GLOAD1(0, KC, aval_);                                                                     \
+    MV_GLOAD1(1, KC, aval_);                                                                     \
+    MV_GLOAD1(2, KC, aval_);                                                                     \
+    MV_GLOAD1(3, KC, aval_);                                                                     \
+  } while (0)
+#define MV_LSTORE1(I, AS, BS)                                                                      \
+  do {                                                                                           \
+    float4 av_ = ra##I;                                                                          \
+    if (AF32) av_ = kcur * 64 + lq * 8 >= g.kvalid ? make_float4(0.f, 0.f, 0.f, 0.f) : pack_bf16x8(ra##I, rx##I); \
+    *reinterpret_cast<float4*>((AS) + (lr + 32 * I) * LDSA + lq * 4) = av_;                      \
+    *reinterpret_cast<float4*>((BS) + (lr + 32 * I) * LDSA + lq * 4) = rv##I;                    \
+  } while (0)
+#define MV_LSTORE(BUF)                                                                             \
+  do {                                                                                           \
+    float* As2_ = s_stage + (BUF) * (2 * BM * LDSA);                                             \
+    float* Bs2_ = As2_ + BM * LDSA;                                                              \
+    MV_LSTORE1(0, As2_, Bs2_);                                                                   \
+    MV_LSTORE1(1, As2_, Bs2_);                                                                   \
+    MV_LSTORE1(2, As2_, Bs2_);                                                                   \
+    MV_LSTORE1(3, As2_, Bs2_);                                                                   \
+  } while (0)
+  MV_GLOAD(kc0);
+  MV_LSTORE(0);
   __syncthreads();
-  for (int kc = 0; kc < nk; ++kc) {
-    const int cur = kc & 1;
-    const int kn = kc + 1 < nk ? kc + 1 : kc;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      ra[i] = *reinterpret_cast<const float4*>(Arow[i] + static_cast<int64_t>(kn) * 128);
-      rv[i] = *reinterpret_cast<const float4*>(Brow[i] + static_cast<int64_t>(kn) * 128);
-    }
+  for (int kc = kc0; kc < kc1; ++kc) {
+    const int cur = (kc - kc0) & 1;
+    kcur = kc + 1 < kc1 ? kc + 1 : kc;
+    MV_GLOAD(kcur);
+    // keep the loads ahead of the multiplies (the scheduler would otherwise
+    // sink them past the MFMAs to reuse operand registers, exposing their latency)
+    __builtin_amdgcn_sched_barrier(0);
     const float* As = s_stage + cur * (2 * BM * LDSA);
     const float* Bs = As + BM * LDSA;
 #pragma unroll
@@ -127,16 +189,51 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
         for (int ni = 0; ni < 4; ++ni)
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
     }
-    {
-      float* As2 = s_stage + (cur ^ 1) * (2 * BM * LDSA);
-      float* Bs2 = As2 + BM * LDSA;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        *reinterpret_cast<float4*>(As2 + (lr + 32 * i) * LDSA + lkb / 4) = ra[i];
-        *reinterpret_cast<float4*>(Bs2 + (lr + 32 * i) * LDSA + lkb / 4) = rv[i];
-      }
-    }
+    if (kc + 1 < kc1) MV_LSTORE(cur ^ 1);
     __syncthreads();
+  }
+#undef MV_GLOAD
+#undef MV_GLOAD1
+#undef MV_LSTORE
+#undef MV_LSTORE1
+
+  if constexpr (EPI == EPI_HEAD) {
+    float w3[4][4];   // [o][ni]
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) {
+      const int col = min(bn + wc * 64 + ni * 16 + (l & 15), g.N - 1);
+#pragma unroll
+      for (int o = 0; o < 4; ++o) w3[o][ni] = o < g.dout ? g.W3[o * g.N + col] : 0.f;
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = bm + wr * 64 + mi * 16 + (l >> 4) * 4 + r;
+        float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+          const int col = bn + wc * 64 + ni * 16 + (l & 15);
+          if (col < g.N) {
+            const float h = bf2f(f2bf(gelu(bf2f(f2bf(acc[mi][ni][r] + g.bias[col])))));
+#pragma unroll
+            for (int o = 0; o < 4; ++o) part[o] = fmaf(h, w3[o][ni], part[o]);
+          }
+        }
+        // the 16 lanes of a row group hold its 64 columns
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+          float v = part[o];
+          v += __shfl_xor(v, 1);
+          v += __shfl_xor(v, 2);
+          v += __shfl_xor(v, 4);
+          v += __shfl_xor(v, 8);
+          part[o] = v;
+        }
+        if ((l & 15) == 0 && row < g.M)
+          for (int o = 0; o < g.dout; ++o) atomicAdd(&static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.dout + o], part[o]);
+      }
+    return;
   }
 
   float csum[4] = {0.f, 0.f, 0.f, 0.f};   // EPI_GELU_GRAD: column sums of this lane's outputs
@@ -153,6 +250,8 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
         const float v = acc[mi][ni][r];
         if (EPI == EPI_F32) {
           static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = v;
+        } else if (EPI == EPI_F32_ADD) {
+          atomicAdd(&static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col], v);
         } else if (EPI == EPI_BF16) {
           static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = f2bf(v);
         } else if (EPI == EPI_BIAS_GELU) {
@@ -573,12 +672,27 @@ static Batch carve_batch(void* ws, const Dims& D, int P, int64_t np, size_t* tot
   return b;
 }
 
-template <int EPI>
-static int gemm(const GemmArgs& g, hipStream_t s) {
-  dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)));
-  hipLaunchKernelGGL((k_gemm_nt<EPI>), grid, dim3(NT), 0, s, g);
+// Launches k_gemm_nt over ceil(N/128) x ceil(M/128) tiles and `splits` K
+// ranges (splits > 1 only with EPI_F32_ADD).
+template <int EPI, bool AF32 = false>
+static int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
+  const int nk = g.K / 64;
+  if (splits < 1) splits = 1;
+  if (splits > nk) splits = nk;
+  g.kchunks = static_cast<int>(ceil_div(nk, splits));
+  splits = static_cast<int>(ceil_div(nk, g.kchunks));
+  dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)),
+            static_cast<unsigned>(splits));
+  hipLaunchKernelGGL((k_gemm_nt<EPI, AF32>), grid, dim3(NT), 0, s, g);
   MV_LAUNCHED("k_gemm_nt");
   return MV_OK;
+}
+
+// K splits for a weight-gradient GEMM: about 256 workgroups in total.
+static int grad_splits(int M, int N, int K) {
+  const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
+  int sp = static_cast<int>(256 / (tiles > 0 ? tiles : 1));
+  return sp < 1 ? 1 : sp;
 }
 
 static int transpose(const bf16_t* in, int rows, int cols, int64_t ldi, bf16_t* out, int64_t ldo, hipStream_t s) {
@@ -623,6 +737,57 @@ static int forward(const Params& W, const Dims& D, const bf16_t* Xb, int P, bf16
   return MV_OK;
 }
 
+// Z[r, o] = b3[o] for the rows of an inference chunk (the fused head adds to it).
+__global__ void k_init_rows(float* __restrict__ Z, int64_t rows, int dout, const float* __restrict__ b3) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < rows * dout) Z[t] = b3[t % dout];
+}
+
+// Inference of `rows` FP32 rows (row stride din): GEMM 1 reads the FP32 rows
+// directly (rounded to BF16 while staged; din a multiple of 8), GEMM 2 ends in
+// the output layer (no H2 in memory). Xb is used only when din % 8 != 0.
+static int forward_infer(const Params& W, const Dims& D, const float* X, int rows, bf16_t* Xb, bf16_t* H1, float* Z,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_init_rows, dim3(static_cast<unsigned>(ceil_div(static_cast<int64_t>(rows) * D.dout, 256))),
+                     dim3(256), 0, s, Z, static_cast<int64_t>(rows), D.dout, W.b3);
+  MV_LAUNCHED("k_init_rows");
+  GemmArgs g{};
+  g.B = W.W1b;
+  g.M = rows;
+  g.N = D.h1;
+  g.K = D.dp;
+  g.ldb = D.dp;
+  g.C = H1;
+  g.ldc = D.h1;
+  g.bias = W.b1;
+  if (D.din % 8 == 0) {
+    g.A = X;
+    g.lda = D.din;
+    g.kvalid = D.din;
+    if (int rc = gemm<EPI_BIAS_GELU, true>(g, s)) return rc;
+  } else {
+    hipLaunchKernelGGL(k_to_bf16, dim3(static_cast<unsigned>(ceil_div(rows, 4))), dim3(256), 0, s, X,
+                       static_cast<int64_t>(rows), D.din, D.dp, Xb);
+    MV_LAUNCHED("k_to_bf16");
+    g.A = Xb;
+    g.lda = D.dp;
+    if (int rc = gemm<EPI_BIAS_GELU>(g, s)) return rc;
+  }
+  g = GemmArgs{};
+  g.A = H1;
+  g.B = W.W2b;
+  g.M = rows;
+  g.N = D.h2;
+  g.K = D.h1;
+  g.lda = D.h1;
+  g.ldb = D.h1;
+  g.C = Z;
+  g.bias = W.b2;
+  g.W3 = W.W3;
+  g.dout = D.dout;
+  return gemm<EPI_HEAD>(g, s);
+}
+
 // Backward from dZ into the gradient buffer (zeroed here).
 static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64_t np, hipStream_t s) {
   float* gW1 = b.grad;
@@ -656,7 +821,7 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   g.ldb = P;
   g.C = gW2;
   g.ldc = D.h1;
-  if (int rc = gemm<EPI_F32>(g, s)) return rc;
+  if (int rc = gemm<EPI_F32_ADD>(g, s, grad_splits(g.M, g.N, g.K))) return rc;
   // dA1 (P x h1) = (dA2 W2) * gelu'(A1) = dA2 (W2^T)^T
   g = GemmArgs{};
   g.A = b.dA2;
@@ -685,7 +850,7 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   g.ldb = P;
   g.C = gW1;
   g.ldc = D.dp;
-  return gemm<EPI_F32>(g, s);
+  return gemm<EPI_F32_ADD>(g, s, grad_splits(g.M, g.N, g.K));
 }
 
 static int adam_all(const Params& W, const Dims& D, const float* grad, int64_t np, float lr, int step, hipStream_t s) {
@@ -900,11 +1065,7 @@ int mv_mlp_forward(const void* blob, int din, int h1, int h2, int dout, const fl
   Batch bt = carve_batch(ws, D, chunk, np, nullptr);
   for (int64_t r0 = 0; r0 < m; r0 += chunk) {
     const int rows = static_cast<int>(m - r0 < chunk ? m - r0 : chunk);
-    hipLaunchKernelGGL(k_to_bf16, dim3(static_cast<unsigned>(ceil_div(rows, 4))), dim3(256), 0, s, X + r0 * din,
-                       static_cast<int64_t>(rows), din, D.dp, bt.Xb);
-    MV_LAUNCHED("k_to_bf16");
-    // inference keeps no pre-activations
-    if (int rc = forward(W, D, bt.Xb, rows, nullptr, bt.H1, nullptr, bt.H2, Z + r0 * dout, s)) return rc;
+    if (int rc = forward_infer(W, D, X + r0 * din, rows, bt.Xb, bt.H1, Z + r0 * dout, s)) return rc;
   }
   return MV_OK;
 }
