@@ -466,6 +466,21 @@ def main():
                         "identical_indices": bool(torch.equal(idx_dt, ref_idx)),
                         "rows_uncertified_first_pass": st_dt[2], "rows_uncertified_after_fp32_rescreen": st_dt[3],
                         "rows_fp64_scan": st_dt[0]}
+        # the whole fit_transform with the split-BF16 screen (same certified kNN)
+        def step_x3():
+            um3 = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=args.epochs, negative_sample_rate=5,
+                       knn_method="brute", random_state=42, knn_dtype="bf16x3")
+            um3.fit_transform(Xd)
+            return um3
+        step_x3()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            um3 = step_x3()
+        torch.cuda.synchronize()
+        prec["fit_transform_bf16x3_screen"] = {
+            "s": round((time.perf_counter() - t) / max(args.steps, 1), 4),
+            "identical_knn_indices": bool(torch.equal(um3.knn_[0], um.knn_[0]))}
         line["precision"] = prec
         log("NNDescent k=15, 12 iterations, max_candidates=30")
         from mlx_vis_amd import NNDescent

@@ -249,6 +249,11 @@ int mv_mlp_sample_edges(const float* eps, const int32_t* tail, int64_t n, int k,
 int mv_knn_overlap(const int32_t* a, int64_t lda, int ka, const int32_t* b, int64_t ldb, int kb, int64_t n,
                    int min_count, int32_t* out_count, uint8_t* out_flag, void* stream);
 
+/* out[0:count] = standard normal draws (Philox-4x32-10 keyed by seed, Box-Muller):
+ * the spectral start vectors, which the reference draws from MLX's generator
+ * (umap.py:325), a stream that cannot be reproduced outside MLX. */
+int mv_random_normal(float* out, int64_t count, uint64_t seed, void* stream);
+
 /* p[0:count] = 0 on the stream (a fill kernel, safe under graph capture). */
 int mv_zero_f32(float* p, int64_t count, void* stream);
 

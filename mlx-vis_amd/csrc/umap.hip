@@ -743,6 +743,26 @@ static int run_epoch(float* Y, const EpochBufs& B, const int32_t* tail, const fl
   return MV_OK;
 }
 
+// Standard normal draws for the spectral start vectors: Philox-4x32-10 on the
+// counter (i, 0x5A11, 0, 0) under key (seed, seed >> 32), two Box-Muller pairs
+// per call (uniforms on (0, 1] from the top 24 bits).
+__global__ __launch_bounds__(256) void k_random_normal(float* __restrict__ out, int64_t count, uint64_t seed) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i * 4 >= count) return;
+  const uint4 r = Philox::gen(make_uint4(static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 0x5A11u, 0u),
+                              static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  const float u0 = (static_cast<float>(r.x >> 8) + 1.f) * (1.f / 16777216.f);
+  const float u1 = static_cast<float>(r.y >> 8) * (1.f / 16777216.f);
+  const float u2 = (static_cast<float>(r.z >> 8) + 1.f) * (1.f / 16777216.f);
+  const float u3 = static_cast<float>(r.w >> 8) * (1.f / 16777216.f);
+  const float m0 = sqrtf(-2.f * logf(u0)), m1 = sqrtf(-2.f * logf(u2));
+  const float a0 = 6.2831853071795865f * u1, a1 = 6.2831853071795865f * u3;
+  const float v[4] = {m0 * cosf(a0), m0 * sinf(a0), m1 * cosf(a1), m1 * sinf(a1)};
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (i * 4 + q < count) out[i * 4 + q] = v[q];
+}
+
 }  // namespace umap
 }  // namespace mv
 
@@ -908,6 +928,17 @@ int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, 
     case 4: return spectral_run<4>(w_sym, eps, tail, n, k, iters, V, w, s);
     default: return spectral_run<5>(w_sym, eps, tail, n, k, iters, V, w, s);
   }
+}
+
+int mv_random_normal(float* out, int64_t count, uint64_t seed, void* stream) {
+  MV_REQUIRE(out || count == 0, "mv_random_normal: null pointer");
+  MV_REQUIRE(count >= 0, "mv_random_normal: count < 0");
+  MV_FAULT_POINT("launch");
+  if (count == 0) return MV_OK;
+  hipLaunchKernelGGL(k_random_normal, dim3(static_cast<unsigned>(ceil_div(ceil_div(count, 4), 256))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), out, count, seed);
+  MV_LAUNCHED("k_random_normal");
+  return MV_OK;
 }
 
 int mv_zero_f32(float* p, int64_t count, void* stream) {

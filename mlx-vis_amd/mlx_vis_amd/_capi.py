@@ -79,6 +79,7 @@ _SIGNATURES: dict[str, list] = {
     "mv_mlp_forward": [_p, _i32, _i32, _i32, _i32, _p, _i64, _p, _i32, _p, _sz, _p],
     "mv_mlp_sample_edges": [_p, _p, _i64, _i32, _u32, _i32, _i32, _i32, _p, _p],
     "mv_knn_overlap": [_p, _i64, _i32, _p, _i64, _i32, _i64, _i32, _p, _p, _p],
+    "mv_random_normal": [_p, _i64, _u64, _p],
 }
 _RESTYPES = {"mv_last_error": C.c_char_p}
 

@@ -1,8 +1,11 @@
 """Spectral initialisation: drop-in for UMAP._spectral_init (umap.py:298-357).
 
-The power iteration runs in libmivis (mv_umap_spectral); the final scaling is
-the reference's host NumPy code path: drop the first vector, scale to
-max |v| = 10, add N(0, 1e-4) noise from RandomState(random_state or 42), then
+The start vectors (drawn by the reference from MLX's generator, umap.py:325,
+a stream that cannot be reproduced here) are standard normals from
+mv_random_normal, keyed by random_state; the power iteration runs in libmivis
+(mv_umap_spectral); the final scaling is the reference's host NumPy code path:
+drop the first vector, scale to max |v| = 10, add N(0, 1e-4) noise from
+RandomState(random_state or 42) (drawn while the device iterates), then
 min-max each column to [0, 10] (umap.py:339-348). Any failure (including a
 non-finite result) falls back to N(0, 1) * 0.01 as the reference's
 ``except Exception`` does (umap.py:354-357).
@@ -23,22 +26,25 @@ def spectral_init(g, n: int, n_components: int, random_state=None, verbose: bool
     dev = g.knn_idx.device
     try:
         nc = n_components + 1
-        start = np.random.RandomState(random_state).standard_normal((n, nc)).astype(np.float32)
-        V = torch.from_numpy(start).to(dev)
         lib = _capi.lib()
+        V = torch.empty((n, nc), dtype=torch.float32, device=dev)
+        seed = 0x5EC7 if random_state is None else int(random_state)
+        seed = (seed * 0x9E3779B97F4A7C15 + 0xD1B54A32D192ED03) & 0xFFFFFFFFFFFFFFFF
+        _capi.check(lib.mv_random_normal(V.data_ptr(), n * nc, seed, _device.stream()), "mv_random_normal")
         nbytes = ctypes.c_size_t(0)
         _capi.check(lib.mv_umap_spectral_ws_bytes(n, g.k, nc, ctypes.byref(nbytes)), "mv_umap_spectral_ws_bytes")
         ws = _device.workspace(nbytes.value)
         _capi.check(lib.mv_umap_spectral(g.w_sym.data_ptr(), g.eps.data_ptr(), g.knn_idx.data_ptr(), n, g.k, nc,
                                          int(iters), V.data_ptr(), ws.data_ptr(), ws.numel(), _device.stream()),
                     "mv_umap_spectral")
+        rng = np.random.RandomState(random_state or 42)
+        noise = rng.normal(scale=0.0001, size=(n, n_components)).astype(np.float32)
         emb = V[:, 1:nc].cpu().numpy()
         if not np.isfinite(emb).all():
             raise FloatingPointError("non-finite spectral embedding")
-        rng = np.random.RandomState(random_state or 42)
         expansion = 10.0 / np.max(np.abs(emb))
         emb = (emb * expansion).astype(np.float32)
-        emb += rng.normal(scale=0.0001, size=emb.shape).astype(np.float32)
+        emb += noise
         emb = 10.0 * (emb - emb.min(axis=0)) / (emb.max(axis=0) - emb.min(axis=0) + 1e-10)
         if verbose:
             print("Using spectral initialization")

@@ -150,3 +150,24 @@ def test_fit_transform_end_to_end():
     # isotropic clusters: neighbourhoods inside a cluster are noise, so check
     # trustworthiness rather than 15-NN overlap
     assert om.trustworthiness(X, Y, 15) > 0.85
+
+
+def test_random_normal_draws():
+    """mv_random_normal (spectral start vectors): standard normal moments,
+    reproducible by seed, different across seeds."""
+    import torch
+
+    from mlx_vis_amd import _capi
+
+    lib = _capi.lib()
+    a = torch.empty(1_000_003, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    c = torch.empty_like(a)
+    for t, seed in ((a, 7), (b, 7), (c, 8)):
+        _capi.check(lib.mv_random_normal(t.data_ptr(), t.numel(), seed, None))
+    x = a.cpu().numpy().astype(np.float64)
+    assert np.isfinite(x).all()
+    assert abs(x.mean()) < 5e-3 and abs(x.std() - 1.0) < 5e-3
+    assert abs((x ** 4).mean() - 3.0) < 0.05          # normal kurtosis
+    assert abs((np.abs(x) < 1.0).mean() - 0.6827) < 3e-3
+    assert torch.equal(a, b) and not torch.equal(a, c)
