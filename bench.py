@@ -464,7 +464,7 @@ def main():
             prec[f"{dt}_c{rr * 15}"] = {"ms_total": round(tot, 3), "ms_phases": [round(v, 3) for v in ph_dt],
                         "speedup_vs_fp32": round(knn_total_ms / tot, 3),
                         "identical_indices": bool(torch.equal(idx_dt, ref_idx)),
-                        "rows_uncertified_first_pass": st_dt[2], "rows_uncertified_after_fp32_rescreen": st_dt[3],
+                        "rows_uncertified_first_pass": st_dt[2], "rows_uncertified_after_rescreen": st_dt[3],
                         "rows_fp64_scan": st_dt[0]}
         # the whole fit_transform with the split-BF16 screen (same certified kNN)
         def step_x3():

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_split_bf16(const float* __restrict__ Xc
 }
 
 // Gathers the FP32 centred rows listed in rows[0:*count] (original query ids)
-// for the FP32 re-screen, with their norms and self indices.
+// for the re-screen, with their norms and self indices.
 __global__ __launch_bounds__(256) void k_gather_rows(const float* __restrict__ Qc, const float* __restrict__ sqq,
                                                      int64_t d_pad, const int32_t* __restrict__ rows,
                                                      const int32_t* __restrict__ count,
@@ -1157,6 +1157,7 @@ struct Buffers {
   uint64_t* scan_d;                             // split exhaustive scan partial lists
   uint32_t* scan_i;
   float *Qc2, *sqq2;                            // FP32 re-screen rows
+  char *Xs, *Qs2;                               // split BF16 copies for the re-screen
   int32_t* self2;
 };
 
@@ -1198,6 +1199,8 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
     b.Qc2 = c.take<float>(static_cast<size_t>(L.nq_pad) * L.d_pad);
     b.sqq2 = c.take<float>(L.nq_pad);
     b.self2 = c.take<int32_t>(L.nq_pad);
+    b.Xs = c.take<char>(static_cast<size_t>(L.n_pad) * L.d_pad * 4);
+    b.Qs2 = c.take<char>(static_cast<size_t>(L.nq_pad) * L.d_pad * 4);
   }
   b.cand = c.take<uint64_t>(static_cast<size_t>(MAX_SPLIT) * L.nq_pad * L.C);
   b.ctr = c.take<int32_t>(16);
@@ -1502,13 +1505,21 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   const int32_t* scan_count = fb1_count;
   const int32_t* scan_rows = B.fb1;
   if (!exact_form) {
-    // FP32 re-screen of the rows the quantised certificate could not settle.
+    // Re-screen of the rows the quantised certificate could not settle, with
+    // the split-BF16 screen (FP32-class error bound, no quantisation term).
     hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, B.Qc, B.sqq,
                        L.d_pad, B.fb1, fb1_count, q_self, self_b, B.Qc2, B.sqq2, B.self2);
     MV_LAUNCHED("k_gather_rows");
+    const int64_t cxs = L.n_pad * L.d_pad, cqs = L.nq_pad * L.d_pad;
+    hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cxs, 256))), dim3(256), 0, s, B.Xc, cxs,
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xs));
+    MV_LAUNCHED("k_split_bf16");
+    hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cqs, 256))), dim3(256), 0, s, B.Qc2, cqs,
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Qs2));
+    MV_LAUNCHED("k_split_bf16");
     ScreenArgs sb = sa;
-    sb.Qq = reinterpret_cast<const char*>(B.Qc2);
-    sb.Xq = reinterpret_cast<const char*>(B.Xc);
+    sb.Qq = B.Qs2;
+    sb.Xq = B.Xs;
     sb.sqq = B.sqq2;
     sb.sqx = B.sqx;
     sb.invq = sb.invx = nullptr;
@@ -1516,11 +1527,12 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     sb.q_self = B.self2;
     sb.self_base = -1;
     sb.nq_dev = fb1_count;
-    if (int rc2 = launch_screen<DT_F32>(sb, grid, s)) return rc2;
+    if (int rc2 = launch_screen<DT_BF16X3>(sb, grid, s)) return rc2;
     RerankArgs rb2 = ra;
     rb2.sqq = B.sqq2;
     rb2.sqmax_bits = sqmax32;
-    rb2.err_fac = static_cast<float>((2.0 * static_cast<double>(L.d_pad) + 10.0) * 5.9604644775390625e-8 * 1.01);
+    rb2.err_fac = static_cast<float>((3.0 * static_cast<double>(L.d_pad) + 3.0 * 256.0 + 40.0) *
+                                     5.9604644775390625e-8 * 1.01);
     rb2.eq = nullptr;
     rb2.nq_dev = fb1_count;
     rb2.row_map = B.fb1;
