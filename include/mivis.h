@@ -249,6 +249,47 @@ int mv_mlp_sample_edges(const float* eps, const int32_t* tail, int64_t n, int k,
 int mv_knn_overlap(const int32_t* a, int64_t lda, int ka, const int32_t* b, int64_t ldb, int kb, int64_t n,
                    int min_count, int32_t* out_count, uint8_t* out_flag, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Edge-gradient family beyond UMAP: the contrastive losses of CNE and the
+ * pair force laws of PaCMAP / LocalMAP / t-SNE, on the same embedding layout
+ * (Y: n x dim FP32, dim <= 4). Gradients accumulate into `grad` (zeroed by
+ * the caller) with atomics, as the reference's scatter-adds do.
+ * ------------------------------------------------------------------------ */
+/* CNE gradient over edges (ei[e], ej[e]) with m negatives each (m <= 32) —
+ * replaces CNE._infonce_grad / _nce_grad / _neg_grad (mlx_vis/_cne/cne.py:
+ * 219-313): loss 0 InfoNCE, 1 NCE, 2 NEG; Cauchy similarity 1 / (1 + d^2).
+ * neg (device, E x m, nullable): explicit negatives (then batch must be 0);
+ * else negatives are drawn uniformly from [0, n) by Philox keyed by seed and
+ * counted by (edge, iter, slot). batch > 0: process `batch` edges drawn
+ * uniformly with replacement (cne.py:337-339). Every term is multiplied by
+ * scale (the reference's E / batch). */
+int mv_cne_grad(const float* Y, int64_t n, int dim, const int32_t* ei, const int32_t* ej, int64_t E,
+                const int32_t* neg, int m, int loss, float scale, uint64_t seed, int iter, int64_t batch,
+                float* grad, void* stream);
+/* CNE iterations [it0, it1) (1-based) of CNE._optimize (cne.py:315-372):
+ * gradient (Philox negatives and batches, scale E / batch when batch < E)
+ * and the bias-corrected Adam step (beta 0.9 / 0.999, eps 1e-8) on Y in
+ * place. adam_m, adam_v, grad: n * dim FP32 each; grad must be zero on entry
+ * and is zero on return. */
+int mv_cne_optimize(float* Y, int64_t n, int dim, const int32_t* ei, const int32_t* ej, int64_t E, int m, int loss,
+                    int it0, int it1, float lr, int64_t batch, uint64_t seed, float* adam_m, float* adam_v,
+                    float* grad, void* stream);
+/* One force law over P pairs (src, dst) — the per-pair terms of PaCMAP's step
+ * functions (mlx_vis/_pacmap/pacmap.py:275-354) and of t-SNE's attractive
+ * term (mlx_vis/_tsne/tsne.py:511-517). law: 0 PaCMAP neighbour
+ * 20w / (10 + d)^2, 1 mid-near 20000w / (10000 + d)^2, 2 further pair
+ * -2w / (1 + d)^2, 3 LocalMAP neighbour 20 w scale / ((10 + d)^2 sqrt(d)),
+ * with d = |y_s - y_t|^2 + 1, each times (y_s - y_t) added to src and
+ * subtracted from dst; 4 t-SNE 4w (y_s - y_t) / (1 + |y_s - y_t|^2) added to
+ * src only. w_pair (device, P, nullable) overrides w per pair. */
+int mv_pair_grad(const float* Y, int64_t n, int dim, const int32_t* src, const int32_t* dst, int64_t P,
+                 const float* w_pair, float w, float scale, int law, float* grad, void* stream);
+/* Adam on Y: m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2,
+ * Y -= step m / (sqrt(v) + eps), then g = 0 (the caller folds the bias
+ * corrections into step and eps: PaCMAP's lr_t form, pacmap.py:300-303). */
+int mv_adam_update(float* Y, float* grad, float* adam_m, float* adam_v, int64_t count, float b1, float b2, float step,
+                   float eps, void* stream);
+
 /* out[0:count] = standard normal draws (Philox-4x32-10 keyed by seed, Box-Muller):
  * the spectral start vectors, which the reference draws from MLX's generator
  * (umap.py:325), a stream that cannot be reproduced outside MLX. */
