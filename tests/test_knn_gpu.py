@@ -114,3 +114,34 @@ def test_precision_screens_queries_and_duplicates(dtype):
     idx, dist, _ = _run(Xq, 12, Q, euclid=False, dtype=dtype)
     oi, od = oknn.exact_knn(Q, Xq, 12)
     _check(idx, dist, oi, od, euclid=False)
+
+
+@pytest.mark.parametrize("n,d,k,dtype", [(2, 4, 1, "fp32"), (129, 33, 1, "fp32"), (130, 7, 32, "fp32"),
+                                         (64, 4096, 5, "fp32"), (1500, 40, 20, "bf16x3")])
+def test_edge_shapes(n, d, k, dtype):
+    """Smallest input, one row past a 128-row tile, k = 32 at n = 130, a very wide row
+    (d = 4096), and the split-BF16 screen at k = 20 (C = 40)."""
+    print(f"case n={n} d={d} k={k} dtype={dtype}", flush=True)
+    rng = np.random.default_rng(n * 7 + d)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    idx, dist, _ = _run(X, k, dtype=dtype)
+    oi, od = oknn.exact_knn(X, X, k, self_base=0)
+    _check(idx, dist, oi, od)
+
+
+def test_single_query_and_invalid_arguments():
+    from mlx_vis_amd import knn_brute_device
+
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((3000, 50)).astype(np.float32)
+    Q = X[1234:1235] + 1e-3
+    idx, dist, _ = _run(X, 10, Q)
+    oi, od = oknn.exact_knn(Q, X, 10)
+    _check(idx, dist, oi, od)
+    Xd = torch.from_numpy(X).cuda()
+    with pytest.raises(ValueError):
+        knn_brute_device(Xd, 40, dtype="fp32")   # k > 32
+    with pytest.raises(ValueError):
+        knn_brute_device(Xd, 5, dtype="int8")
+    with pytest.raises(ValueError):
+        knn_brute_device(Xd[:3], 3)               # k exceeds the selectable rows

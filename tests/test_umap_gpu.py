@@ -171,3 +171,35 @@ def test_random_normal_draws():
     assert abs((x ** 4).mean() - 3.0) < 0.05          # normal kurtosis
     assert abs((np.abs(x) < 1.0).mean() - 0.6827) < 3e-3
     assert torch.equal(a, b) and not torch.equal(a, c)
+
+
+@pytest.mark.parametrize("n_components", [1, 3, 4])
+def test_fit_transform_other_dimensions_and_init_array(n_components):
+    from mlx_vis_amd import UMAP
+
+    X, _, _ = _graph_inputs(n=1500, d=16, seed=12)
+    Y = UMAP(n_components=n_components, n_epochs=60, random_state=1, knn_method="brute").fit_transform(X)
+    assert Y.shape == (1500, n_components) and np.isfinite(Y).all()
+    init = np.random.default_rng(3).uniform(-5, 5, (1500, n_components)).astype(np.float32)
+    Y2 = UMAP(n_components=n_components, n_epochs=60, random_state=1, knn_method="brute",
+              init=init).fit_transform(X)
+    assert np.isfinite(Y2).all() and not np.allclose(Y2, init)
+    with pytest.raises(ValueError):
+        UMAP(n_components=n_components, init=init[:10]).fit_transform(X)
+
+
+def test_fit_transform_nndescent_and_precision_screens_agree():
+    from mlx_vis_amd import UMAP
+
+    X, _, _ = _graph_inputs(n=3000, d=32, seed=13)
+    a = UMAP(n_epochs=30, random_state=2, knn_method="brute")
+    a.fit_transform(X)
+    b = UMAP(n_epochs=30, random_state=2, knn_method="brute", knn_dtype="bf16x3")
+    b.fit_transform(X)
+    assert torch.equal(a.knn_[0], b.knn_[0])      # certified: the screen precision does not change the graph
+    assert torch.equal(a.knn_[1], b.knn_[1])
+    assert torch.equal(a.graph_.w_sym, b.graph_.w_sym)
+    assert np.isfinite(b.embedding_).all()
+    c = UMAP(n_epochs=30, random_state=2, knn_method="nndescent")
+    Yc = c.fit_transform(X)
+    assert np.isfinite(Yc).all()
