@@ -107,22 +107,104 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int src) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-// Ascending bitonic sort of one 64-bit key per lane across the wave.
-__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t key) {
-  const int l = lane_id();
-#pragma unroll
-  for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      uint64_t o = shfl_xor_u64(key, j);
-      bool up = (l & k) == 0;
-      bool lower = (l & j) == 0;
-      uint64_t mn = o < key ? o : key;
-      uint64_t mx = o < key ? key : o;
-      key = (lower == up) ? mn : mx;
+// ---- bitonic networks over one 64-bit key per lane ----------------------
+// Partners within a 16-lane row come from DPP moves (quad_perm for distances
+// 1 and 2, row_half_mirror then a quad reversal for 4, row_ror:8 for 8);
+// across rows, v_permlane16_swap / v_permlane32_swap deliver both elements of
+// every pair (the lower lane's in result [0], the upper lane's in [1]) to both
+// lanes. No stage goes through LDS.
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const int lo = static_cast<int>(static_cast<uint32_t>(v));
+  const int hi = static_cast<int>(static_cast<uint32_t>(v >> 32));
+  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return (static_cast<uint64_t>(static_cast<uint32_t>(rhi)) << 32) | static_cast<uint32_t>(rlo);
+}
+
+// Compare-exchange at distance J (a power of two <= 32): the lane keeps the
+// smaller of its key and its partner's (lane ^ J) when take_min, else the larger.
+template <int J>
+__device__ __forceinline__ uint64_t cmpx_u64(uint64_t x, bool take_min) {
+  if constexpr (J >= 16) {
+    const uint32_t lo = static_cast<uint32_t>(x), hi = static_cast<uint32_t>(x >> 32);
+    uint32_t alo, blo, ahi, bhi;
+    if constexpr (J == 16) {
+      const auto r0 = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+      const auto r1 = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+      alo = r0[0]; blo = r0[1]; ahi = r1[0]; bhi = r1[1];
+    } else {
+      const auto r0 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+      const auto r1 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+      alo = r0[0]; blo = r0[1]; ahi = r1[0]; bhi = r1[1];
     }
+    const uint64_t a = (static_cast<uint64_t>(ahi) << 32) | alo;   // the pair's lower-lane key
+    const uint64_t b = (static_cast<uint64_t>(bhi) << 32) | blo;   // the pair's upper-lane key
+    return ((a < b) == take_min) ? a : b;
+  } else {
+    uint64_t p;
+    if constexpr (J == 1) p = dpp_u64<0xB1>(x);                    // quad_perm [1,0,3,2]
+    else if constexpr (J == 2) p = dpp_u64<0x4E>(x);               // quad_perm [2,3,0,1]
+    else if constexpr (J == 4) p = dpp_u64<0x1B>(dpp_u64<0x141>(x));   // half mirror, then [3,2,1,0]
+    else p = dpp_u64<0x128>(x);                                    // row_ror:8
+    return ((p < x) == take_min) ? p : x;
   }
-  return key;
+}
+
+// Stage (K, J) of the bitonic sorting network, ascending (descending if DESC).
+template <int K, int J, bool DESC>
+__device__ __forceinline__ uint64_t bitonic_stage_u64(uint64_t x, int l) {
+  const bool up = ((l & K) == 0) != DESC;
+  return cmpx_u64<J>(x, ((l & J) == 0) == up);
+}
+
+// Bitonic sort of one 64-bit key per lane across the wave (ascending, or
+// descending if DESC).
+template <bool DESC = false>
+__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t x) {
+  const int l = lane_id();
+  x = bitonic_stage_u64<2, 1, DESC>(x, l);
+  x = bitonic_stage_u64<4, 2, DESC>(x, l);
+  x = bitonic_stage_u64<4, 1, DESC>(x, l);
+  x = bitonic_stage_u64<8, 4, DESC>(x, l);
+  x = bitonic_stage_u64<8, 2, DESC>(x, l);
+  x = bitonic_stage_u64<8, 1, DESC>(x, l);
+  x = bitonic_stage_u64<16, 8, DESC>(x, l);
+  x = bitonic_stage_u64<16, 4, DESC>(x, l);
+  x = bitonic_stage_u64<16, 2, DESC>(x, l);
+  x = bitonic_stage_u64<16, 1, DESC>(x, l);
+  x = bitonic_stage_u64<32, 16, DESC>(x, l);
+  x = bitonic_stage_u64<32, 8, DESC>(x, l);
+  x = bitonic_stage_u64<32, 4, DESC>(x, l);
+  x = bitonic_stage_u64<32, 2, DESC>(x, l);
+  x = bitonic_stage_u64<32, 1, DESC>(x, l);
+  x = bitonic_stage_u64<64, 32, DESC>(x, l);
+  x = bitonic_stage_u64<64, 16, DESC>(x, l);
+  x = bitonic_stage_u64<64, 8, DESC>(x, l);
+  x = bitonic_stage_u64<64, 4, DESC>(x, l);
+  x = bitonic_stage_u64<64, 2, DESC>(x, l);
+  x = bitonic_stage_u64<64, 1, DESC>(x, l);
+  return x;
+}
+
+// Ascending sort of a bitonic sequence of one key per lane (half-cleaners).
+__device__ __forceinline__ uint64_t bitonic_clean_u64(uint64_t x) {
+  const int l = lane_id();
+  x = bitonic_stage_u64<64, 32, false>(x, l);
+  x = bitonic_stage_u64<64, 16, false>(x, l);
+  x = bitonic_stage_u64<64, 8, false>(x, l);
+  x = bitonic_stage_u64<64, 4, false>(x, l);
+  x = bitonic_stage_u64<64, 2, false>(x, l);
+  x = bitonic_stage_u64<64, 1, false>(x, l);
+  return x;
+}
+
+// The 64 smallest of the ascending list `lk` and the unsorted keys `a` (one
+// per lane), ascending: `a` is sorted descending, so that the lane-wise
+// minimum of the two is a bitonic sequence holding the 64 smallest.
+__device__ __forceinline__ uint64_t list_merge64_u64(uint64_t lk, uint64_t a) {
+  a = wave_sort_u64<true>(a);
+  return bitonic_clean_u64(lk < a ? lk : a);
 }
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {

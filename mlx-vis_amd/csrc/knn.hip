@@ -285,45 +285,33 @@ struct ScreenArgs {
   int nq, n, C, S;
   int tiles_per_split, n_col_tiles, n_row_blocks, nq_pad;
   uint64_t* cand;         // [S, nq_pad, C] keys
+  uint64_t* pend;         // [S, nq_pad, PEND] survivors not yet merged
 };
 
-// Inserts key kv into the sorted wave-held list (lane l holds entry l, lanes
-// >= C hold KEY_NONE) and refreshes the list's C-th key `tau`.
-__device__ __forceinline__ void list_insert(uint64_t& lk, uint64_t kv, uint64_t& tau, int C, int l) {
-  if (kv >= tau) return;
-  const int pos = __popcll(ballot(lk < kv));
-  const uint64_t up = wave_shr1_u64(lk);
-  if (l > pos) lk = up;
-  else if (l == pos) lk = kv;
+// Survivors of the screen wait in a per-row buffer of PEND keys (global
+// memory next to the candidate lists) and are merged into the row's sorted
+// list PEND at a time, with one wavefront sort, instead of one insertion each.
+constexpr int PEND = 64;
+
+// Appends the keys of a wavefront's two column halves (KEY_NONE = no key;
+// m0, m1 = their ballots) at positions base.. of buf, lane order, half 0 first.
+__device__ __forceinline__ void pend_append(uint64_t* buf, int base, uint64_t k0, uint64_t m0, uint64_t k1,
+                                            uint64_t m1) {
+  const int p0 = base + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(m0 >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m0), 0u)));
+  const int p1 = base + __popcll(m0) +
+                 static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m1 >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m1), 0u)));
+  if (k0 != KEY_NONE) buf[p0] = k0;
+  if (k1 != KEY_NONE) buf[p1] = k1;
+}
+
+// Merges unsorted keys (one per lane, KEY_NONE padded) into the row's list of
+// C keys and returns the list's new C-th key.
+__device__ __forceinline__ uint64_t list_absorb(uint64_t& lk, uint64_t keys, int C, int l) {
+  lk = list_merge64_u64(lk, keys);
   if (l >= C) lk = KEY_NONE;
-  tau = readlane_u64(lk, C - 1);
-}
-
-// Sorts a bitonic sequence of one key per lane ascending (half-cleaners).
-__device__ __forceinline__ uint64_t bitonic_clean_u64(uint64_t v, int l) {
-#pragma unroll
-  for (int j = 32; j >= 1; j >>= 1) {
-    const uint64_t o = shfl_xor_u64(v, j);
-    v = (l & j) ? (o > v ? o : v) : (o < v ? o : v);
-  }
-  return v;
-}
-
-// Survivor count above which a row's candidates are merged into its list by
-// sorting instead of one insertion each.
-constexpr int BULK_MIN = 24;
-
-// The 64 smallest of the sorted list `lk` (one key per lane, ascending,
-// KEY_NONE padded) and the unsorted keys a, b, ascending, one per lane.
-__device__ __forceinline__ uint64_t list_merge128(uint64_t lk, uint64_t a, uint64_t b, int l) {
-  a = wave_sort_u64(a);
-  b = wave_sort_u64(b);
-  const uint64_t br = shfl_u64(b, 63 - l);
-  uint64_t c = a < br ? a : br;   // the 64 smallest of a and b, bitonic
-  c = bitonic_clean_u64(c, l);
-  const uint64_t cr = shfl_u64(c, 63 - l);
-  uint64_t m = lk < cr ? lk : cr;   // the 64 smallest of the list and c, bitonic
-  return bitonic_clean_u64(m, l);
+  return readlane_u64(lk, C - 1);
 }
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -420,6 +408,8 @@ template <int DT>
 __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
   __shared__ float s_tauf[BM];          // C-th list distance (+inf while the list is not full)
+  __shared__ uint64_t s_tauk[BM];       // C-th list key (KEY_NONE while the list is not full)
+  __shared__ int s_cnt[BM];             // keys waiting in the row's buffer
   __shared__ uint64_t s_hit[2][2];      // [column half][row half]: rows with survivors
   __shared__ float s_sqq[BM];
   __shared__ int s_self[BM];
@@ -453,10 +443,13 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
     }
     s_self[r] = static_cast<int>(self);
     s_tauf[r] = INFINITY;
+    s_tauk[r] = KEY_NONE;
+    s_cnt[r] = 0;
   }
   // The candidate lists live in the output buffer (this workgroup's rows of
   // its split, contiguous, L2-resident); only their C-th keys stay in LDS.
   uint64_t* lists = a.cand + (static_cast<int64_t>(split) * a.nq_pad + static_cast<int64_t>(rb) * BM) * C;
+  uint64_t* pend = a.pend + (static_cast<int64_t>(split) * a.nq_pad + static_cast<int64_t>(rb) * BM) * PEND;
   for (int e = tid; e < BM * C; e += NT) lists[e] = KEY_NONE;
   __syncthreads();
 
@@ -583,66 +576,81 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
     __syncthreads();
 
     // Phase 2: wavefront w owns rows 32w .. 32w + 31 and visits only the rows
-    // with survivors, reading the column halves that have them. The lists of
-    // up to 8 such rows are loaded from L2 together, so their latency is paid
-    // once per batch.
+    // with survivors, reading the column halves that have them. Keys below the
+    // row's C-th key are appended to its buffer. Rows whose buffer would
+    // overflow are merged afterwards, in a second pass: keeping the global
+    // loads of the merges out of the append loop keeps its iterations from
+    // waiting on the buffer stores of the previous ones.
     {
       const int hr = w >> 1, sh = (w & 1) * 32;
-      const uint32_t h0 = static_cast<uint32_t>(s_hit[0][hr] >> sh);
-      const uint32_t h1 = static_cast<uint32_t>(s_hit[1][hr] >> sh);
-      uint32_t todo = h0 | h1;
+      const uint32_t h0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s_hit[0][hr] >> sh));
+      const uint32_t h1 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s_hit[1][hr] >> sh));
+      const int myrow = w * 32 + (l & 31);
+      const uint64_t tau_l = s_tauk[myrow];   // lane r (and r + 32) holds row 32w + r's state
+      int cnt_l = s_cnt[myrow];
+      uint32_t todo = h0 | h1, full = 0;
+#define MV_ROW_KEYS(RR, TAU)                                                                              const int row = w * 32 + (RR);                                                                          const float v0 = (h0 >> (RR)) & 1u ? Dt[row * DTS + l] : INFINITY;                                      const float v1 = (h1 >> (RR)) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;                                 uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));                    uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));               k0 = k0 < (TAU) ? k0 : KEY_NONE;                                                                        k1 = k1 < (TAU) ? k1 : KEY_NONE;                                                                        uint64_t m0 = ballot(k0 != KEY_NONE);                                                                   uint64_t m1 = ballot(k1 != KEY_NONE)
       while (todo) {
-        int rrq[8];
-        uint64_t lkq[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          rrq[q] = -1;
-          lkq[q] = KEY_NONE;
-          if (todo) {
-            const int rr = __ffs(static_cast<int>(todo)) - 1;
-            todo &= todo - 1;
-            rrq[q] = rr;
-            if (l < C) lkq[q] = lists[(w * 32 + rr) * C + l];
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int rr = rrq[q];
-          if (rr < 0) break;
-          const int row = w * 32 + rr;
-          uint64_t tau = readlane_u64(lkq[q], C - 1);   // the list's C-th key (KEY_NONE until full)
-          const float v0 = (h0 >> rr) & 1u ? Dt[row * DTS + l] : INFINITY;
-          const float v1 = (h1 >> rr) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;
-          const uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));
-          const uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));
-          uint64_t m0 = ballot(k0 < tau);
-          uint64_t m1 = ballot(k1 < tau);
-          if ((m0 | m1) == 0) continue;
-          uint64_t lk = lkq[q];
-          if (__popcll(m0) + __popcll(m1) > BULK_MIN) {
-            // many survivors (a list still filling): sort both halves, keep the
-            // 64 smallest, and merge them with the list in one pass
-            lk = list_merge128(lk, k0 < tau ? k0 : KEY_NONE, k1 < tau ? k1 : KEY_NONE, l);
-            if (l >= C) lk = KEY_NONE;
-            tau = readlane_u64(lk, C - 1);
-          } else {
-            while (m0) {
-              const int src = __ffsll(static_cast<long long>(m0)) - 1;
-              m0 &= m0 - 1;
-              list_insert(lk, readlane_u64(k0, src), tau, C, l);
-            }
-            while (m1) {
-              const int src = __ffsll(static_cast<long long>(m1)) - 1;
-              m1 &= m1 - 1;
-              list_insert(lk, readlane_u64(k1, src), tau, C, l);
-            }
-          }
-          if (l < C) lists[row * C + l] = lk;
-          if (l == 0 && tau != KEY_NONE) s_tauf[row] = key_dist(tau);
+        const int rr = __builtin_ctz(todo);
+        todo &= todo - 1;
+        const uint64_t tau = readlane_u64(tau_l, rr);
+        MV_ROW_KEYS(rr, tau);
+        if ((m0 | m1) == 0) continue;
+        const int cnt = __builtin_amdgcn_readlane(cnt_l, rr);
+        const int m = __popcll(m0) + __popcll(m1);
+        if (cnt + m <= PEND) {
+          pend_append(pend + row * PEND, cnt, k0, m0, k1, m1);
+          if (l == rr) cnt_l = cnt + m;
+        } else {
+          full |= 1u << rr;
         }
       }
+      while (full) {
+        const int rr = __builtin_ctz(full);
+        full &= full - 1;
+        uint64_t tau = readlane_u64(tau_l, rr);
+        const int cnt0 = __builtin_amdgcn_readlane(cnt_l, rr);
+        MV_ROW_KEYS(rr, tau);
+        uint64_t* buf = pend + row * PEND;
+        uint64_t lk = l < C ? lists[row * C + l] : KEY_NONE;
+        const uint64_t pk = l < cnt0 ? buf[l] : KEY_NONE;
+        if (cnt0 > 0) {
+          tau = list_absorb(lk, pk, C, l);
+          k0 = k0 < tau ? k0 : KEY_NONE;
+          k1 = k1 < tau ? k1 : KEY_NONE;
+          m0 = ballot(k0 != KEY_NONE);
+          m1 = ballot(k1 != KEY_NONE);
+        }
+        int cnt = __popcll(m0) + __popcll(m1);
+        if (cnt <= PEND) {
+          pend_append(buf, 0, k0, m0, k1, m1);
+        } else {
+          tau = list_absorb(lk, k0, C, l);
+          k1 = k1 < tau ? k1 : KEY_NONE;
+          if (ballot(k1 != KEY_NONE)) tau = list_absorb(lk, k1, C, l);
+          cnt = 0;
+        }
+        if (l < C) lists[row * C + l] = lk;
+        if (l == rr) cnt_l = cnt;
+        if (l == 0) {
+          s_tauk[row] = tau;
+          if (tau != KEY_NONE) s_tauf[row] = key_dist(tau);
+        }
+      }
+#undef MV_ROW_KEYS
+      if (l < 32) s_cnt[myrow] = cnt_l;
     }
     __syncthreads();
+  }
+
+  // Buffered keys of the last tiles go into the lists.
+  for (int rr = 0; rr < 32; ++rr) {
+    const int row = w * 32 + rr;
+    const int cnt = s_cnt[row];
+    if (cnt == 0) continue;
+    uint64_t lk = l < C ? lists[row * C + l] : KEY_NONE;
+    list_absorb(lk, l < cnt ? pend[row * PEND + l] : KEY_NONE, C, l);
+    if (l < C) lists[row * C + l] = lk;
   }
 #undef MV_GLOAD
 #undef MV_LSTORE
@@ -1152,6 +1160,7 @@ struct Buffers {
   char *Xq, *Qq;                                // quantised copies (dtype != FP32)
   float *sqxq, *sqqq, *ex, *eq, *invx, *invq;   // quantised norms, error bounds, 1/scale
   uint64_t* cand;
+  uint64_t* pend;   // screen survivor buffers
   int32_t* ctr;   // [0] fb1 count, [1] sqmax32, [2] sqmaxq, [3] emax, [4] fb2 count, [8..11] stats
   int32_t *fb1, *fb2;
   uint64_t* scan_d;                             // split exhaustive scan partial lists
@@ -1203,6 +1212,7 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
     b.Qs2 = c.take<char>(static_cast<size_t>(L.nq_pad) * L.d_pad * 4);
   }
   b.cand = c.take<uint64_t>(static_cast<size_t>(MAX_SPLIT) * L.nq_pad * L.C);
+  b.pend = c.take<uint64_t>(static_cast<size_t>(MAX_SPLIT) * L.nq_pad * PEND);
   b.ctr = c.take<int32_t>(16);
   b.fb1 = c.take<int32_t>(nq > 0 ? nq : 1);
   b.fb2 = c.take<int32_t>(nq > 0 ? nq : 1);
@@ -1438,6 +1448,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   sa.n_row_blocks = static_cast<int>(row_blocks);
   sa.nq_pad = static_cast<int>(L.nq_pad);
   sa.cand = B.cand;
+  sa.pend = B.pend;
   if (dtype == MV_F32 || dtype == MV_BF16X3) {
     sa.Qq = dtype == MV_F32 ? reinterpret_cast<const char*>(B.Qc) : B.Qq;
     sa.Xq = dtype == MV_F32 ? reinterpret_cast<const char*>(B.Xc) : B.Xq;
