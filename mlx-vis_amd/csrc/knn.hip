@@ -407,13 +407,12 @@ __device__ __forceinline__ void mfma_chunk(const float* As, const float* Bs, int
 template <int DT>
 __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
-  __shared__ float s_tauf[BM];          // C-th list distance (+inf while the list is not full)
+  __shared__ float4 s_row[BM];          // |q|^2, FP8 1/scale, C-th list distance (+inf while
+                                        // the list is not full), self column (int bits)
   __shared__ uint64_t s_tauk[BM];       // C-th list key (KEY_NONE while the list is not full)
-  __shared__ int s_cnt[BM];             // keys waiting in the row's buffer
+  __shared__ int s_cnt[BM];             // slots of the row's buffer reserved this tile
+  __shared__ int s_valid[BM];           // keys waiting in the row's buffer
   __shared__ uint64_t s_hit[2][2];      // [column half][row half]: rows with survivors
-  __shared__ float s_sqq[BM];
-  __shared__ int s_self[BM];
-  __shared__ float s_inv[BM];
 
   // Workgroups are dispatched round-robin over the 8 XCDs (bid mod 8); XCD x
   // works on column split x mod S, so the co-resident workgroups of an XCD
@@ -434,17 +433,16 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
 
   for (int r = tid; r < BM; r += NT) {
     const int64_t grow = static_cast<int64_t>(rb) * BM + r;
-    s_sqq[r] = a.sqq[grow];
-    s_inv[r] = DT == DT_FP8 ? a.invq[grow] : 1.f;
     int64_t self = -1;
     if (grow < nq) {
       if (a.q_self) self = a.q_self[grow];
       else if (a.self_base >= 0) self = a.self_base + grow;
     }
-    s_self[r] = static_cast<int>(self);
-    s_tauf[r] = INFINITY;
+    s_row[r] = make_float4(a.sqq[grow], DT == DT_FP8 ? a.invq[grow] : 1.f, INFINITY,
+                           __int_as_float(static_cast<int>(self)));
     s_tauk[r] = KEY_NONE;
     s_cnt[r] = 0;
+    s_valid[r] = 0;
   }
   // The candidate lists live in the output buffer (this workgroup's rows of
   // its split, contiguous, L2-resident); only their C-th keys stay in LDS.
@@ -463,10 +461,9 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   // Register prefetch one chunk deep into a double-buffered LDS stage: while
   // chunk kc is multiplied out of one buffer, chunk kc + 1 is in flight from
   // memory and then stored to the other buffer (the co-resident workgroup of
-  // the CU hides the remaining latency). The last chunk of a tile prefetches
-  // the first chunk of the next tile, which stays in registers through the
-  // epilogue. Chunk indices past the end are clamped (re-read, never used),
-  // keeping every load and store unconditional.
+  // the CU hides the remaining latency). The first chunk of the next tile is
+  // loaded between the two epilogue phases and stays in registers through
+  // phase 2. Chunk indices past the end are clamped (re-read, never used).
   float4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
 #define MV_GLOAD(BPTR, KCC)                                                                \
   do {                                                                                   \
@@ -514,7 +511,7 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
     for (int kc = 0; kc < nk; ++kc) {
       const int cur = kc & 1;
       const bool last = kc + 1 == nk;
-      MV_GLOAD(last ? Bnext : Bb, last ? 0 : kc + 1);
+      if (!last) MV_GLOAD(Bb, kc + 1);
       // keep the loads ahead of the multiplies (the scheduler would otherwise
       // sink them past the MFMAs to reuse operand registers, exposing their latency)
       __builtin_amdgcn_sched_barrier(0);
@@ -524,10 +521,15 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
     }
 
     // Epilogue, phase 1: screen values D = |q|^2 + |x|^2 - 2 q.x (clamped at 0;
-    // excluded columns +inf) stay in registers and are compared with each row's
-    // current C-th distance (a superset test: equal distances pass). A ballot
-    // per (mi, r) gives the rows of this wavefront's quadrant with survivors;
-    // only quadrants with survivors are written to the D tile.
+    // excluded columns +inf) stay in registers. Keys (distance, index) below
+    // the row's C-th key are appended to the row's buffer straight from here:
+    // per (mi, r) the 16 lanes of group g hold 64 columns of one row, lane
+    // (16g) reserves their slots with an LDS atomic, and each lane stores its
+    // keys at the reserved base plus its prefix count. The two wavefronts
+    // sharing a row reserve from the same counter, so all successful
+    // reservations of a tile precede the first one that does not fit. A row
+    // half that does not fit writes its values to the D tile and is merged in
+    // phase 2.
     float* Dt = s_stage;
     float sqxv[4], invc[4];
     int colv[4];
@@ -537,90 +539,97 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
       sqxv[ni] = a.sqx[colv[ni]];
       invc[ni] = DT == DT_FP8 ? a.invx[colv[ni]] : 1.f;
     }
-    uint64_t hits = 0;   // bit mi*16 + g*4 + r: row wr*64 + mi*16 + g*4 + r has a survivor
+    const int g = l >> 4;
+    uint64_t hits = 0;   // bit mi*16 + g*4 + r: row wr*64 + mi*16 + g*4 + r did not fit its buffer
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = wr * 64 + mi * 16 + (l >> 4) * 4 + r;
-        const float sr = s_sqq[row];
-        const int self = s_self[row];
-        const float ir = s_inv[row];
-        const float tf = s_tauf[row];
-        bool pass = false;
+        const int row = wr * 64 + mi * 16 + g * 4 + r;
+        const float4 rv = s_row[row];
+        const uint64_t tk = s_tauk[row];
+        const int self = __float_as_int(rv.w);
+        int c = 0;
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
-          const float dot = DT == DT_FP8 ? acc[mi][ni][r] * ir * invc[ni] : acc[mi][ni][r];
-          float v = fmaxf(sr + sqxv[ni] - 2.f * dot, 0.f);
+          const float dot = DT == DT_FP8 ? acc[mi][ni][r] * rv.y * invc[ni] : acc[mi][ni][r];
+          float v = fmaxf(rv.x + sqxv[ni] - 2.f * dot, 0.f);
           if (colv[ni] >= a.n || colv[ni] == self) v = INFINITY;
-          acc[mi][ni][r] = v;
-          pass |= (v <= tf) && (v < INFINITY);
+          // values failing the filter become +inf: no later, tighter threshold passes them
+          const bool pass = v <= rv.z && v < INFINITY && dkey(v, static_cast<uint32_t>(colv[ni])) < tk;
+          acc[mi][ni][r] = pass ? v : INFINITY;
+          c += pass ? 1 : 0;
         }
-        const uint64_t b = ballot(pass);
+        if (ballot(c > 0) == 0) continue;
+        // inclusive prefix count over the group's 16 lanes (one DPP row); lane
+        // 15 of the row holds the group total and reserves the slots
+        int incl = c;
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xF, 0xF, false);   // row_shr:1
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xF, 0xF, false);   // row_shr:2
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xF, 0xF, false);   // row_shr:4
+        incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xF, 0xF, false);   // row_shr:8
+        int base = -2;   // lane 15: the reserved base; -1: the keys do not fit; -2: no keys
+        if ((l & 15) == 15 && incl > 0) {
+          base = atomicAdd(&s_cnt[row], incl);
+          if (base + incl > PEND) base = -1;
+          else atomicAdd(&s_valid[row], incl);
+        }
+        const int b0 = __builtin_amdgcn_readlane(base, 15), b1 = __builtin_amdgcn_readlane(base, 31);
+        const int b2 = __builtin_amdgcn_readlane(base, 47), b3 = __builtin_amdgcn_readlane(base, 63);
+        const int gb = g == 0 ? b0 : (g == 1 ? b1 : (g == 2 ? b2 : b3));
+        if (gb >= 0) {
+          uint64_t* buf = pend + row * PEND + gb + incl - c;
+          int j = 0;
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
-          if ((b >> (16 * g)) & 0xFFFFull) hits |= 1ull << (mi * 16 + g * 4 + r);
+          for (int ni = 0; ni < 4; ++ni)
+            if (acc[mi][ni][r] < INFINITY) buf[j++] = dkey(acc[mi][ni][r], static_cast<uint32_t>(colv[ni]));
+        }
+        // a group with keys that did not fit: the row half goes to the D tile
+        const bool spill = gb == -1;
+        const uint64_t sb = ballot(spill);
+        if (sb) {
+          if (spill) {
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni) Dt[row * DTS + wc * 64 + ni * 16 + (l & 15)] = acc[mi][ni][r];
+          }
+#pragma unroll
+          for (int gg = 0; gg < 4; ++gg)
+            if ((sb >> (16 * gg)) & 0xFFFFull) hits |= 1ull << (mi * 16 + gg * 4 + r);
+        }
       }
     }
-    if (hits) {
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = wr * 64 + mi * 16 + (l >> 4) * 4 + r;
-#pragma unroll
-          for (int ni = 0; ni < 4; ++ni) Dt[row * DTS + wc * 64 + ni * 16 + (l & 15)] = acc[mi][ni][r];
-        }
-    }
     if (l == 0) s_hit[wc][wr] = hits;
+    // the first chunk of the next tile is in flight through phase 2 (not
+    // earlier: phase 1 needs the registers)
+    MV_GLOAD(Bnext, 0);
     __syncthreads();
 
-    // Phase 2: wavefront w owns rows 32w .. 32w + 31 and visits only the rows
-    // with survivors, reading the column halves that have them. Keys below the
-    // row's C-th key are appended to its buffer. Rows whose buffer would
-    // overflow are merged afterwards, in a second pass: keeping the global
-    // loads of the merges out of the append loop keeps its iterations from
-    // waiting on the buffer stores of the previous ones.
+    // Phase 2: wavefront w owns rows 32w .. 32w + 31 and merges those whose
+    // buffer overflowed: the buffered keys first (which tightens the C-th
+    // key), then the overflowing halves' values from the D tile.
     {
       const int hr = w >> 1, sh = (w & 1) * 32;
       const uint32_t h0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s_hit[0][hr] >> sh));
       const uint32_t h1 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s_hit[1][hr] >> sh));
-      const int myrow = w * 32 + (l & 31);
-      const uint64_t tau_l = s_tauk[myrow];   // lane r (and r + 32) holds row 32w + r's state
-      int cnt_l = s_cnt[myrow];
-      uint32_t todo = h0 | h1, full = 0;
-#define MV_ROW_KEYS(RR, TAU)                                                                              const int row = w * 32 + (RR);                                                                          const float v0 = (h0 >> (RR)) & 1u ? Dt[row * DTS + l] : INFINITY;                                      const float v1 = (h1 >> (RR)) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;                                 uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));                    uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));               k0 = k0 < (TAU) ? k0 : KEY_NONE;                                                                        k1 = k1 < (TAU) ? k1 : KEY_NONE;                                                                        uint64_t m0 = ballot(k0 != KEY_NONE);                                                                   uint64_t m1 = ballot(k1 != KEY_NONE)
-      while (todo) {
-        const int rr = __builtin_ctz(todo);
-        todo &= todo - 1;
-        const uint64_t tau = readlane_u64(tau_l, rr);
-        MV_ROW_KEYS(rr, tau);
-        if ((m0 | m1) == 0) continue;
-        const int cnt = __builtin_amdgcn_readlane(cnt_l, rr);
-        const int m = __popcll(m0) + __popcll(m1);
-        if (cnt + m <= PEND) {
-          pend_append(pend + row * PEND, cnt, k0, m0, k1, m1);
-          if (l == rr) cnt_l = cnt + m;
-        } else {
-          full |= 1u << rr;
-        }
-      }
+      uint32_t full = h0 | h1;
       while (full) {
         const int rr = __builtin_ctz(full);
         full &= full - 1;
-        uint64_t tau = readlane_u64(tau_l, rr);
-        const int cnt0 = __builtin_amdgcn_readlane(cnt_l, rr);
-        MV_ROW_KEYS(rr, tau);
+        const int row = w * 32 + rr;
         uint64_t* buf = pend + row * PEND;
+        const int valid = s_valid[row];
+        uint64_t tau = s_tauk[row];
         uint64_t lk = l < C ? lists[row * C + l] : KEY_NONE;
-        const uint64_t pk = l < cnt0 ? buf[l] : KEY_NONE;
-        if (cnt0 > 0) {
-          tau = list_absorb(lk, pk, C, l);
-          k0 = k0 < tau ? k0 : KEY_NONE;
-          k1 = k1 < tau ? k1 : KEY_NONE;
-          m0 = ballot(k0 != KEY_NONE);
-          m1 = ballot(k1 != KEY_NONE);
-        }
+        const uint64_t pk = l < valid ? buf[l] : KEY_NONE;
+        const float v0 = (h0 >> rr) & 1u ? Dt[row * DTS + l] : INFINITY;
+        const float v1 = (h1 >> rr) & 1u ? Dt[row * DTS + 64 + l] : INFINITY;
+        if (valid > 0) tau = list_absorb(lk, pk, C, l);
+        uint64_t k0 = v0 == INFINITY ? KEY_NONE : dkey(v0, static_cast<uint32_t>(col0 + l));
+        uint64_t k1 = v1 == INFINITY ? KEY_NONE : dkey(v1, static_cast<uint32_t>(col0 + 64 + l));
+        k0 = k0 < tau ? k0 : KEY_NONE;
+        k1 = k1 < tau ? k1 : KEY_NONE;
+        const uint64_t m0 = ballot(k0 != KEY_NONE);
+        const uint64_t m1 = ballot(k1 != KEY_NONE);
         int cnt = __popcll(m0) + __popcll(m1);
         if (cnt <= PEND) {
           pend_append(buf, 0, k0, m0, k1, m1);
@@ -631,14 +640,13 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
           cnt = 0;
         }
         if (l < C) lists[row * C + l] = lk;
-        if (l == rr) cnt_l = cnt;
         if (l == 0) {
           s_tauk[row] = tau;
-          if (tau != KEY_NONE) s_tauf[row] = key_dist(tau);
+          if (tau != KEY_NONE) s_row[row].z = key_dist(tau);
+          s_cnt[row] = cnt;
+          s_valid[row] = cnt;
         }
       }
-#undef MV_ROW_KEYS
-      if (l < 32) s_cnt[myrow] = cnt_l;
     }
     __syncthreads();
   }
@@ -646,7 +654,7 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   // Buffered keys of the last tiles go into the lists.
   for (int rr = 0; rr < 32; ++rr) {
     const int row = w * 32 + rr;
-    const int cnt = s_cnt[row];
+    const int cnt = s_valid[row];
     if (cnt == 0) continue;
     uint64_t lk = l < C ? lists[row * C + l] : KEY_NONE;
     list_absorb(lk, l < cnt ? pend[row * PEND + l] : KEY_NONE, C, l);
