@@ -75,7 +75,10 @@ int mv_device_info(int* cu_count, int* lds_per_block, int* wave_size);
  * out_stats (device, nullable, 4 x int32): [0] rows recomputed by the FP64
  * scan, [1] rows whose order needed the FP64 tie check, [2] rows that failed
  * the first certificate, [3] reserved.
- * Limits: 1 <= k <= 32, k <= rerank*k <= 64, k <= number of selectable rows.
+ * Limits: 1 <= k <= 64, k <= number of selectable rows. The candidate count
+ * is min(rerank*k, 64) (at least k); above k = 32 it is 64, and a row whose
+ * k-th neighbour is not separated from the 64th candidate by the screen's
+ * error bound goes to the FP64 scan.
  */
 int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rerank,
                     int same_qx, size_t* out_bytes);

@@ -869,7 +869,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
 // ------------------------------------------------------------- exact scan --
 
 constexpr int SCAN_CHUNKS = 64;    // column chunks per failing row in the split scan
-constexpr int CMAX_K = 32;         // largest k
+constexpr int CMAX_K = 64;         // largest k
 constexpr int SCAN_ROWS = 256;     // failing rows handled by the split scan (the rest: one block per row)
 
 // Split exhaustive scan, part 1: block (chunk c, slot y) handles failing rows
@@ -986,7 +986,7 @@ __global__ __launch_bounds__(256) void k_scan_merge(ScanMergeArgs a) {
   const int count = min(*a.fb_count, SCAN_ROWS);
   if (f >= count) return;
   const int k = a.k;
-  const int M = SCAN_CHUNKS * k;   // <= 64 * 32 entries, at most 32 per lane
+  const int M = SCAN_CHUNKS * k;   // <= 64 * 64 entries
   uint64_t outd = KEY_NONE;
   uint32_t outi = 0xFFFFFFFFu;
   int taken = 0;   // per lane: entries of its column consumed (lists are sorted per chunk)
@@ -1331,7 +1331,7 @@ int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rera
   MV_REQUIRE(out_bytes, "mv_knn_ws_bytes: out_bytes is NULL");
   MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "mv_knn_ws_bytes: bad shape nq=%lld n=%lld d=%lld",
              (long long)nq, (long long)n, (long long)d);
-  MV_REQUIRE(k >= 1 && k <= 32, "mv_knn_ws_bytes: k=%d outside [1, 32]", k);
+  MV_REQUIRE(k >= 1 && k <= CMAX_K, "mv_knn_ws_bytes: k=%d outside [1, %d]", k, CMAX_K);
   MV_REQUIRE(rerank >= 1, "mv_knn_ws_bytes: rerank=%d < 1", rerank);
   if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
     mv::set_error("mv_knn_ws_bytes: dtype %d not supported", dtype);
@@ -1361,7 +1361,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "mv_knn_brute: bad shape nq=%lld n=%lld d=%lld", (long long)nq,
              (long long)n, (long long)d);
   MV_REQUIRE(n < (1ll << 31) - 256 && nq < (1ll << 31) - 256, "mv_knn_brute: n or nq too large");
-  MV_REQUIRE(k >= 1 && k <= 32, "mv_knn_brute: k=%d outside [1, 32]", k);
+  MV_REQUIRE(k >= 1 && k <= CMAX_K, "mv_knn_brute: k=%d outside [1, %d]", k, CMAX_K);
   MV_REQUIRE(rerank >= 1, "mv_knn_brute: rerank=%d < 1", rerank);
   const bool excl = q_self != nullptr || self_base >= 0;
   MV_REQUIRE(k <= n - (excl ? 1 : 0), "mv_knn_brute: k=%d exceeds the %lld selectable rows", k,

@@ -2,9 +2,9 @@
 MI355X (gfx950) through hand-written HIP kernels in libmivis.so.
 
 Drop-in names follow mlx_vis/__init__.py:5-17 for this path:
-``UMAP``, ``NNDescent`` and the shared ``compute_knn`` operator, plus ``CNE``
-(the contrastive layouts on the same edge-gradient kernels) and the learned
-form ``ParametricUMAP``.
+``UMAP``, ``NNDescent`` and the shared ``compute_knn`` operator, plus ``CNE``,
+``PaCMAP`` and ``LocalMAP`` (layouts on the same edge-gradient kernels) and
+the learned form ``ParametricUMAP``.
 """
 
 __version__ = "0.1.0"
@@ -15,6 +15,7 @@ from ._nndescent import NNDescent
 from ._umap import UMAP, find_ab_params
 from ._parametric import ParametricUMAP, knn_preservation_device
 from ._cne import CNE
+from ._pacmap import LocalMAP, PaCMAP
 
-__all__ = ["UMAP", "NNDescent", "ParametricUMAP", "CNE", "compute_knn", "knn_brute_device", "knn_preservation_device",
+__all__ = ["UMAP", "NNDescent", "ParametricUMAP", "CNE", "PaCMAP", "LocalMAP", "compute_knn", "knn_brute_device", "knn_preservation_device",
            "normalize_input", "find_ab_params"]

@@ -86,7 +86,9 @@ def compute_knn(X, k, method="auto", verbose=False, random_state=None, return_eu
     Args:
         X: (n, d) array (numpy, torch, or anything ``np.asarray`` accepts).
         k: number of neighbours.
-        method: "auto" (brute for n <= 20000, else NNDescent), "brute", "nndescent".
+        method: "auto" (brute for n <= 20000, else NNDescent; brute as well when
+            k is too large for the device NNDescent's candidate slots),
+            "brute", "nndescent".
         random_state: NNDescent seed (default 42).
         return_euclidean: Euclidean distances if True, squared if False.
         dtype: brute-force screen precision ("fp32", "bf16", "fp8", "bf16x3").
@@ -100,7 +102,9 @@ def compute_knn(X, k, method="auto", verbose=False, random_state=None, return_eu
         X = np.asarray(X)
         n = X.shape[0]
     if method == "auto":
-        method = "brute" if n <= 20000 else "nndescent"
+        from ._nndescent import nndescent_fits
+
+        method = "brute" if n <= 20000 or not nndescent_fits(min(k, n - 1)) else "nndescent"
     if method == "nndescent":
         from ._nndescent import NNDescent
 

@@ -22,6 +22,18 @@ import torch
 from . import _capi, _device
 
 
+# Candidate slots per row of the device update (one wavefront, four keys per
+# lane): the current k, mc * k neighbours of neighbours and k reverse ones.
+NND_SLOTS = 256
+
+
+def nndescent_fits(k: int, max_candidates: int | None = None) -> bool:
+    """Whether the device NNDescent holds a row's candidates at this k (the
+    reference's nn-of-nn expansion, nndescent.py:100-101, 221-280)."""
+    mc = min(max_candidates or k, k)
+    return k + mc * k + k <= NND_SLOTS
+
+
 class NNDescent:
     """Approximate k-NN graph via NNDescent on the GPU."""
 

@@ -213,7 +213,9 @@ class UMAP:
         n = Xd.shape[0]
         method = self.knn_method
         if method == "auto":
-            method = "brute" if n <= 20000 else "nndescent"
+            from ._nndescent import nndescent_fits
+
+            method = "brute" if n <= 20000 or not nndescent_fits(min(self.n_neighbors, n - 1)) else "nndescent"
         if method == "brute":
             from ._knn import knn_brute_device
 

@@ -117,10 +117,12 @@ def test_precision_screens_queries_and_duplicates(dtype):
 
 
 @pytest.mark.parametrize("n,d,k,dtype", [(2, 4, 1, "fp32"), (129, 33, 1, "fp32"), (130, 7, 32, "fp32"),
-                                         (64, 4096, 5, "fp32"), (1500, 40, 20, "bf16x3")])
+                                         (64, 4096, 5, "fp32"), (1500, 40, 20, "bf16x3"),
+                                         (3000, 20, 40, "fp32"), (1000, 16, 64, "fp32"), (2000, 33, 60, "bf16x3")])
 def test_edge_shapes(n, d, k, dtype):
     """Smallest input, one row past a 128-row tile, k = 32 at n = 130, a very wide row
-    (d = 4096), and the split-BF16 screen at k = 20 (C = 40)."""
+    (d = 4096), the split-BF16 screen at k = 20 (C = 40), and k above 32 (64
+    candidates; at k = 64 every row goes through the FP64 scan)."""
     print(f"case n={n} d={d} k={k} dtype={dtype}", flush=True)
     rng = np.random.default_rng(n * 7 + d)
     X = rng.standard_normal((n, d)).astype(np.float32)
@@ -140,7 +142,7 @@ def test_single_query_and_invalid_arguments():
     _check(idx, dist, oi, od)
     Xd = torch.from_numpy(X).cuda()
     with pytest.raises(ValueError):
-        knn_brute_device(Xd, 40, dtype="fp32")   # k > 32
+        knn_brute_device(Xd, 65, dtype="fp32")   # k > 64
     with pytest.raises(ValueError):
         knn_brute_device(Xd, 5, dtype="int8")
     with pytest.raises(ValueError):
