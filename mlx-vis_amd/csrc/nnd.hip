@@ -198,6 +198,8 @@ __device__ __forceinline__ float pair_dist(const Src& s, float sqi, int64_t j, f
 
 // --------------------------------------------------------------- update --
 
+constexpr int NND_NQ_MAX = 16;   // candidate slots per lane of the largest update variant
+
 struct UpdateArgs {
   Src src;
   const int32_t* idx;       // current graph (n, k)
@@ -214,19 +216,8 @@ struct UpdateArgs {
   int recompute_cur;        // distances of the current k recomputed with src
 };
 
-// Selects register q of a 4-entry per-lane array without dynamic indexing.
-template <class T>
-__device__ __forceinline__ T pick4(const T (&v)[4], int q) {
-  return q == 0 ? v[0] : q == 1 ? v[1] : q == 2 ? v[2] : v[3];
-}
-template <class T>
-__device__ __forceinline__ void put4(T (&v)[4], int q, T x) {
-  if (q == 0) v[0] = x;
-  else if (q == 1) v[1] = x;
-  else if (q == 2) v[2] = x;
-  else v[3] = x;
-}
-
+// NQ candidate slots per lane: 64 NQ per row (slot s = 64 q + l).
+template <int NQ>
 __global__ __launch_bounds__(256) void k_nnd_update(UpdateArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -267,12 +258,12 @@ __global__ __launch_bounds__(256) void k_nnd_update(UpdateArgs a) {
 
   const int n_nn = a.mc_new * a.j_nn;
   const int n_rev = a.rev ? k : 0;
-  const int total = k + n_nn + n_rev;   // <= 256 (checked on the host)
+  const int total = k + n_nn + n_rev;   // <= 64 NQ (checked on the host)
 
   // Slot ids (slot s = 64 q + l). The source-row shuffle runs on every lane.
-  int32_t sid[4];
+  int32_t sid[NQ];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int s = 64 * q + l;
     const int rel = s - k;
     const int m = (rel >= 0 && rel < n_nn) ? rel / a.j_nn : 0;
@@ -290,28 +281,34 @@ __global__ __launch_bounds__(256) void k_nnd_update(UpdateArgs a) {
   __builtin_amdgcn_wave_barrier();
   const float sqi = a.src.sq[i];
 
-  // Distances: four slots per round, one per 16-lane group.
-  float sd[4] = {0.f, 0.f, 0.f, 0.f};
+  // Distances: four slots per round, one per 16-lane group; register q holds
+  // the slots of rounds 16 q .. 16 q + 15 (every register index is static:
+  // a dynamically indexed register array would live in scratch memory).
+  float sd[NQ];
   const int g = l >> 4, sub = l & 15;
   const int rounds = (total + 3) / 4;
-  for (int r = 0; r < rounds; ++r) {
-    const int s = 4 * r + g;
-    const int q = r >> 4;   // register holding slots of this round
-    const int32_t id = __shfl(pick4(sid, q), s & 63);
-    const bool stored = s < k && !a.recompute_cur;
-    const bool live = s < total && id >= 0 && id != i && !stored;
-    const float dot = group_dot(a.src, id, lds_f, lds_h, sub, live);
-    float v = 0.f;
-    if (live) v = pair_dist(a.src, sqi, id, dot);
-    const float got = __shfl(v, (l & 3) * 16);
-    if ((l >> 2) == (r & 15)) put4(sd, q, got);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    sd[q] = 0.f;
+    for (int rr = 0; rr < 16 && 16 * q + rr < rounds; ++rr) {
+      const int r = 16 * q + rr;
+      const int s = 4 * r + g;
+      const int32_t id = __shfl(sid[q], s & 63);
+      const bool stored = s < k && !a.recompute_cur;
+      const bool live = s < total && id >= 0 && id != i && !stored;
+      const float dot = group_dot(a.src, id, lds_f, lds_h, sub, live);
+      float v = 0.f;
+      if (live) v = pair_dist(a.src, sqi, id, dot);
+      const float got = __shfl(v, (l & 3) * 16);
+      if ((l >> 2) == rr) sd[q] = got;
+    }
   }
 
   // Keys: (distance, id); self and padding get 1e30 / none; stored current
   // distances are used as is.
-  uint64_t key[4];
+  uint64_t key[NQ];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int s = 64 * q + l;
     float dv = sd[q];
     if (s < k && !a.recompute_cur) dv = cur_d;
@@ -323,28 +320,34 @@ __global__ __launch_bounds__(256) void k_nnd_update(UpdateArgs a) {
   // earliest slot, later copies become 1e30 (nndescent.py:441-449).
   uint64_t outk = KEY_NONE;
   int produced = 0;
-  for (int guard = 0; guard < 4 * 64 && produced < k; ++guard) {
+  for (int guard = 0; guard < NQ * 64 && produced < k; ++guard) {
     uint64_t lm = key[0];
 #pragma unroll
-    for (int q = 1; q < 4; ++q) lm = key[q] < lm ? key[q] : lm;
+    for (int q = 1; q < NQ; ++q) lm = key[q] < lm ? key[q] : lm;
     const uint64_t m = wave_min_u64(lm);
     if (m == KEY_NONE) break;
     const int32_t mid = static_cast<int32_t>(key_idx(m));
     int first_slot = 1 << 20;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < NQ; ++q)
       if (key[q] != KEY_NONE && sid[q] == mid) first_slot = min(first_slot, 64 * q + l);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) first_slot = min(first_slot, __shfl_xor(first_slot, off));
-    const int fq = first_slot >> 6, fl = first_slot & 63;
-    const uint64_t fk = readlane_u64(pick4(key, fq), fl);
+    const int fl = first_slot & 63;
+    uint64_t fkv = KEY_NONE;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (64 * q + l == first_slot) fkv = key[q];
+    const uint64_t fk = readlane_u64(fkv, fl);
     // later copies of this id -> 1e30
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < NQ; ++q)
       if (key[q] != KEY_NONE && sid[q] == mid && (64 * q + l) != first_slot)
         key[q] = dkey(BIG, static_cast<uint32_t>(mid));
     if (fk == m) {
-      if (l == fl) put4(key, fq, KEY_NONE);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (64 * q + l == first_slot) key[q] = KEY_NONE;
       if (l == produced) outk = m;
       ++produced;
     }
@@ -626,7 +629,15 @@ static int launch_update(const Plan& P, const Src& src, const int32_t* idx, cons
   a.converging = converging;
   a.recompute_cur = recompute_cur;
   const size_t lds = 4 * static_cast<size_t>(src.xh ? src.ld * 2 : src.ld * 4);
-  hipLaunchKernelGGL(k_nnd_update, dim3(static_cast<unsigned>(ceil_div(P.n, 4))), dim3(256), lds, s, a);
+  const int total = a.k + mc_new * j_nn + (rev ? a.k : 0);
+  const dim3 grid(static_cast<unsigned>(ceil_div(P.n, 4)));
+  if (total <= 64 * 4) hipLaunchKernelGGL(k_nnd_update<4>, grid, dim3(256), lds, s, a);
+  else if (total <= 64 * 8) hipLaunchKernelGGL(k_nnd_update<8>, grid, dim3(256), lds, s, a);
+  else if (total <= 64 * NND_NQ_MAX) hipLaunchKernelGGL(k_nnd_update<NND_NQ_MAX>, grid, dim3(256), lds, s, a);
+  else {
+    mv::set_error("k_nnd_update: %d candidate slots exceed %d", total, 64 * NND_NQ_MAX);
+    return MV_EINVAL;
+  }
   MV_LAUNCHED("k_nnd_update");
   return MV_OK;
 }
@@ -729,8 +740,8 @@ int mv_nnd_build(const float* X, int64_t n, int64_t d, int k, int n_iters, int m
   }
   {
     const int mcc = std::min(mc, k);
-    MV_REQUIRE(k + mcc * k + k <= 256, "mv_nnd_build: k=%d (max_candidates %d) needs more than 256 candidate slots",
-               k, mc);
+    MV_REQUIRE(k + mcc * k + k <= 64 * NND_NQ_MAX,
+               "mv_nnd_build: k=%d (max_candidates %d) needs more than %d candidate slots", k, mc, 64 * NND_NQ_MAX);
   }
   Plan P = make_plan(n, d, k, p1, p2);
   if (!ws || ws_bytes < P.bytes) {

@@ -22,9 +22,9 @@ import torch
 from . import _capi, _device
 
 
-# Candidate slots per row of the device update (one wavefront, four keys per
-# lane): the current k, mc * k neighbours of neighbours and k reverse ones.
-NND_SLOTS = 256
+# Candidate slots per row of the device update (one wavefront, sixteen keys per
+# lane at most): the current k, mc * k neighbours of neighbours and k reverse ones.
+NND_SLOTS = 1024
 
 
 def nndescent_fits(k: int, max_candidates: int | None = None) -> bool:

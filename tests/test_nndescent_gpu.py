@@ -67,25 +67,37 @@ def test_gather_dists(half):
         np.testing.assert_allclose(got, want, rtol=0, atol=2e-6 * scale.max())
 
 
-@pytest.mark.parametrize("n,d,seed", [(4000, 64, 3), (3000, 784, 4), (6000, 128, 5)])
-def test_build_recall_not_below_reference(n, d, seed):
+@pytest.mark.parametrize("n,d,seed,k", [(4000, 64, 3, 15), (3000, 784, 4, 15), (6000, 128, 5, 15),
+                                        (3000, 64, 7, 20), (2500, 48, 8, 30)])
+def test_build_recall_not_below_reference(n, d, seed, k):
+    """k = 15 uses 256 candidate slots per row, k = 20 and 30 (max_candidates
+    = k: k + k^2 + k slots) the 512- and 1024-slot variants."""
     from mlx_vis_amd import NNDescent
 
-    print(f"case n={n} d={d}", flush=True)
+    print(f"case n={n} d={d} k={k}", flush=True)
     X = _data(n, d, seed)
-    nn = NNDescent(k=15, n_iters=12, random_state=0)
+    nn = NNDescent(k=k, n_iters=12, random_state=0)
     gi, gd = nn.build(X)
-    oi, od, trace = onn.nndescent(X, k=15, n_iters=12, random_state=0, return_trace=True)
-    exact, _ = oknn.exact_knn(X, X, 15, self_base=0)
+    oi, od, trace = onn.nndescent(X, k=k, n_iters=12, random_state=0, return_trace=True)
+    exact, _ = oknn.exact_knn(X, X, k, self_base=0)
     rg, ro = om.recall(gi, exact), om.recall(oi, exact)
     print(f"recall gpu={rg:.4f} oracle={ro:.4f} iters gpu={nn.iterations_} {nn.changed_} oracle={trace}")
     assert (gi != np.arange(n)[:, None]).all()
-    assert all(len(set(r)) == 15 for r in gi)
+    assert all(len(set(r)) == k for r in gi)
     # Distances are not re-sorted after the iteration-4 FP32 recompute
     # (nndescent.py:376-378), so only rows that match the oracle are compared.
     same = (gi == oi).all(axis=1)
     assert same.mean() > 0.9
-    np.testing.assert_allclose(gd[same], od[same], rtol=1e-3, atol=1e-3)
+    close = np.abs(gd[same] - od[same]) <= 1e-3 * (1.0 + od[same])
+    assert close.mean() > 0.999
+    # An entry may keep a distance from an FP16 iteration when the run stops
+    # before the FP32 recompute (the reference keeps stored distances too):
+    # every returned distance is the pair's distance within FP16 rounding of
+    # the dot product (inputs and output rounded to 11 bits).
+    sq = np.einsum("ij,ij->i", X.astype(np.float64), X.astype(np.float64))
+    ex2 = ((X[gi].astype(np.float64) - X[:, None, :]) ** 2).sum(-1)
+    bound = 2.0 ** -9 * (sq[:, None] + sq[gi])
+    assert (np.abs(gd.astype(np.float64) ** 2 - ex2) <= bound).all()
     assert rg >= ro - 0.002
 
 
@@ -97,3 +109,11 @@ def test_knn_method_nndescent_via_compute_knn():
     i2, d2 = compute_knn(X, 10, method="nndescent", random_state=1, return_euclidean=False)
     np.testing.assert_array_equal(i1, i2)
     np.testing.assert_allclose(d1 ** 2, d2, rtol=1e-6)
+
+
+def test_build_rejects_k_beyond_the_candidate_slots():
+    from mlx_vis_amd import NNDescent
+
+    X = _data(500, 8, 9)
+    with pytest.raises(ValueError):
+        NNDescent(k=32, random_state=0).build(X)   # 32 + 32 * 32 + 32 > 1024 slots
