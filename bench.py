@@ -142,7 +142,7 @@ def learned_form(Xd, um, fresh_n, reps=5):
     Xf = mixture_fresh_784(fresh_n, seed=3)
     Xfd = torch.from_numpy(Xf).cuda()
     del Xf
-    chunk = 131072
+    chunk = 1 << 20   # one launch pair per 1,048,576 points (H1 workspace 1 GiB)
     ws = pu.forward_workspace(min(chunk, fresh_n))
     Z = pu.transform_device(Xfd, chunk=chunk, ws=ws)
     torch.cuda.synchronize()

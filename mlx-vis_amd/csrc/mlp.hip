@@ -21,6 +21,8 @@
 //   k_transpose_bf16, k_gather_x, k_to_bf16, k_head_fwd, k_loss_grad,
 //   k_head_bwd, k_adam, k_sample_edges.
 #include <hip/hip_bf16.h>
+
+#include <algorithm>
 #include <math.h>
 
 #include "common.hpp"
@@ -35,17 +37,35 @@ typedef uint16_t bf16_t;   // storage type: IEEE bfloat16 bit pattern
 constexpr int BM = 128, BN = 128, LDSA = 36;   // LDS row stride in floats (128 B data + 16 B pad)
 constexpr int NT = 256;
 
-__device__ __forceinline__ bf16_t f2bf(float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7F800000u) == 0x7F800000u) return static_cast<bf16_t>(u >> 16);   // inf / nan
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return static_cast<bf16_t>(u >> 16);
-}
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ bf16_t f2bf(float v) { return __builtin_bit_cast(bf16_t, static_cast<__bf16>(v)); }
 __device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+// GELU with the standard normal CDF Phi(x) = 1 - erfc(x / sqrt2) / 2, where
+// erfc(z) (z >= 0) follows Abramowitz & Stegun 7.1.26:
+//   erfc(z) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2), t = 1 / (1 + p z),
+// absolute error at most 1.5e-7 (far below the BF16 resolution the network
+// stores its activations in). Negative x use Phi(x) = erfc(|x| / sqrt2) / 2,
+// which has no cancellation.
+struct NormCdf {
+  float cdf;   // Phi(x)
+  float e;     // exp(-x^2 / 2)
+};
+__device__ __forceinline__ NormCdf norm_cdf(float x) {
+  const float z = fabsf(x) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.f));
+  const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
+                              0.254829592f);
+  const float e = __expf(-z * z);
+  const float half_erfc = 0.5f * poly * e;
+  return NormCdf{x >= 0.f ? 1.f - half_erfc : half_erfc, e};
+}
+__device__ __forceinline__ float gelu(float x) { return x * norm_cdf(x).cdf; }
 __device__ __forceinline__ float gelu_grad(float x) {
-  return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+  const NormCdf n = norm_cdf(x);
+  return n.cdf + x * 0.39894228040143268f * n.e;
 }
 
 enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_GRAD = 2, EPI_BF16 = 3, EPI_F32_ADD = 4, EPI_HEAD = 5 };
@@ -54,7 +74,8 @@ struct GemmArgs {
   const void* A;     // M x K, row stride lda (elements): BF16, or FP32 when the kernel's AF32 is set
   const bf16_t* B;   // N x K, row stride ldb
   int M, N, K;       // K multiple of 64
-  int kchunks;       // K chunks (of 64) per split along grid z
+  int kchunks;       // K chunks (of 64) per split
+  int m_tiles, n_tiles, splits;   // tile grid (set by the launcher)
   int kvalid;        // AF32: A elements per row that exist (a multiple of 8); later ones read as 0
   int64_t lda, ldb, ldc, ldx;
   void* C;           // F32 or BF16 output, row stride ldc
@@ -66,13 +87,24 @@ struct GemmArgs {
   int dout;                // EPI_HEAD: outputs (<= 4); C is Z (M x dout FP32), pre-filled with b3
 };
 
+// Sum over the 16 lanes of a DPP row (row_ror 8, 4, 2, 1); every lane of the
+// row receives the total.
+__device__ __forceinline__ float row16_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xF, 0xF, false));
+  return v;
+}
+
 // Eight FP32 values rounded to BF16 and packed into 16 bytes.
 __device__ __forceinline__ float4 pack_bf16x8(float4 a, float4 b) {
-  const uint32_t p0 = static_cast<uint32_t>(f2bf(a.x)) | (static_cast<uint32_t>(f2bf(a.y)) << 16);
-  const uint32_t p1 = static_cast<uint32_t>(f2bf(a.z)) | (static_cast<uint32_t>(f2bf(a.w)) << 16);
-  const uint32_t p2 = static_cast<uint32_t>(f2bf(b.x)) | (static_cast<uint32_t>(f2bf(b.y)) << 16);
-  const uint32_t p3 = static_cast<uint32_t>(f2bf(b.z)) | (static_cast<uint32_t>(f2bf(b.w)) << 16);
-  return make_float4(__uint_as_float(p0), __uint_as_float(p1), __uint_as_float(p2), __uint_as_float(p3));
+  const v2bf p0 = {static_cast<__bf16>(a.x), static_cast<__bf16>(a.y)};
+  const v2bf p1 = {static_cast<__bf16>(a.z), static_cast<__bf16>(a.w)};
+  const v2bf p2 = {static_cast<__bf16>(b.x), static_cast<__bf16>(b.y)};
+  const v2bf p3 = {static_cast<__bf16>(b.z), static_cast<__bf16>(b.w)};
+  return make_float4(__builtin_bit_cast(float, p0), __builtin_bit_cast(float, p1), __builtin_bit_cast(float, p2),
+                     __builtin_bit_cast(float, p3));
 }
 
 // C = A B^T on v_mfma_f32_16x16x32_bf16: 128 x 128 output tiles, four
@@ -86,17 +118,27 @@ __device__ __forceinline__ float4 pack_bf16x8(float4 a, float4 b) {
 //   EPI_GELU_GRAD        C = bf16(acc * gelu'(pre_in)), colsum += C;
 //   EPI_HEAD             h = bf16(gelu(bf16(acc + bias))), Z[row, o] += sum_col h W3[o, col].
 template <int EPI, bool AF32>
-__global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
+__global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) float s_stage[2 * 2 * BM * LDSA];
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63;
   const int wr = w >> 1, wc = w & 1;
-  const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
   const int lr = tid >> 3, lq = tid & 7;   // staging row, 16-byte slot in the 128-byte BF16 chunk row
   const int nk_all = g.K / 64;
-  const int kc0 = blockIdx.z * g.kchunks;
+  // Persistent grid (a multiple of 8 workgroups): workgroup b takes tiles b,
+  // b + G, ... Tile t runs on XCD t mod 8 and covers a row block m with
+  // m mod 8 = t mod 8, so that the column tiles and K splits of a row block
+  // share one XCD's L2.
+  const int ncols = g.n_tiles * g.splits;
+  const int slots = 8 * ((g.m_tiles + 7) / 8) * ncols;
+  for (int t = blockIdx.x; t < slots; t += gridDim.x) {
+  const int mt = (t & 7) + 8 * ((t >> 3) / ncols);
+  if (mt >= g.m_tiles) continue;
+  const int cc = (t >> 3) % ncols;
+  const int bm = mt * BM, bn = (cc % g.n_tiles) * BN;
+  const int kc0 = (cc / g.n_tiles) * g.kchunks;
   const int kc1 = min(nk_all, kc0 + g.kchunks);
-  if (kc0 >= kc1) return;
+  if (kc0 >= kc1) continue;
 
   // Rows past M / N are clamped (read in bounds, never stored).
   const char* Arow[4];
@@ -115,88 +157,106 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  float4 ra0, ra1, ra2, ra3;   // A slot (BF16 chunk, or the first half of an FP32 slot)
-  float4 rx0, rx1, rx2, rx3;   // FP32 A: second half of the slot
-  float4 rv0, rv1, rv2, rv3;   // B slot
-  int kcur = kc0;              // chunk held in the prefetch registers
-  const char* Ar0 = Arow[0];
-  const char* Ar1 = Arow[1];
-  const char* Ar2 = Arow[2];
-  const char* Ar3 = Arow[3];
-  const char* Br0 = Brow[0];
-  const char* Br1 = Brow[1];
-  const char* Br2 = Brow[2];
-  const char* Br3 = Brow[3];
+  // Register ring of NPF chunks in flight: chunk c (relative to kc0) is
+  // loaded into set c % NPF NPF chunks before it is staged to LDS, so a
+  // chunk's global-memory latency is covered by the multiplies of the NPF
+  // chunks before it (a BF16 chunk is only ~512 MFMA cycles per wavefront).
+  constexpr int NPF = AF32 ? 1 : 2;   // the FP32-input variant needs its registers for the wider slots
+  // Named registers, not arrays: identical steps on different arrays get
+  // merged by the compiler into one path indexing memory.
+  float4 ra0_0, ra0_1, ra0_2, ra0_3, ra1_0, ra1_1, ra1_2, ra1_3;   // A slots
+  float4 rx0_0, rx0_1, rx0_2, rx0_3, rx1_0, rx1_1, rx1_2, rx1_3;   // FP32 A, second halves
+  float4 rv0_0, rv0_1, rv0_2, rv0_3, rv1_0, rv1_1, rv1_2, rv1_3;   // B slots
+  const int nch = kc1 - kc0;
   // FP32 A: an 8-element slot past the row's valid length is read at the row
   // start (in bounds) and zeroed when staged.
-#define MV_GLOAD1(I, KC, AVAL)                                                                     \
-  do {                                                                                           \
-    const char* pa_ = (AVAL) ? Ar##I + static_cast<int64_t>(KC) * 64 * AEB : Ar##I - lq * 8 * AEB; \
-    ra##I = *reinterpret_cast<const float4*>(pa_);                                               \
-    if (AF32) rx##I = *reinterpret_cast<const float4*>(pa_ + 16);                                \
-    rv##I = *reinterpret_cast<const float4*>(Br##I + static_cast<int64_t>(KC) * 128);            \
+#define MV_GLOAD1(SET, I, KQ, AVAL)                                                                 \
+  do {                                                                                            \
+    const char* pa_ = (AVAL) ? Arow[I] + static_cast<int64_t>(KQ) * 64 * AEB : Arow[I] - lq * 8 * AEB; \
+    ra##SET##_##I = *reinterpret_cast<const float4*>(pa_);                                           \
+    if (AF32) rx##SET##_##I = *reinterpret_cast<const float4*>(pa_ + 16);                            \
+    rv##SET##_##I = *reinterpret_cast<const float4*>(Brow[I] + static_cast<int64_t>(KQ) * 128);      \
   } while (0)
-#define MV_GLOAD(KC)                                                                               \
-  do {                                                                                           \
-    const bool aval_ = !AF32 || (KC) * 64 + lq * 8 < g.kvalid;                                   \
-    MV_GLOAD1(0, KC, aval_);                                                                     \
-    MV_GLOAD1(1, KC, aval_);                                                                     \
-    MV_GLOAD1(2, KC, aval_);                                                                     \
-    MV_GLOAD1(3, KC, aval_);                                                                     \
+#define MV_GLOAD(SET, C)                                                                            \
+  do {                                                                                            \
+    const int kq_ = kc0 + ((C) < nch ? (C) : nch - 1);                                            \
+    const bool aval_ = !AF32 || kq_ * 64 + lq * 8 < g.kvalid;                                     \
+    MV_GLOAD1(SET, 0, kq_, aval_);                                                                \
+    MV_GLOAD1(SET, 1, kq_, aval_);                                                                \
+    MV_GLOAD1(SET, 2, kq_, aval_);                                                                \
+    MV_GLOAD1(SET, 3, kq_, aval_);                                                                \
   } while (0)
-#define MV_LSTORE1(I, AS, BS)                                                                      \
-  do {                                                                                           \
-    float4 av_ = ra##I;                                                                          \
-    if (AF32) av_ = kcur * 64 + lq * 8 >= g.kvalid ? make_float4(0.f, 0.f, 0.f, 0.f) : pack_bf16x8(ra##I, rx##I); \
-    *reinterpret_cast<float4*>((AS) + (lr + 32 * I) * LDSA + lq * 4) = av_;                      \
-    *reinterpret_cast<float4*>((BS) + (lr + 32 * I) * LDSA + lq * 4) = rv##I;                    \
+#define MV_LSTORE1(SET, I, KQ, AS, BS)                                                              \
+  do {                                                                                            \
+    float4 av_ = ra##SET##_##I;                                                                      \
+    if (AF32) av_ = (KQ) * 64 + lq * 8 >= g.kvalid ? make_float4(0.f, 0.f, 0.f, 0.f) : pack_bf16x8(ra##SET##_##I, rx##SET##_##I); \
+    *reinterpret_cast<float4*>((AS) + (lr + 32 * (I)) * LDSA + lq * 4) = av_;                     \
+    *reinterpret_cast<float4*>((BS) + (lr + 32 * (I)) * LDSA + lq * 4) = rv##SET##_##I;              \
   } while (0)
-#define MV_LSTORE(BUF)                                                                             \
-  do {                                                                                           \
-    float* As2_ = s_stage + (BUF) * (2 * BM * LDSA);                                             \
-    float* Bs2_ = As2_ + BM * LDSA;                                                              \
-    MV_LSTORE1(0, As2_, Bs2_);                                                                   \
-    MV_LSTORE1(1, As2_, Bs2_);                                                                   \
-    MV_LSTORE1(2, As2_, Bs2_);                                                                   \
-    MV_LSTORE1(3, As2_, Bs2_);                                                                   \
+#define MV_LSTORE(SET, C, BUF)                                                                      \
+  do {                                                                                            \
+    float* As2_ = s_stage + (BUF) * (2 * BM * LDSA);                                              \
+    float* Bs2_ = As2_ + BM * LDSA;                                                               \
+    const int kq_ = kc0 + (C);                                                                    \
+    MV_LSTORE1(SET, 0, kq_, As2_, Bs2_);                                                          \
+    MV_LSTORE1(SET, 1, kq_, As2_, Bs2_);                                                          \
+    MV_LSTORE1(SET, 2, kq_, As2_, Bs2_);                                                          \
+    MV_LSTORE1(SET, 3, kq_, As2_, Bs2_);                                                          \
   } while (0)
-  MV_GLOAD(kc0);
-  MV_LSTORE(0);
+  MV_GLOAD(0, 0);
+  if constexpr (NPF > 1) MV_GLOAD(1, 1);
+  MV_LSTORE(0, 0, 0);
+  MV_GLOAD(0, NPF);
   __syncthreads();
-  for (int kc = kc0; kc < kc1; ++kc) {
-    const int cur = (kc - kc0) & 1;
-    kcur = kc + 1 < kc1 ? kc + 1 : kc;
-    MV_GLOAD(kcur);
-    // keep the loads ahead of the multiplies (the scheduler would otherwise
-    // sink them past the MFMAs to reuse operand registers, exposing their latency)
-    __builtin_amdgcn_sched_barrier(0);
-    const float* As = s_stage + cur * (2 * BM * LDSA);
-    const float* Bs = As + BM * LDSA;
-#pragma unroll
-    for (int gk = 0; gk < 2; ++gk) {
-      v8bf af[4], bf[4];
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-        af[mi] = __builtin_bit_cast(
-            v8bf, *reinterpret_cast<const v4f*>(As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + gk * 16 + (l >> 4) * 4));
-#pragma unroll
-      for (int ni = 0; ni < 4; ++ni)
-        bf[ni] = __builtin_bit_cast(
-            v8bf, *reinterpret_cast<const v4f*>(Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + gk * 16 + (l >> 4) * 4));
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 4; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+  // One step per chunk t = base + U; NS = (U + 1) % NPF names the register
+  // set holding chunk t + 1.
+#define MV_STEP(U, NS)                                                                                \
+  do {                                                                                            \
+    const int t_ = base + (U);                                                                    \
+    if (t_ < nch) {                                                                               \
+      const float* As = s_stage + (t_ & 1) * (2 * BM * LDSA);                                     \
+      const float* Bs = As + BM * LDSA;                                                           \
+      _Pragma("unroll") for (int gk = 0; gk < 2; ++gk) {                                          \
+        v8bf af[4], bf[4];                                                                        \
+        _Pragma("unroll") for (int mi = 0; mi < 4; ++mi) af[mi] = __builtin_bit_cast(             \
+            v8bf, *reinterpret_cast<const v4f*>(As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + gk * 16 + (l >> 4) * 4)); \
+        _Pragma("unroll") for (int ni = 0; ni < 4; ++ni) bf[ni] = __builtin_bit_cast(             \
+            v8bf, *reinterpret_cast<const v4f*>(Bs + (wc * 64 + ni * 16 + (l & 15)) * LDSA + gk * 16 + (l >> 4) * 4)); \
+        _Pragma("unroll") for (int mi = 0; mi < 4; ++mi)                                          \
+          _Pragma("unroll") for (int ni = 0; ni < 4; ++ni)                                        \
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0); \
+      }                                                                                           \
+      if (t_ + 1 < nch) {                                                                         \
+        MV_LSTORE(NS, t_ + 1, (t_ + 1) & 1);                                                      \
+        MV_GLOAD(NS, t_ + 1 + NPF);                                                               \
+      }                                                                                           \
+      __syncthreads();                                                                            \
+    }                                                                                             \
+  } while (0)
+  for (int base = 0; base < nch; base += NPF) {
+    if constexpr (NPF == 2) {
+      MV_STEP(0, 1);
+      MV_STEP(1, 0);
+    } else {
+      MV_STEP(0, 0);
     }
-    if (kc + 1 < kc1) MV_LSTORE(cur ^ 1);
-    __syncthreads();
   }
+#undef MV_STEP
 #undef MV_GLOAD
 #undef MV_GLOAD1
 #undef MV_LSTORE
 #undef MV_LSTORE1
 
+  // Epilogue. Per-column operands (bias, W3) and per-element inputs (pre_in)
+  // are read into registers before any output is written: the compiler
+  // cannot prove that the output does not alias them and would otherwise
+  // re-load them after every store, each load paying a full memory latency.
+  float bv[4];
+#pragma unroll
+  for (int ni = 0; ni < 4; ++ni) {
+    const int col = min(bn + wc * 64 + ni * 16 + (l & 15), g.N - 1);
+    bv[ni] = (EPI == EPI_BIAS_GELU || EPI == EPI_HEAD) ? g.bias[col] : 0.f;
+  }
   if constexpr (EPI == EPI_HEAD) {
     float w3[4][4];   // [o][ni]
 #pragma unroll
@@ -215,57 +275,103 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
         for (int ni = 0; ni < 4; ++ni) {
           const int col = bn + wc * 64 + ni * 16 + (l & 15);
           if (col < g.N) {
-            const float h = bf2f(f2bf(gelu(bf2f(f2bf(acc[mi][ni][r] + g.bias[col])))));
+            const float h = bf2f(f2bf(gelu(bf2f(f2bf(acc[mi][ni][r] + bv[ni])))));
 #pragma unroll
             for (int o = 0; o < 4; ++o) part[o] = fmaf(h, w3[o][ni], part[o]);
           }
         }
-        // the 16 lanes of a row group hold its 64 columns
+        // the 16 lanes of a row group (one DPP row) hold its 64 columns
+        float* z = static_cast<float*>(g.C) + static_cast<int64_t>(row) * g.dout;
 #pragma unroll
         for (int o = 0; o < 4; ++o) {
-          float v = part[o];
-          v += __shfl_xor(v, 1);
-          v += __shfl_xor(v, 2);
-          v += __shfl_xor(v, 4);
-          v += __shfl_xor(v, 8);
-          part[o] = v;
+          if (o < g.dout) {
+            const float v = row16_sum(part[o]);
+            if ((l & 15) == 0 && row < g.M) atomicAdd(&z[o], v);
+          }
         }
-        if ((l & 15) == 0 && row < g.M)
-          for (int o = 0; o < g.dout; ++o) atomicAdd(&static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.dout + o], part[o]);
       }
-    return;
+    continue;
   }
 
+  // BF16 outputs go through LDS (the staging area is free after the K loop)
+  // and leave as 16-byte row segments instead of one 2-byte store per value.
+  constexpr bool BF16_OUT = EPI == EPI_BF16 || EPI == EPI_BIAS_GELU || EPI == EPI_GELU_GRAD;
+  constexpr int TS = BN + 8;   // LDS row stride of the output tile (BF16 elements)
+  bf16_t* s_out = reinterpret_cast<bf16_t*>(s_stage);
+  bf16_t* s_pre = s_out + BM * TS;
+  const bool vec_out = BF16_OUT && g.N % 8 == 0 && g.ldc % 8 == 0 && (!g.pre_out || g.ldx % 8 == 0);
   float csum[4] = {0.f, 0.f, 0.f, 0.f};   // EPI_GELU_GRAD: column sums of this lane's outputs
+  // VEC: BF16 outputs to the LDS tile (uniform choice, made once per tile)
+#define MV_EPI_LOOP(VEC)                                                                            \
+  _Pragma("unroll") for (int mi = 0; mi < 4; ++mi) {                                               \
+    bf16_t pre_v[4][4];   /* EPI_GELU_GRAD: [r][ni], loaded before the row group's outputs */     \
+    if constexpr (EPI == EPI_GELU_GRAD) {                                                         \
+      _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                             \
+        const int row = min(bm + wr * 64 + mi * 16 + (l >> 4) * 4 + r, g.M - 1);                  \
+        _Pragma("unroll") for (int ni = 0; ni < 4; ++ni) {                                        \
+          const int col = min(bn + wc * 64 + ni * 16 + (l & 15), g.N - 1);                        \
+          pre_v[r][ni] = g.pre_in[static_cast<int64_t>(row) * g.ldx + col];                       \
+        }                                                                                         \
+      }                                                                                           \
+    }                                                                                             \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                               \
+      const int lrow = wr * 64 + mi * 16 + (l >> 4) * 4 + r;                                      \
+      const int row = bm + lrow;                                                                  \
+      _Pragma("unroll") for (int ni = 0; ni < 4; ++ni) {                                          \
+        const int lcol = wc * 64 + ni * 16 + (l & 15);                                            \
+        const int col = bn + lcol;                                                                \
+        const bool ok = row < g.M && col < g.N;                                                   \
+        const float v = acc[mi][ni][r];                                                           \
+        if constexpr (EPI == EPI_F32) {                                                           \
+          if (ok) static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = v;          \
+        } else if constexpr (EPI == EPI_F32_ADD) {                                                \
+          if (ok) atomicAdd(&static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col], v); \
+        } else {                                                                                  \
+          bf16_t o, pb = 0;                                                                       \
+          if constexpr (EPI == EPI_BF16) {                                                        \
+            o = f2bf(v);                                                                          \
+          } else if constexpr (EPI == EPI_BIAS_GELU) {                                            \
+            pb = f2bf(v + bv[ni]);                                                                \
+            o = f2bf(gelu(bf2f(pb)));                                                             \
+          } else {                                                                                \
+            o = f2bf(v * gelu_grad(bf2f(pre_v[r][ni])));                                          \
+            if (ok) csum[ni] += bf2f(o);                                                          \
+          }                                                                                       \
+          if (VEC) {                                                                              \
+            s_out[lrow * TS + lcol] = o;                                                          \
+            if (EPI == EPI_BIAS_GELU && g.pre_out) s_pre[lrow * TS + lcol] = pb;                  \
+          } else if (ok) {                                                                        \
+            static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = o;               \
+            if (EPI == EPI_BIAS_GELU && g.pre_out) g.pre_out[static_cast<int64_t>(row) * g.ldx + col] = pb; \
+          }                                                                                       \
+        }                                                                                         \
+      }                                                                                           \
+    }                                                                                             \
+  }
+  if (vec_out) {
+    MV_EPI_LOOP(true)
+  } else {
+    MV_EPI_LOOP(false)
+  }
+#undef MV_EPI_LOOP
+  if (vec_out) {
+    __syncthreads();
+    // 128 rows x 16 segments of 8 values; thread tid takes segments tid + 256 j
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = bm + wr * 64 + mi * 16 + (l >> 4) * 4 + r;
-      if (row >= g.M) continue;
-#pragma unroll
-      for (int ni = 0; ni < 4; ++ni) {
-        const int col = bn + wc * 64 + ni * 16 + (l & 15);
-        if (col >= g.N) continue;
-        const float v = acc[mi][ni][r];
-        if (EPI == EPI_F32) {
-          static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = v;
-        } else if (EPI == EPI_F32_ADD) {
-          atomicAdd(&static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col], v);
-        } else if (EPI == EPI_BF16) {
-          static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = f2bf(v);
-        } else if (EPI == EPI_BIAS_GELU) {
-          const bf16_t pb = f2bf(v + g.bias[col]);
-          if (g.pre_out) g.pre_out[static_cast<int64_t>(row) * g.ldx + col] = pb;
-          static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = f2bf(gelu(bf2f(pb)));
-        } else {
-          const float pre = bf2f(g.pre_in[static_cast<int64_t>(row) * g.ldx + col]);
-          const bf16_t o = f2bf(v * gelu_grad(pre));
-          static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = o;
-          csum[ni] += bf2f(o);
-        }
+    for (int j = 0; j < (BM * BN / 8) / NT; ++j) {
+      const int sgi = tid + NT * j;
+      const int lrow = sgi >> 4, lcol = (sgi & 15) * 8;
+      const int row = bm + lrow, col = bn + lcol;
+      if (row < g.M && col < g.N) {
+        *reinterpret_cast<float4*>(static_cast<bf16_t*>(g.C) + static_cast<int64_t>(row) * g.ldc + col) =
+            *reinterpret_cast<const float4*>(s_out + lrow * TS + lcol);
+        if (EPI == EPI_BIAS_GELU && g.pre_out)
+          *reinterpret_cast<float4*>(g.pre_out + static_cast<int64_t>(row) * g.ldx + col) =
+              *reinterpret_cast<const float4*>(s_pre + lrow * TS + lcol);
       }
     }
+    __syncthreads();   // the next tile's staging overwrites the output tile
+  }
   if (EPI == EPI_GELU_GRAD && g.colsum) {
     // lanes l, l^16, l^32, l^48 hold the same columns
 #pragma unroll
@@ -277,6 +383,7 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(GemmArgs g) {
       if (l < 16 && col < g.N) atomicAdd(&g.colsum[col], v);
     }
   }
+  }   // tiles
 }
 
 // out[c, r] = in[r, c] for a rows x cols BF16 matrix (64 x 64 tiles through LDS).
@@ -672,8 +779,20 @@ static Batch carve_batch(void* ws, const Dims& D, int P, int64_t np, size_t* tot
   return b;
 }
 
+static int cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
 // Launches k_gemm_nt over ceil(N/128) x ceil(M/128) tiles and `splits` K
-// ranges (splits > 1 only with EPI_F32_ADD).
+// ranges (splits > 1 only with EPI_F32_ADD) on a persistent grid.
 template <int EPI, bool AF32 = false>
 static int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
   const int nk = g.K / 64;
@@ -681,9 +800,13 @@ static int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
   if (splits > nk) splits = nk;
   g.kchunks = static_cast<int>(ceil_div(nk, splits));
   splits = static_cast<int>(ceil_div(nk, g.kchunks));
-  dim3 grid(static_cast<unsigned>(ceil_div(g.N, BN)), static_cast<unsigned>(ceil_div(g.M, BM)),
-            static_cast<unsigned>(splits));
-  hipLaunchKernelGGL((k_gemm_nt<EPI, AF32>), grid, dim3(NT), 0, s, g);
+  g.m_tiles = static_cast<int>(ceil_div(g.M, BM));
+  g.n_tiles = static_cast<int>(ceil_div(g.N, BN));
+  g.splits = splits;
+  const int64_t slots = 8 * ceil_div(g.m_tiles, 8) * static_cast<int64_t>(g.n_tiles) * splits;
+  const int64_t resident = 2 * static_cast<int64_t>(cu_count());   // two workgroups per CU (LDS)
+  const int64_t grid = std::max<int64_t>(8, (std::min(slots, resident) + 7) / 8 * 8);
+  hipLaunchKernelGGL((k_gemm_nt<EPI, AF32>), dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, g);
   MV_LAUNCHED("k_gemm_nt");
   return MV_OK;
 }
