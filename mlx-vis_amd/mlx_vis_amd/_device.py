@@ -28,6 +28,14 @@ def as_device_f32(X) -> torch.Tensor:
     return torch.from_numpy(arr).to(device())
 
 
+def require_device(what: str, *tensors) -> None:
+    """Raises ValueError unless every given tensor (None skipped) lives on the
+    GPU: the kernels take raw device pointers, and a host pointer would fault."""
+    for t in tensors:
+        if t is not None and (not isinstance(t, torch.Tensor) or t.device.type != "cuda"):
+            raise ValueError(f"{what}: expected device (cuda) tensors, got {getattr(t, 'device', type(t))}")
+
+
 def workspace(nbytes: int) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
 

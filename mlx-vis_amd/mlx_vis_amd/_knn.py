@@ -42,6 +42,7 @@ def knn_brute_device(
     """
     if dtype not in _capi.DTYPES:
         raise ValueError(f"unknown kNN dtype {dtype!r}; expected one of {sorted(_capi.DTYPES)}")
+    _device.require_device("knn_brute_device", X, Q, stats)
     same = Q is None
     if same:
         Q = X

@@ -64,6 +64,7 @@ class NNDescent:
 
     def build_device(self, Xd: torch.Tensor):
         """Device-level build: returns (indices int32, Euclidean distances float32) CUDA tensors."""
+        _device.require_device("NNDescent.build_device", Xd)
         n, d = Xd.shape
         k = min(self.k, n - 1)
         if k < 1:

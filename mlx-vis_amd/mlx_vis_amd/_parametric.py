@@ -39,6 +39,7 @@ def _align(x: int, a: int = 256) -> int:
 def knn_preservation_device(knn_hi: torch.Tensor, knn_lo: torch.Tensor, threshold: float | None = None):
     """Per-row overlap counts |kNN_hi(i) & kNN_lo(i)| (int32) and, with a
     threshold, the uint8 flags count / k < threshold."""
+    _device.require_device("knn_preservation_device", knn_hi, knn_lo)
     n, k = knn_hi.shape
     if knn_lo.shape[0] != n:
         raise ValueError("neighbour lists must have the same number of rows")
@@ -208,6 +209,7 @@ class ParametricUMAP:
     def transform_device(self, Xd: torch.Tensor, chunk: int = 65536, ws: torch.Tensor | None = None) -> torch.Tensor:
         if self.blob_ is None:
             raise RuntimeError("ParametricUMAP is not fitted")
+        _device.require_device("ParametricUMAP.transform_device", Xd, ws)
         m, din = Xd.shape
         if din != self.din_:
             raise ValueError(f"expected {self.din_} features, got {din}")
@@ -238,6 +240,7 @@ class ParametricUMAP:
         (k = n_neighbors) against the exact high-D kNN and moves the points below
         `threshold` by `n_epochs` plain layout epochs on the exact-kNN graph; the
         others stay fixed. Returns (refined Z, per-point counts before)."""
+        _device.require_device("ParametricUMAP.refine_device", Xd, Zd)
         k = self.n_neighbors
         if knn is None:
             knn = knn_brute_device(Xd, k, dtype=knn_dtype, euclid=True)

@@ -81,6 +81,7 @@ class FuzzyGraph:
 def fuzzy_graph_device(knn_idx: torch.Tensor, knn_dist: torch.Tensor, n_epochs: int,
                        keep_sigma: bool = False) -> FuzzyGraph:
     """Fuzzy simplicial set + prune + schedule on the device (umap.py:199-270, 399-404)."""
+    _device.require_device("fuzzy_graph_device", knn_idx, knn_dist)
     n, k = knn_idx.shape
     dev = knn_idx.device
     knn_idx = knn_idx.to(torch.int32).contiguous()
@@ -143,14 +144,17 @@ def run_epochs(Y: torch.Tensor, ws: torch.Tensor | None, g: FuzzyGraph, e0: int,
 
 def sgd_step_device(Y: torch.Tensor, ef, et, neg_from, neg_to, alpha: float, a: float, b: float) -> torch.Tensor:
     """One exact _sgd_step (umap.py:359-387) with explicit lists; returns the new Y."""
+    _device.require_device("sgd_step_device", Y)
     n, dim = Y.shape
+    Y = Y.float().contiguous()
     out = torch.empty_like(Y)
-    ef = ef.to(torch.int32).contiguous()
-    et = et.to(torch.int32).contiguous()
-    neg_from = neg_from.to(torch.int32).contiguous()
-    neg_to = neg_to.to(torch.int32).contiguous()
+    # index lists may come from the host: the kernel must only ever see device pointers
+    ef = ef.to(device=Y.device, dtype=torch.int32).contiguous()
+    et = et.to(device=Y.device, dtype=torch.int32).contiguous()
+    neg_from = neg_from.to(device=Y.device, dtype=torch.int32).contiguous()
+    neg_to = neg_to.to(device=Y.device, dtype=torch.int32).contiguous()
     _capi.check(_capi.lib().mv_umap_sgd_step(
-        out.data_ptr(), Y.contiguous().data_ptr(), n, dim, ef.data_ptr(), et.data_ptr(), ef.numel(),
+        out.data_ptr(), Y.data_ptr(), n, dim, ef.data_ptr(), et.data_ptr(), ef.numel(),
         neg_from.data_ptr(), neg_to.data_ptr(), neg_from.numel(), float(alpha), float(a), float(b),
         _device.stream()), "mv_umap_sgd_step")
     return out

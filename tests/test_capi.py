@@ -53,3 +53,25 @@ def test_fault_injection_maps_to_mivis_error(monkeypatch):
         _capi.check(_capi.lib().mv_row_sqnorms(ctypes.c_void_p(16), 4, 4, ctypes.c_void_p(16), None))
     assert ei.value.status == _capi.MV_EINJECTED
     assert "MIVIS_FAULT" in str(ei.value)
+
+
+def test_device_entry_points_reject_host_tensors():
+    """The kernels take raw device pointers: host tensors must be refused
+    before any launch (a host pointer in a kernel faults the GPU)."""
+    import torch
+
+    from mlx_vis_amd import NNDescent, knn_brute_device, knn_preservation_device
+    from mlx_vis_amd._umap import fuzzy_graph_device, sgd_step_device
+
+    X = torch.zeros((16, 4))
+    idx = torch.zeros((16, 3), dtype=torch.int32)
+    with pytest.raises(ValueError, match="device"):
+        knn_brute_device(X, 3)
+    with pytest.raises(ValueError, match="device"):
+        NNDescent(k=3).build_device(X)
+    with pytest.raises(ValueError, match="device"):
+        knn_preservation_device(idx, idx)
+    with pytest.raises(ValueError, match="device"):
+        fuzzy_graph_device(idx, torch.zeros((16, 3)), 10)
+    with pytest.raises(ValueError, match="device"):
+        sgd_step_device(torch.zeros((16, 2)), idx[:, 0], idx[:, 1], idx[:, 0], idx[:, 1], 1.0, 1.5, 0.9)
