@@ -207,6 +207,7 @@ constexpr int NEG_CHUNK = 8;
 
 template <int D, int G>
 __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ gtail,
+                                               const int32_t* __restrict__ pos_in,
                                                float* __restrict__ dHead, const int32_t* __restrict__ tail,
                                                const float* __restrict__ eps, float* __restrict__ epn, int64_t n,
                                                int k, int epoch, int neg_rate, Law law, uint32_t seed_lo,
@@ -290,8 +291,12 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     }
   }
   if (in_row) {
+    // in-edge order (pos_in set): the slot's entry in the tail-sorted list (< 0: pruned)
+    const int64_t pos = pos_in ? pos_in[e] : e;
+    if (pos >= 0) {
 #pragma unroll
-    for (int q = 0; q < D; ++q) gtail[e * D + q] = gt[q];
+      for (int q = 0; q < D; ++q) gtail[static_cast<int64_t>(pos) * D + q] = gt[q];
+    }
   }
   // Reduce the head's updates over its G lanes (contiguous in the wavefront).
 #pragma unroll
@@ -305,14 +310,18 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   }
 }
 
-// Phase 2: tail sums over the in-edge list (slots grouped by tail node,
-// in_node[t] the node of entry t). A wavefront-segmented inclusive scan adds
-// each node's consecutive entries; the last lane of a segment adds the
-// segment's total to dTail with one atomic, so a hub with m in-edges costs
-// about m / 64 atomics per epoch instead of m.
+// Phase 2: tail sums. Phase 1 stored the tail gradients in in-edge order
+// (entries of a node contiguous, in_node[t] the node of entry t), read
+// contiguously, or, for buffers larger than the caches, in slot order,
+// gathered through in_slots (scattered 8-byte writes cost more than scattered
+// reads once the lines are not cache resident); a wavefront-segmented inclusive scan adds each node's
+// entries and the last lane of a segment adds the segment's total to dTail
+// with one atomic (a hub with m in-edges costs about m / 64 atomics). A node
+// is split over at most two wavefronts unless it has more than 64 in-edges, so
+// its total does not depend on the order of the atomics (two terms commute):
+// runs with the same seed repeat.
 template <int D>
-__global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtail,
-                                                  const int32_t* __restrict__ in_slots,
+__global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtail, const int32_t* __restrict__ in_slots,
                                                   const int32_t* __restrict__ in_node, int64_t m,
                                                   float* __restrict__ dTail) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -320,14 +329,10 @@ __global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtai
   const bool valid = t < m;
   const int node = valid ? in_node[t] : -1;
   float v[D];
-  if (valid) {
-    const int64_t e = in_slots[t];
+  // in_slots == nullptr: stored in in-edge order; else in slot order, gathered
+  const int64_t src = valid ? (in_slots ? static_cast<int64_t>(in_slots[t]) : t) : 0;
 #pragma unroll
-    for (int q = 0; q < D; ++q) v[q] = gtail[e * D + q];
-  } else {
-#pragma unroll
-    for (int q = 0; q < D; ++q) v[q] = 0.f;
-  }
+  for (int q = 0; q < D; ++q) v[q] = valid ? gtail[src * D + q] : 0.f;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const int on = __shfl_up(node, off);
@@ -345,9 +350,8 @@ __global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtai
   }
 }
 
-// Phase 3: Y += dHead + dTail; dTail = 0.
-// Y += dHead + dTail, then dTail = 0. With `movable` set, rows whose flag is
-// 0 keep their position (their gradients are dropped).
+// Phase 3: Y += dHead + dTail, then dTail = 0. With `movable` set, rows whose
+// flag is 0 keep their position (their gradients are dropped).
 __global__ __launch_bounds__(256) void k_apply2(float* __restrict__ Y, const float* __restrict__ dHead,
                                                 float* __restrict__ dTail, int64_t count, int dim,
                                                 const uint8_t* __restrict__ movable) {
@@ -356,6 +360,18 @@ __global__ __launch_bounds__(256) void k_apply2(float* __restrict__ Y, const flo
     if (!movable || movable[t / dim]) Y[t] += dHead[t] + dTail[t];
     dTail[t] = 0.f;
   }
+}
+
+// pos_in[e] = t for the in-edge list entry t of slot e (pos_in pre-filled with -1).
+__global__ __launch_bounds__(256) void k_pos_in(const int32_t* __restrict__ in_slots, int64_t m,
+                                                int32_t* __restrict__ pos_in) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < m) pos_in[in_slots[t]] = static_cast<int32_t>(t);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int64_t count, int32_t v) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < count) p[t] = v;
 }
 
 // In-edge lists of the kept slots: counts per tail, exclusive scan, atomic
@@ -692,14 +708,18 @@ static size_t spectral_ws(int64_t n, int k, int nc) {
 }
 
 struct EpochBufs {
-  float* gtail;    // n*k*dim
-  float* dHead;    // n*dim
-  float* dTail;    // n*dim
+  float* gtail;     // n*k*dim, in in-edge order
+  float* dHead;     // n*dim
+  float* dTail;     // n*dim
+  int32_t* pos_in;  // n*k: in-edge position of each slot (-1: pruned); nullptr: slot order
 };
+
+// Tail gradients in in-edge order while their buffer stays cache resident.
+constexpr size_t GTAIL_INEDGE_MAX = size_t(32) << 20;
 
 static size_t epochs_ws(int64_t n, int k, int dim) {
   return align_up(static_cast<size_t>(n) * k * dim * 4, 256) + 2 * align_up(static_cast<size_t>(n) * dim * 4, 256) +
-         256;
+         align_up(static_cast<size_t>(n) * k * 4, 256) + 256;
 }
 
 static EpochBufs carve_epochs(void* ws, int64_t n, int k, int dim) {
@@ -708,6 +728,8 @@ static EpochBufs carve_epochs(void* ws, int64_t n, int k, int dim) {
   b.gtail = c.take<float>(static_cast<size_t>(n) * k * dim);
   b.dHead = c.take<float>(static_cast<size_t>(n) * dim);
   b.dTail = c.take<float>(static_cast<size_t>(n) * dim);
+  b.pos_in = c.take<int32_t>(static_cast<size_t>(n) * k);
+  if (static_cast<size_t>(n) * k * dim * 4 > GTAIL_INEDGE_MAX) b.pos_in = nullptr;
   return b;
 }
 
@@ -716,7 +738,8 @@ static int launch_epoch(const float* Y, const EpochBufs& B, const int32_t* tail,
                         int64_t n, int k, int epoch, int neg_rate, Law law, uint64_t seed, hipStream_t s) {
   const int64_t threads = n * G;
   const unsigned blocks = static_cast<unsigned>(ceil_div(threads, 256));
-  hipLaunchKernelGGL((k_epoch<D, G>), dim3(blocks), dim3(256), 0, s, Y, B.gtail, B.dHead, tail, eps, epn, n, k,
+  hipLaunchKernelGGL((k_epoch<D, G>), dim3(blocks), dim3(256), 0, s, Y, B.gtail, B.pos_in, B.dHead, tail, eps, epn,
+                     n, k,
                      epoch, neg_rate, law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
   MV_LAUNCHED("k_epoch");
   return MV_OK;
@@ -734,7 +757,7 @@ static int run_epoch(float* Y, const EpochBufs& B, const int32_t* tail, const fl
   if (rc) return rc;
   if (n_in > 0) {
     hipLaunchKernelGGL((k_tail_sum<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, B.gtail,
-                       in_slots, in_node, n_in, B.dTail);
+                       B.pos_in ? nullptr : in_slots, in_node, n_in, B.dTail);
     MV_LAUNCHED("k_tail_sum");
   }
   hipLaunchKernelGGL(k_apply2, dim3(static_cast<unsigned>(ceil_div(n * D, 256))), dim3(256), 0, s, Y, B.dHead,
@@ -874,9 +897,19 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
   MV_FAULT_POINT("launch");
   hipStream_t s = static_cast<hipStream_t>(stream);
   EpochBufs B = carve_epochs(ws, n, k, dim);
-  const int64_t count = n * dim;
-  hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, B.dTail, count);
+  hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(n * dim, 256))), dim3(256), 0, s, B.dTail,
+                     n * dim);
   MV_LAUNCHED("k_zero_f32");
+  if (B.pos_in) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>(ceil_div(n * k, 256))), dim3(256), 0, s, B.pos_in, n * k,
+                       -1);
+    MV_LAUNCHED("k_fill_i32");
+  }
+  if (B.pos_in && n_in > 0) {
+    hipLaunchKernelGGL(k_pos_in, dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, in_slots, n_in,
+                       B.pos_in);
+    MV_LAUNCHED("k_pos_in");
+  }
   for (int ep = e0; ep < e1; ++ep) {
     Law law;
     law.a = a;
