@@ -299,6 +299,16 @@ int mv_adam_update(float* Y, float* grad, float* adam_m, float* adam_v, int64_t 
  * the spectral start vectors, which the reference draws from MLX's generator
  * (umap.py:325), a stream that cannot be reproduced outside MLX. */
 int mv_random_normal(float* out, int64_t count, uint64_t seed, void* stream);
+/* Spectral embedding to [0, 10] (the tail of _spectral_init, umap.py:344-348):
+ * e = V[:, 1:nc] * (10 / max |V[:, 1:nc]|) + 1e-4 * noise, then per column
+ * Y = 10 (e - min) / (max - min + 1e-10). V: n x nc device (as mv_umap_spectral
+ * leaves it), Y: n x (nc - 1) device. The noise is standard normal from
+ * Philox-4x32-10 keyed by seed (the reference draws it from NumPy's legacy
+ * generator). out_nonfinite (device int32, nullable) receives 1 if V holds a
+ * non-finite value (the reference then falls back to a random init).
+ * ws: >= 64 bytes of device memory. */
+int mv_umap_spectral_embed(const float* V, int64_t n, int nc, uint64_t seed, float* Y, int32_t* out_nonfinite,
+                           void* ws, size_t ws_bytes, void* stream);
 
 /* p[0:count] = 0 on the stream (a fill kernel, safe under graph capture). */
 int mv_zero_f32(float* p, int64_t count, void* stream);

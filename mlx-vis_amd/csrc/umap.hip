@@ -29,6 +29,8 @@
 // by the seed and counted by (epoch, edge slot), so a run is reproducible.
 #include <math.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace mv {
@@ -786,6 +788,109 @@ __global__ __launch_bounds__(256) void k_random_normal(float* __restrict__ out, 
     if (i * 4 + q < count) out[i * 4 + q] = v[q];
 }
 
+// Spectral embedding to [0, 10] (umap.py:344-348): e = V[:, 1:nc] * (10 / max|V[:, 1:nc]|)
+// + noise, then per column 10 (e - min) / (max - min + 1e-10). `scal` holds the
+// maximum |v| (float bits), the per-column minima and maxima (order-preserving
+// integer keys) and a non-finite flag.
+__device__ __forceinline__ uint32_t ordered_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ordered_value(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+struct EmbedScal {
+  uint32_t absmax;
+  uint32_t lo[4];   // min keys
+  uint32_t hi[4];   // max keys
+  int32_t nonfinite;
+};
+
+__global__ void k_embed_init(EmbedScal* sc) {
+  if (threadIdx.x == 0) {
+    sc->absmax = 0u;
+    for (int q = 0; q < 4; ++q) {
+      sc->lo[q] = 0xFFFFFFFFu;
+      sc->hi[q] = 0u;
+    }
+    sc->nonfinite = 0;
+  }
+}
+
+__device__ __forceinline__ uint32_t block_reduce_max_u32(uint32_t v, uint32_t* red) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off)));
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  uint32_t r = red[0];
+  for (int i = 1; i < static_cast<int>(blockDim.x >> 6); ++i) r = max(r, red[i]);
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_embed_absmax(const float* __restrict__ V, int64_t n, int nc,
+                                                      EmbedScal* __restrict__ sc) {
+  __shared__ uint32_t red[4];
+  const int dim = nc - 1;
+  uint32_t m = 0u;
+  int bad = 0;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n * dim;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const float v = V[(t / dim) * nc + 1 + t % dim];
+    if (!isfinite(v)) bad = 1;
+    else m = max(m, __float_as_uint(fabsf(v)));
+  }
+  m = block_reduce_max_u32(m, red);
+  const uint32_t anybad = block_reduce_max_u32(static_cast<uint32_t>(bad), red);
+  if (threadIdx.x == 0) {
+    atomicMax(&sc->absmax, m);
+    if (anybad) atomicOr(&sc->nonfinite, 1);
+  }
+}
+
+// Y holds the noise draws on entry (standard normal); Y = e * expansion + 1e-4 * noise.
+__global__ __launch_bounds__(256) void k_embed_scale(const float* __restrict__ V, int64_t n, int nc,
+                                                     float* __restrict__ Y, EmbedScal* __restrict__ sc) {
+  __shared__ uint32_t red[4];
+  const int dim = nc - 1;
+  const float amax = __uint_as_float(sc->absmax);
+  const float expansion = 10.0f / amax;
+  uint32_t lo[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[4] = {0u, 0u, 0u, 0u};
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n * dim;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int q = static_cast<int>(t % dim);
+    const float e = V[(t / dim) * nc + 1 + q] * expansion + Y[t] * 0.0001f;
+    Y[t] = e;
+    const uint32_t key = ordered_key(e);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c == q) {
+        lo[c] = min(lo[c], key);
+        hi[c] = max(hi[c], key);
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (c >= dim) break;
+    const uint32_t l = ~block_reduce_max_u32(~lo[c], red);
+    const uint32_t h = block_reduce_max_u32(hi[c], red);
+    if (threadIdx.x == 0) {
+      atomicMin(&sc->lo[c], l);
+      atomicMax(&sc->hi[c], h);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_embed_norm(float* __restrict__ Y, int64_t n, int dim,
+                                                    const EmbedScal* __restrict__ sc) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n * dim) return;
+  const int q = static_cast<int>(t % dim);
+  const float lo = ordered_value(sc->lo[q]), hi = ordered_value(sc->hi[q]);
+  Y[t] = 10.0f * (Y[t] - lo) / (hi - lo + 1e-10f);
+}
+
 }  // namespace umap
 }  // namespace mv
 
@@ -961,6 +1066,35 @@ int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, 
     case 4: return spectral_run<4>(w_sym, eps, tail, n, k, iters, V, w, s);
     default: return spectral_run<5>(w_sym, eps, tail, n, k, iters, V, w, s);
   }
+}
+
+int mv_umap_spectral_embed(const float* V, int64_t n, int nc, uint64_t seed, float* Y, int32_t* out_nonfinite,
+                           void* ws, size_t ws_bytes, void* stream) {
+  MV_REQUIRE(V && Y, "mv_umap_spectral_embed: null pointer");
+  MV_REQUIRE(n >= 1 && nc >= 2 && nc <= 5, "mv_umap_spectral_embed: bad n=%lld nc=%d", (long long)n, nc);
+  if (!ws || ws_bytes < sizeof(EmbedScal)) {
+    mv::set_error("mv_umap_spectral_embed: workspace of %zu bytes < required %zu", ws_bytes, sizeof(EmbedScal));
+    return MV_EWORKSPACE;
+  }
+  MV_FAULT_POINT("launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  EmbedScal* sc = static_cast<EmbedScal*>(ws);
+  const int dim = nc - 1;
+  const int64_t count = n * dim;
+  const unsigned red_grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(count, 256), 1024));
+  hipLaunchKernelGGL(k_embed_init, dim3(1), dim3(64), 0, s, sc);
+  MV_LAUNCHED("k_embed_init");
+  hipLaunchKernelGGL(k_random_normal, dim3(static_cast<unsigned>(ceil_div(ceil_div(count, 4), 256))), dim3(256), 0, s,
+                     Y, count, seed);
+  MV_LAUNCHED("k_random_normal");
+  hipLaunchKernelGGL(k_embed_absmax, dim3(red_grid), dim3(256), 0, s, V, n, nc, sc);
+  MV_LAUNCHED("k_embed_absmax");
+  hipLaunchKernelGGL(k_embed_scale, dim3(red_grid), dim3(256), 0, s, V, n, nc, Y, sc);
+  MV_LAUNCHED("k_embed_scale");
+  hipLaunchKernelGGL(k_embed_norm, dim3(static_cast<unsigned>(ceil_div(count, 256))), dim3(256), 0, s, Y, n, dim, sc);
+  MV_LAUNCHED("k_embed_norm");
+  if (out_nonfinite) MV_TRY_HIP(hipMemcpyAsync(out_nonfinite, &sc->nonfinite, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  return MV_OK;
 }
 
 int mv_random_normal(float* out, int64_t count, uint64_t seed, void* stream) {

@@ -3,12 +3,12 @@
 The start vectors (drawn by the reference from MLX's generator, umap.py:325,
 a stream that cannot be reproduced here) are standard normals from
 mv_random_normal, keyed by random_state; the power iteration runs in libmivis
-(mv_umap_spectral); the final scaling is the reference's host NumPy code path:
-drop the first vector, scale to max |v| = 10, add N(0, 1e-4) noise from
-RandomState(random_state or 42) (drawn while the device iterates), then
-min-max each column to [0, 10] (umap.py:339-348). Any failure (including a
-non-finite result) falls back to N(0, 1) * 0.01 as the reference's
-``except Exception`` does (umap.py:354-357).
+(mv_umap_spectral); the final scaling (umap.py:339-348: drop the first vector,
+scale to max |v| = 10, add N(0, 1e-4) noise, min-max each column to [0, 10])
+runs on the device too (mv_umap_spectral_embed), with the noise drawn from
+Philox keyed by random_state where the reference draws it from NumPy's
+RandomState(random_state or 42). A non-finite spectral result falls back to
+N(0, 1) * 0.01 as the reference's ``except Exception`` does (umap.py:354-357).
 """
 
 from __future__ import annotations
@@ -37,18 +37,17 @@ def spectral_init(g, n: int, n_components: int, random_state=None, verbose: bool
         _capi.check(lib.mv_umap_spectral(g.w_sym.data_ptr(), g.eps.data_ptr(), g.knn_idx.data_ptr(), n, g.k, nc,
                                          int(iters), V.data_ptr(), ws.data_ptr(), ws.numel(), _device.stream()),
                     "mv_umap_spectral")
-        rng = np.random.RandomState(random_state or 42)
-        noise = rng.normal(scale=0.0001, size=(n, n_components)).astype(np.float32)
-        emb = V[:, 1:nc].cpu().numpy()
-        if not np.isfinite(emb).all():
+        Y = torch.empty((n, n_components), dtype=torch.float32, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        seed2 = (seed ^ 0x6A09E667F3BCC909) * 0xBF58476D1CE4E5B9 & 0xFFFFFFFFFFFFFFFF
+        _capi.check(lib.mv_umap_spectral_embed(V.data_ptr(), n, nc, seed2, Y.data_ptr(), flag.data_ptr(),
+                                               ws.data_ptr(), ws.numel(), _device.stream()),
+                    "mv_umap_spectral_embed")
+        if int(flag.item()) != 0:
             raise FloatingPointError("non-finite spectral embedding")
-        expansion = 10.0 / np.max(np.abs(emb))
-        emb = (emb * expansion).astype(np.float32)
-        emb += noise
-        emb = 10.0 * (emb - emb.min(axis=0)) / (emb.max(axis=0) - emb.min(axis=0) + 1e-10)
         if verbose:
             print("Using spectral initialization")
-        return torch.from_numpy(np.ascontiguousarray(emb, dtype=np.float32)).to(dev)
+        return Y
     except (_capi.MivisError, ValueError):
         raise
     except Exception as e:  # noqa: BLE001 - mirrors the reference's fallback

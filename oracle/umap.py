@@ -185,3 +185,15 @@ def spectral_power(rows, cols, vals, n, V0, iters=100):
                 V[:, j] -= (V[:, j] @ V[:, i]) * V[:, i]
             V[:, j] /= np.sqrt(V[:, j] @ V[:, j] + 1e-10)
     return V
+
+
+def spectral_scale(V, noise):
+    """Scaling of the spectral vectors to [0, 10] — umap.py:339-348, FP32 as
+    the reference's NumPy code: drop column 0, scale to max |v| = 10, add the
+    noise (already N(0, 1); scaled by 1e-4 here), min-max each column."""
+    e = np.asarray(V, dtype=F32)[:, 1:]
+    expansion = F32(10.0) / np.max(np.abs(e))
+    e = (e * expansion).astype(F32)
+    e = e + (np.asarray(noise, dtype=F32) * F32(1e-4)).astype(F32)
+    lo, hi = e.min(axis=0), e.max(axis=0)
+    return (F32(10.0) * (e - lo) / (hi - lo + F32(1e-10))).astype(F32)

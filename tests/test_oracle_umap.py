@@ -125,6 +125,29 @@ def test_spectral_power_matches_eigsh_on_symmetric_graph():
     assert abs(abs(top[:, 0] @ V[:, 0]) - 1) < 1e-6
 
 
+def test_spectral_scale_hand_worked():
+    """umap.py:339-348 on a hand-worked case: column 0 dropped, max |v| -> 10,
+    1e-4 noise scale, each column min-maxed to [0, 10]."""
+    V = np.array([[9.0, 1.0, -2.0], [-7.0, 3.0, 0.0], [3.0, -4.0, 2.0]], dtype=np.float32)
+    out = ou.spectral_scale(V, np.zeros((3, 2)))
+    # e = V[:, 1:] * 10 / 4 = [[2.5, -5], [7.5, 0], [-10, 5]]
+    np.testing.assert_allclose(out, [[12.5 / 17.5 * 10, 0.0], [10.0, 5.0], [0.0, 10.0]], rtol=1e-6, atol=1e-5)
+    # noise N(0,1) enters as 1e-4 * noise: 5000 adds 0.5 -> e col0 = [2.5, 7.5, -9.5]
+    noise = np.array([[0.0, 0.0], [0.0, 0.0], [5000.0, 0.0]])
+    out = ou.spectral_scale(V, noise)
+    np.testing.assert_allclose(out[:, 0], [12.0 / 17.0 * 10, 10.0, 0.0], rtol=1e-6, atol=1e-5)
+    # invariant to positive rescaling of V and to column 0 (without noise)
+    rng = np.random.default_rng(3)
+    W = rng.standard_normal((50, 3)).astype(np.float32)
+    a = ou.spectral_scale(W, np.zeros((50, 2)))
+    W2 = W * np.float32(4.0)
+    W2[:, 0] = rng.standard_normal(50)
+    np.testing.assert_allclose(ou.spectral_scale(W2, np.zeros((50, 2))), a, atol=1e-5)
+    # a sign flip of a column mirrors it: y -> 10 - y
+    W3 = W.copy(); W3[:, 2] *= -1
+    np.testing.assert_allclose(ou.spectral_scale(W3, np.zeros((50, 2)))[:, 1], 10.0 - a[:, 1], atol=1e-5)
+
+
 def test_trustworthiness_matches_sklearn():
     from sklearn.manifold import trustworthiness
     rng = np.random.default_rng(5)

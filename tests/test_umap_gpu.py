@@ -136,6 +136,44 @@ def test_spectral_init_subspace_matches_oracle():
     assert s.min() > 0.99
 
 
+@pytest.mark.parametrize("n,nc", [(1, 2), (7, 3), (100_003, 3), (4099, 5)])
+def test_spectral_embed_matches_oracle(n, nc):
+    """mv_umap_spectral_embed against the oracle's umap.py:339-348 scaling, fed
+    the same N(0, 1) noise (mv_random_normal with the kernel's seed and count);
+    a non-finite V raises the flag."""
+    import ctypes
+
+    from mlx_vis_amd import _capi, _device
+
+    lib = _capi.lib()
+    rng = np.random.default_rng(n)
+    V0 = rng.standard_normal((n, nc)).astype(np.float32) * np.float32(0.01)
+    V = torch.from_numpy(V0).cuda()
+    Y = torch.empty((n, nc - 1), dtype=torch.float32, device="cuda")
+    noise = torch.empty(n * (nc - 1), dtype=torch.float32, device="cuda")
+    flag = torch.full((1,), 7, dtype=torch.int32, device="cuda")
+    ws = _device.workspace(64)
+    seed = 0x1234567 + n
+    _capi.check(lib.mv_random_normal(noise.data_ptr(), noise.numel(), seed, None))
+    _capi.check(lib.mv_umap_spectral_embed(V.data_ptr(), n, nc, seed, Y.data_ptr(), flag.data_ptr(),
+                                           ws.data_ptr(), ws.numel(), None))
+    assert int(flag.item()) == 0
+    want = ou.spectral_scale(V0, noise.cpu().numpy().reshape(n, nc - 1))
+    got = Y.cpu().numpy()
+    # the device may contract v * expansion + 1e-4 * noise into one FMA
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+    if n > 1:
+        np.testing.assert_allclose(got.min(axis=0), 0.0, atol=1e-6)
+        np.testing.assert_allclose(got.max(axis=0), 10.0, atol=1e-5)
+    V0[n // 2, nc - 1] = np.nan
+    V.copy_(torch.from_numpy(V0))
+    _capi.check(lib.mv_umap_spectral_embed(V.data_ptr(), n, nc, seed, Y.data_ptr(), flag.data_ptr(),
+                                           ws.data_ptr(), ws.numel(), None))
+    assert int(flag.item()) == 1
+    small = ctypes.c_size_t(8)
+    assert lib.mv_umap_spectral_embed(V.data_ptr(), n, nc, seed, Y.data_ptr(), None, ws.data_ptr(), small, None) != 0
+
+
 def test_fit_transform_end_to_end():
     from mlx_vis_amd import UMAP
 
