@@ -3,7 +3,7 @@
 // Replaces, in mlx_vis/_umap/umap.py:
 //   _fuzzy_simplicial_set (199-270)  -> k_fuzzy_rows, k_fuzzy_sym, k_fuzzy_sched
 //   _optimize schedule    (400-414)  -> the per-edge epochs_per_next test in k_epoch
-//   _optimize epoch loop  (423-447)  -> k_epoch + k_apply per epoch
+//   _optimize epoch loop  (423-447)  -> k_epoch + k_tail_apply per epoch
 //   _sgd_step             (359-387)  -> k_sgd_step (explicit negatives, for parity)
 //
 // Graph layout in HBM: the reference keeps the forward edges (i, knn[i, c]) in
@@ -174,130 +174,166 @@ __global__ __launch_bounds__(256) void k_fuzzy_sched(const float* __restrict__ w
 
 // -------------------------------------------------------------- gradients --
 
-// x^y for x >= 1e-6 as exp2(y * log2(x)) with the hardware v_log_f32 /
-// v_exp_f32 (about 1 ulp each); relative error ~|y log2 x| * 2^-23, below
-// 1e-6 over the clamped range used here.
-__device__ __forceinline__ float pow_pos(float x, float y) { return exp2f(y * __log2f(x)); }
+// Gradient coefficients on the hardware v_log_f32 / v_exp_f32 / v_rcp_f32
+// (about 1 ulp each; no range scaling is needed: d2 is clamped to >= 1e-6, so
+// log2 d2 >= -19.94 and the exponents b log2 d2 stay far from the denormal
+// range). log2 d2 is shared by d2^b and d2^(b-1). Relative error of a
+// coefficient ~ (|b log2 d2| + 3) 2^-23, below 3e-6 over the clamped range.
+__device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 struct Law {
   float a, b, alpha;
-  // Attractive coefficient of umap.py:367-370 (FP32, same operation order).
+  // Attractive coefficient of umap.py:367-370: -2ab d2^(b-1) / (1 + a d2^b).
   __device__ __forceinline__ float attract(float d2) const {
     d2 = fmaxf(d2, 1e-6f);
-    const float pw = pow_pos(d2, b);
-    return -2.0f * a * b * pow_pos(d2, b - 1.0f) / (1.0f + a * pw);
+    const float l = hw_log2(d2);
+    const float pw = hw_exp2(b * l);
+    const float pw1 = hw_exp2((b - 1.0f) * l);
+    return -2.0f * a * b * pw1 * hw_rcp(1.0f + a * pw);
   }
-  // Repulsive coefficient of umap.py:377-380.
+  // Repulsive coefficient of umap.py:377-380: 2b / ((0.001 + d2)(1 + a d2^b)).
   __device__ __forceinline__ float repulse(float d2) const {
     d2 = fmaxf(d2, 1e-6f);
-    const float pw = pow_pos(d2, b);
-    return 2.0f * b / ((0.001f + d2) * (1.0f + a * pw));
+    const float pw = hw_exp2(b * hw_log2(d2));
+    return 2.0f * b * hw_rcp((0.001f + d2) * (1.0f + a * pw));
   }
 };
 
 __device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 4.0f); }
 
-// One synchronous epoch, phase 1, over the dense slot array. Lanes are
-// grouped per head row (G lanes, G = next power of two >= k). An active edge
-// issues all its gathers (head, tail and up to 8 negatives drawn from
-// Philox-4x32-10 counter (slot, epoch, group of 4)) before computing. The
-// head's attractive and repulsive updates are reduced over the row's lanes and
-// stored to dHead[i]; the tail's update (minus the attractive gradient) is
-// stored per slot to gtail[e] (zero for inactive slots). No atomics: a point
-// that is the tail of many edges (a hub of the kNN graph) is summed in phase 2.
+// One synchronous epoch, phase 1, over the dense slot array. Each head row
+// is served by G lanes, each lane holding S slots (c, c + G, ...; G S >= k),
+// so that a lane keeps several independent gathers in flight. Loads that do
+// not depend on each other are issued first (slot schedules, tail indices,
+// in-edge positions, head coordinates), then the tail gathers; per chunk of
+// up to 8 negatives (Philox-4x32-10 counter (slot, epoch, group of 4)) all
+// slots' draws are computed and gathered before any gradient. The head's
+// updates are reduced over the row's lanes into the next positions Ynext[i]
+// (Y is double buffered); the tail's update (minus the attractive gradient)
+// is stored per slot to gtail. No
+// atomics: a point that is the tail of many edges (a hub of the kNN graph) is
+// summed in phase 2. In in-edge order (pos_in set) only active slots store,
+// since phase 2 clears what it reads; in slot order every slot stores (zero
+// when inactive).
 constexpr int NEG_CHUNK = 8;
 
-template <int D, int G>
-__global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ gtail,
-                                               const int32_t* __restrict__ pos_in,
-                                               float* __restrict__ dHead, const int32_t* __restrict__ tail,
+template <int D, int G, int S>
+__global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ Ynext,
+                                               float* __restrict__ gtail, const int32_t* __restrict__ pos_in,
+                                               float* __restrict__ wbase, const int32_t* __restrict__ span,
+                                               const uint8_t* __restrict__ movable, const int32_t* __restrict__ tail,
                                                const float* __restrict__ eps, float* __restrict__ epn, int64_t n,
                                                int k, int epoch, int neg_rate, Law law, uint32_t seed_lo,
                                                uint32_t seed_hi) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;      // head row
   const int c = static_cast<int>(gid % G);
-  const bool in_row = i < n && c < k;
-  const int64_t e = i * k + c;
-  float acc[D], gt[D];
+  float acc[D];
 #pragma unroll
-  for (int q = 0; q < D; ++q) {
-    acc[q] = 0.f;
-    gt[q] = 0.f;
-  }
-  bool active = false;
-  if (in_row) {
-    const float s = eps[e];
-    if (s > 0.f) {
-      const float nx = epn[e];
-      if (nx <= static_cast<float>(epoch)) {
-        active = true;
-        epn[e] = nx + s;
-      }
+  for (int q = 0; q < D; ++q) acc[q] = 0.f;
+  int64_t e[S], j[S], pos[S];
+  float sch[S], nx[S];
+  bool in_row[S], act[S];
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    e[u] = i * k + c + u * G;
+    in_row[u] = i < n && c + u * G < k;
+    sch[u] = -1.f;
+    nx[u] = INFINITY;
+    j[u] = 0;
+    pos[u] = -1;
+    if (in_row[u]) {
+      sch[u] = eps[e[u]];
+      nx[u] = epn[e[u]];
+      j[u] = tail[e[u]];
+      pos[u] = pos_in ? static_cast<int64_t>(pos_in[e[u]]) : e[u];
     }
   }
-  if (active) {
-    const int64_t j = tail[e];
-    float yi[D], yj[D];
+  float yi[D];
 #pragma unroll
-    for (int q = 0; q < D; ++q) {
-      yi[q] = Y[i * D + q];
-      yj[q] = Y[j * D + q];
+  for (int q = 0; q < D; ++q) yi[q] = i < n ? Y[i * D + q] : 0.f;
+  float yj[S][D];
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    act[u] = in_row[u] && sch[u] > 0.f && nx[u] <= static_cast<float>(epoch);
+#pragma unroll
+    for (int q = 0; q < D; ++q) yj[u][q] = 0.f;
+    if (act[u]) {
+      epn[e[u]] = nx[u] + sch[u];
+#pragma unroll
+      for (int q = 0; q < D; ++q) yj[u][q] = Y[j[u] * D + q];
     }
-    for (int base = 0; base < neg_rate; base += NEG_CHUNK) {
-      const uint4 r0 = Philox::gen(make_uint4(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32),
-                                              static_cast<uint32_t>(epoch), static_cast<uint32_t>(base >> 2)),
-                                   seed_lo, seed_hi);
-      const uint4 r1 = Philox::gen(make_uint4(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32),
-                                              static_cast<uint32_t>(epoch), static_cast<uint32_t>((base >> 2) + 1)),
-                                   seed_lo, seed_hi);
-      const uint32_t words[NEG_CHUNK] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
-      float yt[NEG_CHUNK][D];
+  }
+  for (int base = 0; base < neg_rate; base += NEG_CHUNK) {
+    float yt[S][NEG_CHUNK][D];
 #pragma unroll
-      for (int s2 = 0; s2 < NEG_CHUNK; ++s2) {
-        if (base + s2 < neg_rate) {
-          const int64_t t = uniform_below(words[s2], static_cast<uint32_t>(n));
+    for (int u = 0; u < S; ++u) {
+      if (act[u]) {
+        const uint4 r0 = Philox::gen(make_uint4(static_cast<uint32_t>(e[u]), static_cast<uint32_t>(e[u] >> 32),
+                                                static_cast<uint32_t>(epoch), static_cast<uint32_t>(base >> 2)),
+                                     seed_lo, seed_hi);
+        const uint4 r1 = Philox::gen(make_uint4(static_cast<uint32_t>(e[u]), static_cast<uint32_t>(e[u] >> 32),
+                                                static_cast<uint32_t>(epoch), static_cast<uint32_t>((base >> 2) + 1)),
+                                     seed_lo, seed_hi);
+        const uint32_t words[NEG_CHUNK] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 #pragma unroll
-          for (int q = 0; q < D; ++q) yt[s2][q] = Y[t * D + q];
-        }
-      }
+        for (int s2 = 0; s2 < NEG_CHUNK; ++s2) {
+          if (base + s2 < neg_rate) {
+            const int64_t t = uniform_below(words[s2], static_cast<uint32_t>(n));
 #pragma unroll
-      for (int s2 = 0; s2 < NEG_CHUNK; ++s2) {
-        if (base + s2 < neg_rate) {
-          float nd[D];
-          float n2 = 0.f;
-#pragma unroll
-          for (int q = 0; q < D; ++q) {
-            nd[q] = yi[q] - yt[s2][q];
-            n2 += nd[q] * nd[q];
+            for (int q = 0; q < D; ++q) yt[u][s2][q] = Y[t * D + q];
           }
-          const float cr = law.repulse(n2);
-#pragma unroll
-          for (int q = 0; q < D; ++q) acc[q] += clip4(cr * nd[q]) * law.alpha;
         }
       }
     }
-    float diff[D];
-    float d2 = 0.f;
 #pragma unroll
-    for (int q = 0; q < D; ++q) {
-      diff[q] = yi[q] - yj[q];
-      d2 += diff[q] * diff[q];
-    }
-    const float ca = law.attract(d2);
+    for (int u = 0; u < S; ++u) {
+      if (act[u]) {
 #pragma unroll
-    for (int q = 0; q < D; ++q) {
-      const float g = clip4(ca * diff[q]) * law.alpha;
-      acc[q] += g;
-      gt[q] = -g;
+        for (int s2 = 0; s2 < NEG_CHUNK; ++s2) {
+          if (base + s2 < neg_rate) {
+            float nd[D];
+            float n2 = 0.f;
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+              nd[q] = yi[q] - yt[u][s2][q];
+              n2 += nd[q] * nd[q];
+            }
+            const float cr = law.repulse(n2);
+#pragma unroll
+            for (int q = 0; q < D; ++q) acc[q] += clip4(cr * nd[q]) * law.alpha;
+          }
+        }
+      }
     }
   }
-  if (in_row) {
-    // in-edge order (pos_in set): the slot's entry in the tail-sorted list (< 0: pruned)
-    const int64_t pos = pos_in ? pos_in[e] : e;
-    if (pos >= 0) {
 #pragma unroll
-      for (int q = 0; q < D; ++q) gtail[static_cast<int64_t>(pos) * D + q] = gt[q];
+  for (int u = 0; u < S; ++u) {
+    float gt[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) gt[q] = 0.f;
+    if (act[u]) {
+      float diff[D];
+      float d2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        diff[q] = yi[q] - yj[u][q];
+        d2 += diff[q] * diff[q];
+      }
+      const float ca = law.attract(d2);
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        const float g = clip4(ca * diff[q]) * law.alpha;
+        acc[q] += g;
+        gt[q] = -g;
+      }
+    }
+    // pos < 0: a pruned slot (in-edge order)
+    if (pos[u] >= 0 && (act[u] || (in_row[u] && !pos_in))) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) gtail[pos[u] * D + q] = gt[q];
     }
   }
   // Reduce the head's updates over its G lanes (contiguous in the wavefront).
@@ -306,35 +342,70 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) acc[q] += __shfl_xor(acc[q], m);
   }
+  // Row owner: the next positions start as Y + head update. A node whose
+  // in-edge entries cross a wavefront boundary of phase 2 (span[i] = index of
+  // the wavefront holding its first entry) starts at 0 instead, and Y + head
+  // update goes to wbase[span[i]] for that wavefront to add with its sum; a
+  // frozen node keeps Y.
   if (c == 0 && i < n) {
+    const int32_t sp = span[i];
+    if (movable && !movable[i]) {
 #pragma unroll
-    for (int q = 0; q < D; ++q) dHead[i * D + q] = acc[q];
+      for (int q = 0; q < D; ++q) Ynext[i * D + q] = yi[q];
+    } else if (sp >= 0) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        wbase[static_cast<int64_t>(sp) * D + q] = yi[q] + acc[q];
+        Ynext[i * D + q] = 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < D; ++q) Ynext[i * D + q] = yi[q] + acc[q];
+    }
   }
 }
 
-// Phase 2: tail sums. Phase 1 stored the tail gradients in in-edge order
-// (entries of a node contiguous, in_node[t] the node of entry t), read
-// contiguously, or, for buffers larger than the caches, in slot order,
-// gathered through in_slots (scattered 8-byte writes cost more than scattered
-// reads once the lines are not cache resident); a wavefront-segmented inclusive scan adds each node's
-// entries and the last lane of a segment adds the segment's total to dTail
-// with one atomic (a hub with m in-edges costs about m / 64 atomics). A node
-// is split over at most two wavefronts unless it has more than 64 in-edges, so
-// its total does not depend on the order of the atomics (two terms commute):
-// runs with the same seed repeat.
+// Phase 2: tail sums, applied. Phase 1 stored the tail gradients in in-edge
+// order (entries of a node contiguous, in_node[t] the node of entry t), read
+// contiguously and cleared, or, for buffers larger than the caches, in slot
+// order, gathered through in_slots (scattered 8-byte writes cost more than
+// scattered reads once the lines are not cache resident). A
+// wavefront-segmented inclusive scan adds each node's entries, and the last
+// lane of a segment adds the sum to Ynext with one atomic, onto the value
+// phase 1 left there: Y + head update for a node whose entries lie in one
+// wavefront (one term onto a fixed value), 0 for a node crossing a wavefront
+// boundary, whose first segment adds wbase[w] (its Y + head update, indexed
+// by the wavefront) with its sum. Over two wavefronts the result does not
+// depend on the order of the two atomics (two terms onto zero commute); only a
+// node with more than 64 in-edges can span more. Every load the wavefront
+// needs is independent of the others, so it waits for one memory latency.
+// Frozen nodes (movable[j] == 0) are skipped.
 template <int D>
-__global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtail, const int32_t* __restrict__ in_slots,
-                                                  const int32_t* __restrict__ in_node, int64_t m,
-                                                  float* __restrict__ dTail) {
+__global__ __launch_bounds__(256) void k_tail_apply(float* __restrict__ gtail, const int32_t* __restrict__ in_slots,
+                                                    const int32_t* __restrict__ in_node, int64_t m,
+                                                    float* __restrict__ Ynext, const float* __restrict__ wbase,
+                                                    const uint8_t* __restrict__ movable) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int l = threadIdx.x & 63;
+  const int64_t w0 = t - l;  // first entry of this wavefront
   const bool valid = t < m;
   const int node = valid ? in_node[t] : -1;
+  // the node before the wavefront (lane 0) and after it (lane 63)
+  const int before = (l == 0 && w0 > 0 && valid) ? in_node[w0 - 1] : -1;
+  const int after = (l == 63 && t + 1 < m) ? in_node[t + 1] : -1;
+  float base[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) base[q] = l == 63 && valid ? wbase[(w0 >> 6) * D + q] : 0.f;
   float v[D];
   // in_slots == nullptr: stored in in-edge order; else in slot order, gathered
   const int64_t src = valid ? (in_slots ? static_cast<int64_t>(in_slots[t]) : t) : 0;
 #pragma unroll
   for (int q = 0; q < D; ++q) v[q] = valid ? gtail[src * D + q] : 0.f;
+  // in-edge order: phase 1 stores only active slots, so leave the entries cleared
+  if (valid && !in_slots) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) gtail[src * D + q] = 0.f;
+  }
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const int on = __shfl_up(node, off);
@@ -345,23 +416,32 @@ __global__ __launch_bounds__(256) void k_tail_sum(const float* __restrict__ gtai
     }
   }
   const int next = __shfl_down(node, 1);
+  const int first_node = __shfl(node, 0);
+  const int before_node = __shfl(before, 0);
+  const int after_node = __shfl(after, 63);
   const bool last = valid && (l == 63 || next != node);
-  if (last) {
+  if (last && (!movable || movable[node])) {
+    const bool from_before = first_node == node && before_node == node;
+    const bool to_after = l == 63 && after_node == node;
+    if (to_after && !from_before) {
+      // first segment of a node crossing into the next wavefront
 #pragma unroll
-    for (int q = 0; q < D; ++q) atomicAdd(&dTail[static_cast<int64_t>(node) * D + q], v[q]);
+      for (int q = 0; q < D; ++q) v[q] += base[q];
+    }
+#pragma unroll
+    for (int q = 0; q < D; ++q) atomicAdd(&Ynext[static_cast<int64_t>(node) * D + q], v[q]);
   }
 }
 
-// Phase 3: Y += dHead + dTail, then dTail = 0. With `movable` set, rows whose
-// flag is 0 keep their position (their gradients are dropped).
-__global__ __launch_bounds__(256) void k_apply2(float* __restrict__ Y, const float* __restrict__ dHead,
-                                                float* __restrict__ dTail, int64_t count, int dim,
-                                                const uint8_t* __restrict__ movable) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t < count) {
-    if (!movable || movable[t / dim]) Y[t] += dHead[t] + dTail[t];
-    dTail[t] = 0.f;
-  }
+// span[j] = index of the wavefront (64 in-edge entries) holding node j's first
+// entry if its entries cross a 64-aligned boundary, else -1 (span pre-filled
+// with -1, i.e. 0xFFFFFFFF as unsigned, so the unsigned minimum keeps the
+// first boundary's wavefront).
+__global__ __launch_bounds__(256) void k_span(const int32_t* __restrict__ in_node, int64_t m,
+                                              int32_t* __restrict__ span) {
+  const int64_t t = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x + 1) * 64;
+  if (t < m && in_node[t - 1] == in_node[t])
+    atomicMin(reinterpret_cast<uint32_t*>(&span[in_node[t]]), static_cast<uint32_t>((t >> 6) - 1));
 }
 
 // pos_in[e] = t for the in-edge list entry t of slot e (pos_in pre-filled with -1).
@@ -711,60 +791,65 @@ static size_t spectral_ws(int64_t n, int k, int nc) {
 
 struct EpochBufs {
   float* gtail;     // n*k*dim, in in-edge order
-  float* dHead;     // n*dim
-  float* dTail;     // n*dim
+  float* wbase;     // ceil(n*k/64)*dim: Y + head update of the node crossing out of each wavefront
+  float* ybuf;      // n*dim, the other half of the double-buffered positions
   int32_t* pos_in;  // n*k: in-edge position of each slot (-1: pruned); nullptr: slot order
+  int32_t* span;    // n: wavefront of the node's first in-edge entry if its entries cross a boundary, else -1
 };
 
 // Tail gradients in in-edge order while their buffer stays cache resident.
 constexpr size_t GTAIL_INEDGE_MAX = size_t(32) << 20;
 
 static size_t epochs_ws(int64_t n, int k, int dim) {
-  return align_up(static_cast<size_t>(n) * k * dim * 4, 256) + 2 * align_up(static_cast<size_t>(n) * dim * 4, 256) +
-         align_up(static_cast<size_t>(n) * k * 4, 256) + 256;
+  return align_up(static_cast<size_t>(n) * k * dim * 4, 256) +
+         align_up(static_cast<size_t>(ceil_div(n * k, 64)) * dim * 4, 256) +
+         align_up(static_cast<size_t>(n) * dim * 4, 256) + align_up(static_cast<size_t>(n) * k * 4, 256) +
+         align_up(static_cast<size_t>(n) * 4, 256) + 256;
 }
 
 static EpochBufs carve_epochs(void* ws, int64_t n, int k, int dim) {
   Carve c(ws);
   EpochBufs b;
   b.gtail = c.take<float>(static_cast<size_t>(n) * k * dim);
-  b.dHead = c.take<float>(static_cast<size_t>(n) * dim);
-  b.dTail = c.take<float>(static_cast<size_t>(n) * dim);
+  b.wbase = c.take<float>(static_cast<size_t>(ceil_div(n * k, 64)) * dim);
+  b.ybuf = c.take<float>(static_cast<size_t>(n) * dim);
   b.pos_in = c.take<int32_t>(static_cast<size_t>(n) * k);
+  b.span = c.take<int32_t>(static_cast<size_t>(n));
   if (static_cast<size_t>(n) * k * dim * 4 > GTAIL_INEDGE_MAX) b.pos_in = nullptr;
   return b;
 }
 
-template <int D, int G>
-static int launch_epoch(const float* Y, const EpochBufs& B, const int32_t* tail, const float* eps, float* epn,
-                        int64_t n, int k, int epoch, int neg_rate, Law law, uint64_t seed, hipStream_t s) {
+template <int D, int G, int S>
+static int launch_epoch(const float* Y, float* Ynext, const EpochBufs& B, const uint8_t* movable,
+                        const int32_t* tail, const float* eps, float* epn, int64_t n, int k, int epoch, int neg_rate,
+                        Law law, uint64_t seed, hipStream_t s) {
   const int64_t threads = n * G;
   const unsigned blocks = static_cast<unsigned>(ceil_div(threads, 256));
-  hipLaunchKernelGGL((k_epoch<D, G>), dim3(blocks), dim3(256), 0, s, Y, B.gtail, B.pos_in, B.dHead, tail, eps, epn,
-                     n, k,
-                     epoch, neg_rate, law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  hipLaunchKernelGGL((k_epoch<D, G, S>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.gtail, B.pos_in, B.wbase, B.span,
+                     movable, tail, eps, epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed),
+                     static_cast<uint32_t>(seed >> 32));
   MV_LAUNCHED("k_epoch");
   return MV_OK;
 }
 
+// One epoch from positions Y into Ynext (two launches: edge gradients, then
+// tail sums applied).
 template <int D>
-static int run_epoch(float* Y, const EpochBufs& B, const int32_t* tail, const float* eps, float* epn,
-                     const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int64_t n, int k, int epoch,
-                     int neg_rate, Law law, uint64_t seed, const uint8_t* movable, hipStream_t s) {
+static int run_epoch(const float* Y, float* Ynext, const EpochBufs& B, const int32_t* tail, const float* eps,
+                     float* epn, const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int64_t n, int k,
+                     int epoch, int neg_rate, Law law, uint64_t seed, const uint8_t* movable, hipStream_t s) {
   int rc;
-  if (k <= 8) rc = launch_epoch<D, 8>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
-  else if (k <= 16) rc = launch_epoch<D, 16>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
-  else if (k <= 32) rc = launch_epoch<D, 32>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
-  else rc = launch_epoch<D, 64>(Y, B, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  // two slots per lane from k = 9 on (G lanes per row, G * S >= k)
+  if (k <= 8) rc = launch_epoch<D, 8, 1>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else if (k <= 16) rc = launch_epoch<D, 8, 2>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else if (k <= 32) rc = launch_epoch<D, 16, 2>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else rc = launch_epoch<D, 32, 2>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
   if (rc) return rc;
   if (n_in > 0) {
-    hipLaunchKernelGGL((k_tail_sum<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, B.gtail,
-                       B.pos_in ? nullptr : in_slots, in_node, n_in, B.dTail);
-    MV_LAUNCHED("k_tail_sum");
+    hipLaunchKernelGGL((k_tail_apply<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, B.gtail,
+                       B.pos_in ? nullptr : in_slots, in_node, n_in, Ynext, B.wbase, movable);
+    MV_LAUNCHED("k_tail_apply");
   }
-  hipLaunchKernelGGL(k_apply2, dim3(static_cast<unsigned>(ceil_div(n * D, 256))), dim3(256), 0, s, Y, B.dHead,
-                     B.dTail, n * D, D, movable);
-  MV_LAUNCHED("k_apply2");
   return MV_OK;
 }
 
@@ -1002,19 +1087,30 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
   MV_FAULT_POINT("launch");
   hipStream_t s = static_cast<hipStream_t>(stream);
   EpochBufs B = carve_epochs(ws, n, k, dim);
-  hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(n * dim, 256))), dim3(256), 0, s, B.dTail,
-                     n * dim);
-  MV_LAUNCHED("k_zero_f32");
   if (B.pos_in) {
     hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>(ceil_div(n * k, 256))), dim3(256), 0, s, B.pos_in, n * k,
                        -1);
     MV_LAUNCHED("k_fill_i32");
+    // in-edge order: k_epoch stores only active slots, k_tail_apply clears what it reads
+    hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(std::max<int64_t>(n_in, 1) * dim, 256))),
+                       dim3(256), 0, s, B.gtail, n_in * dim);
+    MV_LAUNCHED("k_zero_f32");
   }
   if (B.pos_in && n_in > 0) {
     hipLaunchKernelGGL(k_pos_in, dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, in_slots, n_in,
                        B.pos_in);
     MV_LAUNCHED("k_pos_in");
   }
+  hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, B.span, n, -1);
+  MV_LAUNCHED("k_fill_i32");
+  if (n_in > 64) {
+    hipLaunchKernelGGL(k_span, dim3(static_cast<unsigned>(ceil_div(ceil_div(n_in, 64), 256))), dim3(256), 0, s, in_node,
+                       n_in, B.span);
+    MV_LAUNCHED("k_span");
+  }
+  // positions alternate between Y and ybuf; an odd number of epochs ends in ybuf
+  float* cur = Y;
+  float* nxt = B.ybuf;
   for (int ep = e0; ep < e1; ++ep) {
     Law law;
     law.a = a;
@@ -1022,12 +1118,20 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
     law.alpha = static_cast<float>(static_cast<double>(lr) * (1.0 - static_cast<double>(ep) / n_epochs));
     int rc = MV_OK;
     switch (dim) {
-      case 1: rc = run_epoch<1>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 2: rc = run_epoch<2>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 3: rc = run_epoch<3>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      default: rc = run_epoch<4>(Y, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 1: rc = run_epoch<1>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 2: rc = run_epoch<2>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 3: rc = run_epoch<3>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      default: rc = run_epoch<4>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
     }
     if (rc != MV_OK) return rc;
+    float* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (cur != Y) {
+    hipLaunchKernelGGL(k_copy_f32, dim3(static_cast<unsigned>(ceil_div(n * dim, 256))), dim3(256), 0, s, Y, cur,
+                       n * dim);
+    MV_LAUNCHED("k_copy_f32");
   }
   return MV_OK;
 }

@@ -83,6 +83,64 @@ def test_single_sgd_step_matches_fp64_oracle(dim):
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("n,k,dim,hubs,masked", [(5000, 15, 2, True, False), (3000, 10, 3, False, True),
+                                                   (300_000, 15, 2, True, True)])
+def test_epochs_without_negatives_match_oracle(n, k, dim, hubs, masked):
+    """With negative_sample_rate 0 a layout epoch is the attraction step of
+    the epoch's active edges from one snapshot, so several epochs can be
+    checked against the oracle's sgd_step (FP64), schedule included. Covers
+    tail nodes with more than 64 in-edges (summed over several wavefronts),
+    nodes without in-edges, pruned slots, frozen nodes, and at 300,000 x 15
+    the slot-order tail buffer used beyond the cache-resident size."""
+    from mlx_vis_amd import find_ab_params
+    from mlx_vis_amd._umap import FuzzyGraph, run_epochs
+
+    rng = np.random.default_rng(n + k)
+    rows = np.repeat(np.arange(n), k)
+    tails = ((rows + 1 + rng.integers(0, n - 1, n * k)) % n).reshape(n, k)
+    if hubs:
+        pick = rng.uniform(size=n) < 0.3
+        tails[pick, 0] = rng.integers(0, 5, pick.sum())        # five hubs, ~0.06 n in-edges each
+    tails[np.arange(n) % 7 == 0] %= max(1, n // 2)             # the upper half gets fewer in-edges
+    tails = tails.astype(np.int32)
+    eps = rng.uniform(1.0, 4.0, (n, k)).astype(np.float32)
+    eps[rng.uniform(size=(n, k)) < 0.1] = -1.0                  # pruned slots
+    kept = np.flatnonzero(eps.ravel() > 0)
+    order = np.argsort(tails.ravel()[kept], kind="stable")
+    in_slots = kept[order].astype(np.int32)
+    in_node = tails.ravel()[in_slots].astype(np.int32)
+    epn0 = np.where(eps > 0, eps, np.float32(np.inf)).astype(np.float32)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    g = FuzzyGraph(dev(tails), None, dev(eps), dev(epn0.copy()), len(in_slots), 1.0,
+                   in_slots=dev(in_slots), in_node=dev(in_node))
+    a, b = find_ab_params(1.0, 0.1)
+    Y0 = rng.uniform(-1, 1, (n, dim)).astype(np.float32)
+    frozen = (np.arange(n) % 5 == 2) if masked else np.zeros(n, dtype=bool)
+    Y = dev(Y0.copy())
+    n_epochs, lr, e0, e1 = 200, 0.01, 1, 5
+    run_epochs(Y, None, g, e0, e1, n_epochs, 0, lr, a, b, 7,
+               movable=dev((~frozen).astype(np.uint8)) if masked else None)
+    Yo = Y0.astype(np.float64)
+    epn = epn0.ravel().copy()
+    none = np.empty(0, dtype=np.int64)
+    for ep in range(e0, e1):
+        alpha = np.float32(lr * (1.0 - ep / n_epochs))
+        act = np.flatnonzero(epn <= np.float32(ep))
+        epn[act] = epn[act] + eps.ravel()[act]
+        prev = Yo
+        Yo = ou.sgd_step(Yo, rows[act], tails.ravel()[act], none, none, alpha, a, b, dtype=np.float64)
+        Yo[frozen] = prev[frozen]
+    np.testing.assert_array_equal(g.epn.cpu().numpy().ravel(), epn)
+    got = Y.cpu().numpy()
+    np.testing.assert_array_equal(got[frozen], Y0[frozen])
+    # one attraction term is up to 4 alpha ~ 4e-2: a dropped or doubled term
+    # shows; the hubs sum ~0.06 n terms in FP32
+    hub = np.zeros(n, dtype=bool)
+    hub[:5] = hubs
+    np.testing.assert_allclose(got[~hub], Yo[~hub], rtol=0, atol=2e-5)
+    np.testing.assert_allclose(got[hub], Yo[hub], rtol=1e-4, atol=1e-3)
+
+
 def test_layout_quality_matches_oracle_run():
     """Same graph, a/b and init: 15-NN preservation and trustworthiness of the
     GPU layout within 0.01 of the oracle's epoch loop."""
