@@ -1,0 +1,34 @@
+"""The oracle and the host code against the committed golden fixtures of
+tests/golden (values from scikit-learn and SciPy, written by
+tests/golden/make_golden.py)."""
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+from oracle import knn as oknn
+from oracle import umap as ou
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def c0_input():
+    return np.random.default_rng(0).standard_normal((10000, 128), dtype=np.float32)
+
+
+def test_oracle_knn_matches_golden_c0_head():
+    g = np.load(GOLDEN / "knn_c0_head1000.npz")
+    X = c0_input()
+    oi, od = oknn.exact_knn(X[:1000], X, 15, self_base=0)
+    np.testing.assert_array_equal(oi, g["idx"])
+    np.testing.assert_allclose(od, g["dist2"], rtol=1e-9)
+
+
+def test_ab_params_match_golden():
+    from mlx_vis_amd import find_ab_params
+
+    for row in json.loads((GOLDEN / "umap_ab.json").read_text()):
+        for fn in (ou.find_ab_params, find_ab_params):
+            a, b = fn(row["spread"], row["min_dist"])
+            np.testing.assert_allclose([a, b], [row["a"], row["b"]], rtol=1e-5)
