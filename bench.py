@@ -5,13 +5,15 @@ Workload: X = 70,000 x 784 float32, a mixture of 10 isotropic Gaussians
 (centres N(0, 4^2) per dimension, sigma 1, equal weights, seed 1), min-max
 scaled to [0, 1], resident in HBM. One step = UMAP(n_neighbors=15,
 min_dist=0.1, n_epochs=500, negative_sample_rate=5, knn_method="brute")
-.fit_transform(X): exact brute-force kNN (k=15, FP32 MFMA screen + exact
-re-rank), fuzzy graph, spectral init, 500 epochs of layout SGD, and the copy of
-the embedding to the host.
+.fit_transform(X): exact brute-force kNN (k=15, FP32 MFMA screen of the
+upper-triangle distance tiles + per-row selection + exact re-rank), fuzzy
+graph, spectral init, 500 epochs of layout SGD, and the copy of the embedding
+to the host.
 
 Prints one JSON line. `value` is seconds per fit_transform (lower is better).
-`roofline` covers the dominant kernel (the MFMA distance screen of the kNN),
-timed with HIP events on its launch stream; `cpu_baseline` is the NumPy
+`roofline` covers the dominant kernel (the MFMA distance kernel of the kNN,
+k_dist_sym: 2 d FLOP per pair on and above the diagonal), timed with HIP
+events on its launch stream; `cpu_baseline` is the NumPy
 restatement of the reference (the oracle) on a bounded sample of the same
 workload, scaled to the whole job.
 """
@@ -32,17 +34,20 @@ for _p in (ROOT, ROOT / "mlx-vis_amd"):
 
 MFMA_FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-SCREEN_PROFILE = ROOT / "profiles" / "k_screen_f32_70000x784.md"
+SCREEN_PROFILES = {"k_screen_f32": ROOT / "profiles" / "k_screen_f32_70000x784.md",
+                   "k_dist_sym_f32": ROOT / "profiles" / "k_dist_sym_f32_70000x784.md"}
 
 
-def screen_traffic(n, d):
-    """HBM bytes per k_screen launch at 70,000 x 784, as measured by two
-    rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1, factors from a 1 GiB
-    copy) and recorded with their commands in the profile summary; None for
-    other shapes or when the summary is absent."""
-    if (n, d) != (70_000, 784) or not SCREEN_PROFILE.exists():
+def screen_traffic(n, d, kernel):
+    """HBM bytes per launch of the kNN distance kernel at 70,000 x 784, as
+    measured by two rocprofv3 PMC passes (FETCH_SIZE x 2 + WRITE_SIZE x 1,
+    factors from a 1 GiB copy) and recorded with their commands in the
+    kernel's profile summary; None for other shapes or when the summary is
+    absent."""
+    prof = SCREEN_PROFILES.get(kernel)
+    if (n, d) != (70_000, 784) or prof is None or not prof.exists():
         return None
-    for line in SCREEN_PROFILE.read_text().splitlines():
+    for line in prof.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch"):
             return int(float(line.split("=")[1]))
     return None
@@ -82,7 +87,8 @@ def knn_phase_timing(Xd, reps, dtype="fp32", rerank=2):
 
     lib = _capi.lib()
     lib.mv_knn_set_timing(1)
-    phases = []
+    phases, split = [], []
+    sym = ctypes.c_int32(0)
     stats = torch.zeros(4, dtype=torch.int32, device=Xd.device)
     try:
         for _ in range(reps):
@@ -90,11 +96,17 @@ def knn_phase_timing(Xd, reps, dtype="fp32", rerank=2):
             out = (ctypes.c_float * 4)()
             _capi.check(lib.mv_knn_last_phase_ms(out), "mv_knn_last_phase_ms")
             phases.append(list(out))
+            sp = (ctypes.c_float * 2)()
+            _capi.check(lib.mv_knn_last_screen_ms(sp, ctypes.byref(sym)), "mv_knn_last_screen_ms")
+            split.append(list(sp))
     finally:
         lib.mv_knn_set_timing(0)
     n = len(phases)
     avg = [sum(p[i] for p in phases) / n for i in range(4)]
-    return avg, stats.cpu().tolist(), idx
+    # distance kernel and row selection inside the screen phase
+    screen = {"symmetric": bool(sym.value), "kernel_ms": sum(p[0] for p in split) / n,
+              "select_ms": sum(p[1] for p in split) / n}
+    return avg, stats.cpu().tolist(), idx, screen
 
 
 def knn_preservation_gpu(knn_hi, Y):
@@ -361,9 +373,9 @@ def main():
     torch.cuda.synchronize()
 
     if args.only == "knn":
-        phases, stats, _ = knn_phase_timing(Xd, args.knn_reps, args.dtype)
-        print(json.dumps({"knn_dtype": args.dtype, "ms_phases": phases, "stats": stats,
-                          "tflops_screen": 2.0 * n * n * d / (phases[1] * 1e-3) / 1e12}), flush=True)
+        phases, stats, _, scr = knn_phase_timing(Xd, args.knn_reps, args.dtype)
+        print(json.dumps({"knn_dtype": args.dtype, "ms_phases": phases, "stats": stats, "screen": scr,
+                          "tflops_equiv_full_matrix": 2.0 * n * n * d / (phases[1] * 1e-3) / 1e12}), flush=True)
         return
 
     if args.only == "scale":
@@ -421,10 +433,14 @@ def main():
     ph["epochs"] = time.perf_counter() - t
 
     log("kNN phase timing")
-    phases, stats, ref_idx = knn_phase_timing(Xd, args.knn_reps)
+    phases, stats, ref_idx, scr = knn_phase_timing(Xd, args.knn_reps)
     screen_ms = phases[1]
     flops = 2.0 * n * n * d
-    screen_tflops = flops / (screen_ms * 1e-3) / 1e12
+    # the dominant kernel: the symmetric distance kernel multiplies the n(n+1)/2
+    # pairs on and above the diagonal (the fused screen: all n^2)
+    kern = "k_dist_sym_f32" if scr["symmetric"] else "k_screen_f32"
+    kern_flops = 2.0 * d * n * (n + 1) / 2 if scr["symmetric"] else flops
+    screen_tflops = kern_flops / (scr["kernel_ms"] * 1e-3) / 1e12
     knn_total_ms = sum(phases)
     quality = knn_preservation_gpu(um.knn_[0], Y)
 
@@ -444,22 +460,28 @@ def main():
         "config": {"workload": f"UMAP fit_transform {n}x{d}, exact brute kNN k=15 (FP32), spectral init, "
                                f"{args.epochs} epochs, 5 negatives", "n": n, "d": d, "k": 15,
                    "n_epochs": args.epochs, "knn_method": "brute", "knn_dtype": "fp32"},
-        "knn": {"tflops_screen": round(screen_tflops, 2), "ms_total": round(knn_total_ms, 3),
+        "knn": {"screen_kernel": kern, "tflops_screen_kernel": round(screen_tflops, 2),
+                "tflops_equiv_full_matrix": round(flops / (screen_ms * 1e-3) / 1e12, 2),
+                "ms_total": round(knn_total_ms, 3),
                 "ms_phases": {"prepass": round(phases[0], 3), "screen": round(phases[1], 3),
                               "rerank": round(phases[2], 3), "exact_scan": round(phases[3], 3)},
+                "ms_screen_split": {"distance_kernel": round(scr["kernel_ms"], 3),
+                                    "row_select": round(scr["select_ms"], 3)},
+                "row_select_gbs": (round(4.0 * ((n + 127) // 128 * 128) ** 2 / (scr["select_ms"] * 1e-3) / 1e9, 1)
+                                   if scr["symmetric"] and scr["select_ms"] > 0 else None),
                 "rows_fp64_scan": stats[0], "rows_fp64_tiecheck": stats[1],
                 "tflops_whole_knn": round(flops / (knn_total_ms * 1e-3) / 1e12, 2)},
         "phases_s": {k: round(v, 4) for k, v in ph.items()},
         "quality": {"knn15_preservation": round(quality, 4), "n_edges": g.n_edges},
-        "roofline": {"kernel": "k_screen_f32", "bound": "mfma", "achieved": round(screen_tflops, 2),
+        "roofline": {"kernel": kern, "bound": "mfma", "achieved": round(screen_tflops, 2),
                      "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": screen_traffic(n, d)},
+                     "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": screen_traffic(n, d, kern)},
     }
     if not args.no_extras:
         prec = {}
         for dt, rr in (("bf16x3", 2), ("bf16", 2), ("bf16", 4), ("fp8", 2), ("fp8", 4)):
             log(f"kNN {dt} screen, rerank {rr}")
-            ph_dt, st_dt, idx_dt = knn_phase_timing(Xd, args.knn_reps, dt, rr)
+            ph_dt, st_dt, idx_dt, _ = knn_phase_timing(Xd, args.knn_reps, dt, rr)
             tot = sum(ph_dt)
             prec[f"{dt}_c{rr * 15}"] = {"ms_total": round(tot, 3), "ms_phases": [round(v, 3) for v in ph_dt],
                         "speedup_vs_fp32": round(knn_total_ms / tot, 3),

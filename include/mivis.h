@@ -93,6 +93,21 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
  * exact scan] to the host array out4. */
 int mv_knn_set_timing(int on);
 int mv_knn_last_phase_ms(float* out4);
+/* The screen phase of the last timed call split by kernel: out2[0] = the
+ * distance kernel (k_dist_sym for the symmetric screen, else k_screen),
+ * out2[1] = the row selection (k_select; 0 for the fused screen), in ms;
+ * *out_sym = 1 if the symmetric screen ran. */
+int mv_knn_last_screen_ms(float* out2, int32_t* out_sym);
+
+/* Symmetric screen for self kNN (Q == X, every dtype, 4,096 <= n and
+ * n_pad^2 * 4 bytes <= 48 GiB; on by default): only the tiles
+ * on and above the diagonal of the distance matrix are multiplied, the full
+ * matrix is kept in the workspace (mv_knn_ws_bytes includes it) and each row's
+ * candidate list is selected from it; the returned neighbours and distances
+ * are those of the fused screen.
+ * Off: every self kNN uses the fused screen. Process-wide; set it before
+ * mv_knn_ws_bytes and keep it until the matching mv_knn_brute has run. */
+int mv_knn_set_symmetric(int on);
 
 /* Row squared norms: out[i] = sum_t X[i,t]^2 in FP32 (mx.sum(X*X, 1) at
  * mlx_vis/_knn.py:49 and mlx_vis/_nndescent/nndescent.py:51,77). */

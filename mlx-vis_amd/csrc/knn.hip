@@ -404,6 +404,93 @@ __device__ __forceinline__ void mfma_chunk(const float* As, const float* Bs, int
   }
 }
 
+// mfma_chunk for a wavefront tile of 64 rows x 16*NI columns (4 x NI
+// accumulators; the columns start at wc * 16 * NI).
+template <int DT, int NI>
+__device__ __forceinline__ void mfma_chunk_n(const float* As, const float* Bs, int wr, int wc, int l,
+                                             v4f (&acc)[4][NI]) {
+  if constexpr (DT == DT_F32) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      v4f af[4], bf[NI];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+        af[mi] = *reinterpret_cast<const v4f*>(As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4);
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni)
+        bf[ni] = *reinterpret_cast<const v4f*>(Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < NI; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mi][j], bf[ni][j], acc[mi][ni], 0, 0, 0);
+    }
+  } else if constexpr (DT == DT_BF16X3) {
+    // one K group of 32: hi in floats [0, 16) of the row, lo in [16, 32)
+    v8bf ah[4], al[4], bh[NI], bl[NI];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const float* p = As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + (l >> 4) * 4;
+      ah[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p));
+      al[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p + 16));
+    }
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const float* p = Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + (l >> 4) * 4;
+      bh[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p));
+      bl[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p + 16));
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) {
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+      }
+  } else if constexpr (DT == DT_BF16) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      v8bf af[4], bf[NI];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+        af[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(
+                                              As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4));
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni)
+        bf[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(
+                                              Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4));
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
+    }
+  } else {
+    v8i af[4], bf[NI];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const float* p = As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + (l >> 4) * 8;
+      const v4f lo = *reinterpret_cast<const v4f*>(p), hi = *reinterpret_cast<const v4f*>(p + 4);
+      af[mi] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+    }
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const float* p = Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + (l >> 4) * 8;
+      const v4f lo = *reinterpret_cast<const v4f*>(p), hi = *reinterpret_cast<const v4f*>(p + 4);
+      bf[ni] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[mi], bf[ni], acc[mi][ni], 0, 0, 0,
+                                                                        FP8_UNIT_SCALE, 0, FP8_UNIT_SCALE);
+  }
+}
+
 template <int DT>
 __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
@@ -663,6 +750,296 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
 #undef MV_GLOAD
 #undef MV_LSTORE
 
+}
+
+// ------------------------------------------------------- symmetric screen --
+//
+// Self kNN (Q = X) of up to SYM_MAX_BYTES of distance matrix: D is symmetric,
+// so only the tiles (I, J) with J >= I are multiplied (half the MFMA work of
+// the screen above), each stored twice (D[I, J] and its transpose D[J, I]) to
+// an n_pad x n_pad FP32 matrix in HBM; k_select then reads each row once and
+// keeps its C smallest keys. The screen values are the screen's own
+// expression (|x_i|^2 + |x_j|^2 - 2 x_i.x_j, clamped at 0, same K order), so
+// the candidate lists, and everything downstream of them, are those of the
+// fused screen.
+constexpr size_t SYM_MAX_BYTES = size_t(48) << 30;
+constexpr int64_t SYM_MIN_N = 4096;
+
+struct SymArgs {
+  const char* Xq;      // [n_pad, d_pad] centred data (FP32 or split BF16)
+  const float* sqx;    // [n_pad]
+  const float* invx;   // FP8: per-row 1/scale (else unused)
+  int64_t row_bytes;
+  int nb;              // row blocks (n_pad / 128)
+  int nsb;             // super-blocks of SYM_G x SYM_G tiles per dimension (ceil(nb / SYM_G))
+  int64_t n_slots;     // nsb (nsb + 1) / 2 * SYM_G^2 tile slots (some empty)
+  int64_t per_xcd;     // slots per XCD (ceil(n_slots / 8))
+  int64_t ldd;         // n_pad
+  float* D;            // [n_pad, n_pad]
+};
+
+constexpr int SYM_G = 8;   // tiles per super-block side
+
+// Tile t of the upper triangle in row-major order: row block I holds tiles
+// start(I) .. start(I) + nb - I - 1 with start(I) = I nb - I (I - 1) / 2.
+__device__ __forceinline__ void sym_tile(int64_t t, int nb, int& I, int& J) {
+  const double b = 2.0 * nb + 1.0;
+  int i = static_cast<int>((b - sqrt(b * b - 8.0 * static_cast<double>(t))) * 0.5);
+  if (i < 0) i = 0;
+  if (i > nb - 1) i = nb - 1;
+  auto start = [nb](int64_t r) { return r * nb - r * (r - 1) / 2; };
+  while (i > 0 && start(i) > t) --i;
+  while (i + 1 < nb && start(i + 1) <= t) ++i;
+  I = i;
+  J = i + static_cast<int>(t - start(i));
+}
+
+// Workgroup of W wavefronts: W = 4 gives 64 x 64 wavefront tiles (2 x 2),
+// W = 8 gives 64 x 32 tiles (2 x 4) with half the registers, so that four
+// wavefronts share each SIMD.
+#ifndef MV_SYM_WAVES
+#define MV_SYM_WAVES 8
+#endif
+constexpr int SYM_W = MV_SYM_WAVES;
+
+template <int DT, int W>
+__global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
+  constexpr int NTS = 64 * W;
+  constexpr int NI = W == 4 ? 4 : 2;          // 16-column accumulator blocks per wavefront
+  constexpr int WCOLS = W == 4 ? 2 : 4;       // wavefront columns
+  constexpr int NQ = 128 / (NTS / 8);         // staged rows per thread and operand
+  // two register sets (a chunk loaded two steps before it is stored to LDS)
+  // where the multiplies of a chunk are short and registers allow
+  constexpr bool PF2 = DT != DT_F32 && W == 4;
+  __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
+  // Workgroups are dispatched round-robin over the 8 XCDs (bid mod 8); XCD x
+  // takes a contiguous run of tile slots. Slots are ordered by super-block
+  // (SYM_G x SYM_G tiles, upper triangle of super-blocks in row-major order),
+  // so the ~64 workgroups resident on an XCD work on one super-block: its
+  // SYM_G row blocks and SYM_G column blocks are each read by SYM_G
+  // workgroups at about the same K step, and all but one read hit the XCD's
+  // L2 (a row-major tile order reads every column block from beyond the L2).
+  const int bid = blockIdx.x;
+  const int64_t u = static_cast<int64_t>(bid & 7) * a.per_xcd + (bid >> 3);
+  if (u >= a.n_slots) return;
+  int SI, SJ;
+  sym_tile(u / (SYM_G * SYM_G), a.nsb, SI, SJ);
+  const int I = SI * SYM_G + static_cast<int>(u % (SYM_G * SYM_G)) / SYM_G;
+  const int J = SJ * SYM_G + static_cast<int>(u % SYM_G);
+  if (I >= a.nb || J >= a.nb || J < I) return;
+
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int wr = w / WCOLS, wc = w % WCOLS;
+  const int64_t rbytes = a.row_bytes;
+  const char* Ab = a.Xq + static_cast<int64_t>(I) * BM * rbytes;
+  const char* Bb = a.Xq + static_cast<int64_t>(J) * BN * rbytes;
+  const int lr = tid >> 3;            // staged row (plus NTS / 8 * q)
+  const int lkk = (tid & 7) * 4;
+  const int nk = static_cast<int>(rbytes / 128);
+
+  float4 ra0_0, ra1_0, ra2_0, ra3_0, rb0_0, rb1_0, rb2_0, rb3_0;
+  float4 ra0_1, ra1_1, ra2_1, ra3_1, rb0_1, rb1_1, rb2_1, rb3_1;
+#define MV_SGLOAD(SET, KCC)                                                                 \
+  do {                                                                                    \
+    const int64_t ko_ = static_cast<int64_t>(KCC) * 128 + lkk * 4;                        \
+    constexpr int st_ = NTS / 8;                                                          \
+    ra0_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr) * rbytes + ko_);            \
+    ra1_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + st_) * rbytes + ko_);      \
+    rb0_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr) * rbytes + ko_);            \
+    rb1_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + st_) * rbytes + ko_);      \
+    if constexpr (NQ == 4) {                                                              \
+      ra2_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 2 * st_) * rbytes + ko_); \
+      ra3_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 3 * st_) * rbytes + ko_); \
+      rb2_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 2 * st_) * rbytes + ko_); \
+      rb3_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 3 * st_) * rbytes + ko_); \
+    }                                                                                     \
+  } while (0)
+#define MV_SLSTORE(SET, BUF)                                                                \
+  do {                                                                                    \
+    constexpr int st_ = NTS / 8;                                                          \
+    float* As2_ = s_stage + (BUF) * (2 * BM * LDSA);                                      \
+    float* Bs2_ = As2_ + BM * LDSA;                                                       \
+    *reinterpret_cast<float4*>(As2_ + (lr) * LDSA + lkk) = ra0_##SET;                     \
+    *reinterpret_cast<float4*>(As2_ + (lr + st_) * LDSA + lkk) = ra1_##SET;               \
+    *reinterpret_cast<float4*>(Bs2_ + (lr) * LDSA + lkk) = rb0_##SET;                     \
+    *reinterpret_cast<float4*>(Bs2_ + (lr + st_) * LDSA + lkk) = rb1_##SET;               \
+    if constexpr (NQ == 4) {                                                              \
+      *reinterpret_cast<float4*>(As2_ + (lr + 2 * st_) * LDSA + lkk) = ra2_##SET;         \
+      *reinterpret_cast<float4*>(As2_ + (lr + 3 * st_) * LDSA + lkk) = ra3_##SET;         \
+      *reinterpret_cast<float4*>(Bs2_ + (lr + 2 * st_) * LDSA + lkk) = rb2_##SET;         \
+      *reinterpret_cast<float4*>(Bs2_ + (lr + 3 * st_) * LDSA + lkk) = rb3_##SET;         \
+    }                                                                                     \
+  } while (0)
+
+  v4f acc[4][NI];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  float* L0 = s_stage;
+  float* L1 = s_stage + 2 * BM * LDSA;
+  if constexpr (!PF2) {
+    MV_SGLOAD(0, 0);
+    MV_SLSTORE(0, 0);
+    __syncthreads();
+    for (int kc = 0; kc < nk; ++kc) {
+      const int cur = kc & 1;
+      const bool last = kc + 1 == nk;
+      if (!last) MV_SGLOAD(0, kc + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_chunk_n<DT, NI>(cur ? L1 : L0, (cur ? L1 : L0) + BM * LDSA, wr, wc, l, acc);
+      if (!last) MV_SLSTORE(0, cur ^ 1);
+      __syncthreads();
+    }
+  } else {
+    MV_SGLOAD(0, 0);
+    MV_SLSTORE(0, 0);
+    if (nk > 1) MV_SGLOAD(1, 1);
+    if (nk > 2) MV_SGLOAD(0, 2);
+    __syncthreads();
+    for (int kc = 0; kc < nk; kc += 2) {
+      // chunk kc in L0, kc + 1 in set 1, kc + 2 in set 0 (in flight)
+      if (kc + 1 < nk) MV_SLSTORE(1, 1);
+      if (kc + 3 < nk) MV_SGLOAD(1, kc + 3);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_chunk_n<DT, NI>(L0, L0 + BM * LDSA, wr, wc, l, acc);
+      __syncthreads();
+      if (kc + 1 >= nk) break;
+      // chunk kc + 1 in L1, kc + 2 in set 0, kc + 3 in set 1 (in flight)
+      if (kc + 2 < nk) MV_SLSTORE(0, 0);
+      if (kc + 4 < nk) MV_SGLOAD(0, kc + 4);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_chunk_n<DT, NI>(L1, L1 + BM * LDSA, wr, wc, l, acc);
+      __syncthreads();
+    }
+  }
+#undef MV_SGLOAD
+#undef MV_SLSTORE
+
+  // Epilogue: lane (g = l / 16, c = l % 16) of (mi, ni) holds rows
+  // wr*64 + mi*16 + 4g + r (r = 0..3) of column wc*16*NI + ni*16 + c. The
+  // screen values go through LDS (the staging area is free after the last
+  // K step; row stride 129 floats spreads the column reads over the banks),
+  // so that both the block and its transpose leave as rows of 512 bytes (32
+  // lanes x 16 bytes) instead of 64-byte pieces.
+  constexpr int TS = BN + 1;
+  static_assert(BM * TS <= STAGE_FLOATS, "the screen tile must fit in the staging area");
+  float* Dt = s_stage;
+  {
+    const int g = l >> 4, c = l & 15;
+    const int64_t rb0 = static_cast<int64_t>(I) * BM, cb0 = static_cast<int64_t>(J) * BN + wc * (16 * NI);
+    float sqc[NI], sqr[4][4], ivc[NI], ivr[4][4];
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      sqc[ni] = a.sqx[cb0 + ni * 16 + c];
+      ivc[ni] = DT == DT_FP8 ? a.invx[cb0 + ni * 16 + c] : 1.f;
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        sqr[mi][r] = a.sqx[rb0 + wr * 64 + mi * 16 + g * 4 + r];
+        ivr[mi][r] = DT == DT_FP8 ? a.invx[rb0 + wr * 64 + mi * 16 + g * 4 + r] : 1.f;
+      }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          // the screen's expression (FP8: the two per-row scales applied to the product)
+          const float dot = DT == DT_FP8 ? acc[mi][ni][r] * ivr[mi][r] * ivc[ni] : acc[mi][ni][r];
+          Dt[(wr * 64 + mi * 16 + g * 4 + r) * TS + wc * (16 * NI) + ni * 16 + c] =
+              fmaxf(sqr[mi][r] + sqc[ni] - 2.f * dot, 0.f);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t r0 = static_cast<int64_t>(I) * BM, c0 = static_cast<int64_t>(J) * BN;
+#pragma unroll
+  for (int q = 0; q < BM * BN / 4 / NTS; ++q) {
+    const int e = tid + q * NTS;
+    const int row = e >> 5, c4 = (e & 31) * 4;
+    v4f x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) x[k] = Dt[row * TS + c4 + k];
+    *reinterpret_cast<v4f*>(a.D + (r0 + row) * a.ldd + c0 + c4) = x;
+  }
+  if (I != J) {
+#pragma unroll
+    for (int q = 0; q < BM * BN / 4 / NTS; ++q) {
+      const int e = tid + q * NTS;
+      const int col = e >> 5, r4 = (e & 31) * 4;
+      v4f x;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x[k] = Dt[(r4 + k) * TS + col];
+      *reinterpret_cast<v4f*>(a.D + (c0 + col) * a.ldd + r0 + r4) = x;
+    }
+  }
+}
+
+struct SelectArgs {
+  const float* D;         // [n_pad, ldd]
+  int64_t ldd;
+  int n, nq, C;
+  const int32_t* q_self;  // nullable
+  int64_t self_base;      // used when q_self == NULL; < 0: no exclusion
+  uint64_t* cand;         // [nq_pad, C] keys (split 0 of the screen's layout)
+};
+
+// The C smallest keys (distance, index) of each row of D, one wavefront per
+// row: a float threshold (the list's C-th distance) rejects four columns per
+// lane per step; keys that pass wait in an LDS buffer of 64 and are merged
+// into the sorted list 64 at a time.
+__global__ __launch_bounds__(256) void k_select(SelectArgs a) {
+  __shared__ uint64_t s_pend[4][64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (row >= a.nq) return;
+  int64_t self = -1;
+  if (a.q_self) self = a.q_self[row];
+  else if (a.self_base >= 0) self = a.self_base + row;
+  const float* Dr = a.D + row * a.ldd;
+  uint64_t* pend = s_pend[w];
+  const int C = a.C;
+  uint64_t lk = KEY_NONE;
+  uint64_t tk = KEY_NONE;
+  float tf = INFINITY;
+  int cnt = 0;
+  // the loop bound is uniform: every lane takes part in the merges
+  for (int64_t base = 0; base < a.n; base += 256) {
+    const int64_t j0 = base + l * 4;
+    const v4f v = j0 < a.ldd ? *reinterpret_cast<const v4f*>(Dr + j0) : v4f{INFINITY, INFINITY, INFINITY, INFINITY};
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) any |= v[q] <= tf;
+    if (ballot(any) == 0) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t j = j0 + q;
+      const bool ok = j < a.n && j != self && v[q] <= tf;
+      uint64_t key = ok ? dkey(v[q], static_cast<uint32_t>(j)) : KEY_NONE;
+      key = key < tk ? key : KEY_NONE;
+      uint64_t m = ballot(key != KEY_NONE);
+      if (m == 0) continue;
+      if (cnt + __popcll(m) > 64) {
+        tk = list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
+        if (tk != KEY_NONE) tf = key_dist(tk);
+        cnt = 0;
+        key = key < tk ? key : KEY_NONE;
+        m = ballot(key != KEY_NONE);
+      }
+      const int pos = cnt + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+      if (key != KEY_NONE) pend[pos] = key;
+      cnt += __popcll(m);
+    }
+  }
+  if (cnt > 0) list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
+  if (l < C) a.cand[row * C + l] = lk;
 }
 
 // ----------------------------------------------------------------- rerank --
@@ -1158,6 +1535,7 @@ struct Layout {
   int64_t dq_pad;                          // quantised row elements (128-byte rows)
   int C, dtype;
   bool same;
+  bool sym;                                // symmetric screen: full distance matrix in the workspace
   size_t bytes;
 };
 
@@ -1176,6 +1554,7 @@ struct Buffers {
   float *Qc2, *sqq2;                            // FP32 re-screen rows
   char *Xs, *Qs2;                               // split BF16 copies for the re-screen
   int32_t* self2;
+  float* Dfull;                                 // symmetric screen: [n_pad, n_pad] screen values
 };
 
 static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L, size_t* total) {
@@ -1227,9 +1606,13 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
   const int64_t srows = nq < SCAN_ROWS ? (nq > 0 ? nq : 1) : SCAN_ROWS;
   b.scan_d = c.take<uint64_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
   b.scan_i = c.take<uint32_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
+  if (L.sym) b.Dfull = c.take<float>(static_cast<size_t>(L.n_pad) * L.n_pad);
   *total = c.off + 256;
   return b;
 }
+
+// Symmetric screen for self kNN (mv_knn_set_symmetric; on by default).
+static bool g_symmetric = true;
 
 static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, bool same, int dtype) {
   Layout L;
@@ -1245,6 +1628,8 @@ static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, b
   L.C = C;
   L.dtype = dtype;
   L.same = same;
+  L.sym = same && g_symmetric && n >= SYM_MIN_N &&
+          static_cast<size_t>(L.n_pad) * L.n_pad * 4 <= SYM_MAX_BYTES;
   size_t total = 0;
   carve(nullptr, nq, n, d, L, &total);
   L.bytes = total;
@@ -1252,11 +1637,13 @@ static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, b
 }
 
 // Optional phase timing (mv_knn_set_timing): HIP events recorded on the launch
-// stream around the prepass, screen, rerank and exact-scan launches.
+// stream around the prepass, screen, rerank and exact-scan launches, and
+// between the distance kernel and the selection of the symmetric screen.
 struct PhaseTimer {
   bool on = false;
   bool valid = false;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool last_sym = false;   // the last timed call used the symmetric screen (ev[5]: between its two kernels)
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int ensure() {
     for (auto& e : ev)
       if (!e) MV_TRY_HIP(hipEventCreate(&e));
@@ -1474,11 +1861,45 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     sa.row_bytes = L.dq_pad * elem_bytes(dtype);
   }
   if (int rc = g_timer.mark(1, s)) return rc;
-  int rc = dtype == MV_F32      ? launch_screen<DT_F32>(sa, grid, s)
-           : dtype == MV_BF16   ? launch_screen<DT_BF16>(sa, grid, s)
-           : dtype == MV_BF16X3 ? launch_screen<DT_BF16X3>(sa, grid, s)
-                                : launch_screen<DT_FP8>(sa, grid, s);
-  if (rc) return rc;
+  if (L.sym) {
+    // symmetric screen: upper-triangle tiles into the full matrix, then one
+    // list per row (a single split)
+    SymArgs ya;
+    ya.Xq = sa.Xq;
+    ya.sqx = sa.sqx;
+    ya.invx = sa.invx;
+    ya.row_bytes = sa.row_bytes;
+    ya.nb = static_cast<int>(col_tiles);
+    ya.nsb = static_cast<int>(ceil_div(col_tiles, SYM_G));
+    ya.n_slots = static_cast<int64_t>(ya.nsb) * (ya.nsb + 1) / 2 * SYM_G * SYM_G;
+    ya.per_xcd = ceil_div(ya.n_slots, 8);
+    ya.ldd = L.n_pad;
+    ya.D = B.Dfull;
+    const unsigned sgrid = static_cast<unsigned>(ya.per_xcd * 8);
+    auto ks = dtype == MV_F32 ? k_dist_sym<DT_F32, SYM_W>
+              : dtype == MV_BF16 ? k_dist_sym<DT_BF16, SYM_W>
+              : dtype == MV_BF16X3 ? k_dist_sym<DT_BF16X3, SYM_W> : k_dist_sym<DT_FP8, SYM_W>;
+    hipLaunchKernelGGL(ks, dim3(sgrid), dim3(64 * SYM_W), 0, s, ya);
+    MV_LAUNCHED("k_dist_sym");
+    if (int rc = g_timer.mark(5, s)) return rc;
+    SelectArgs xa;
+    xa.D = B.Dfull;
+    xa.ldd = L.n_pad;
+    xa.n = static_cast<int>(n);
+    xa.nq = static_cast<int>(nq);
+    xa.C = L.C;
+    xa.q_self = q_self;
+    xa.self_base = self_b;
+    xa.cand = B.cand;
+    hipLaunchKernelGGL(k_select, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
+    MV_LAUNCHED("k_select");
+  } else {
+    int rc = dtype == MV_F32      ? launch_screen<DT_F32>(sa, grid, s)
+             : dtype == MV_BF16   ? launch_screen<DT_BF16>(sa, grid, s)
+             : dtype == MV_BF16X3 ? launch_screen<DT_BF16X3>(sa, grid, s)
+                                  : launch_screen<DT_FP8>(sa, grid, s);
+    if (rc) return rc;
+  }
   if (int rc2 = g_timer.mark(2, s)) return rc2;
 
   // Screen arithmetic bound, in units of (|q|^2 + max |x|^2). FP32: two
@@ -1498,7 +1919,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   ra.n = static_cast<int>(n);
   ra.k = k;
   ra.C = L.C;
-  ra.S = S;
+  ra.S = L.sym ? 1 : S;
   ra.nq_pad = static_cast<int>(L.nq_pad);
   ra.cand = B.cand;
   ra.sqq = sa.sqq;
@@ -1548,6 +1969,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     sb.nq_dev = fb1_count;
     if (int rc2 = launch_screen<DT_BF16X3>(sb, grid, s)) return rc2;
     RerankArgs rb2 = ra;
+    rb2.S = S;   // the fused screen's lists (also after a symmetric first pass)
     rb2.sqq = B.sqq2;
     rb2.sqmax_bits = sqmax32;
     rb2.err_fac = static_cast<float>((3.0 * static_cast<double>(L.d_pad) + 3.0 * 256.0 + 40.0) *
@@ -1613,6 +2035,12 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   }
   if (int rc2 = g_timer.mark(4, s)) return rc2;
   g_timer.valid = g_timer.on;
+  g_timer.last_sym = L.sym;
+  return MV_OK;
+}
+
+int mv_knn_set_symmetric(int on) {
+  g_symmetric = on != 0;
   return MV_OK;
 }
 
@@ -1630,6 +2058,24 @@ int mv_knn_last_phase_ms(float* out4) {
   }
   MV_TRY_HIP(hipEventSynchronize(g_timer.ev[4]));
   for (int i = 0; i < 4; ++i) MV_TRY_HIP(hipEventElapsedTime(&out4[i], g_timer.ev[i], g_timer.ev[i + 1]));
+  return MV_OK;
+}
+
+int mv_knn_last_screen_ms(float* out2, int32_t* out_sym) {
+  MV_REQUIRE(out2 && out_sym, "mv_knn_last_screen_ms: null output");
+  if (!g_timer.valid) {
+    mv::set_error("mv_knn_last_screen_ms: no timed mv_knn_brute call (enable with mv_knn_set_timing)");
+    return MV_EINVAL;
+  }
+  MV_TRY_HIP(hipEventSynchronize(g_timer.ev[4]));
+  *out_sym = g_timer.last_sym ? 1 : 0;
+  if (g_timer.last_sym) {
+    MV_TRY_HIP(hipEventElapsedTime(&out2[0], g_timer.ev[1], g_timer.ev[5]));
+    MV_TRY_HIP(hipEventElapsedTime(&out2[1], g_timer.ev[5], g_timer.ev[2]));
+  } else {
+    MV_TRY_HIP(hipEventElapsedTime(&out2[0], g_timer.ev[1], g_timer.ev[2]));
+    out2[1] = 0.f;
+  }
   return MV_OK;
 }
 

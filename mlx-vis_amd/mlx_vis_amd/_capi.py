@@ -52,6 +52,8 @@ _SIGNATURES: dict[str, list] = {
     "mv_knn_brute": [_p, _i64, _p, _i64, _i64, _i32, _i32, _i32, _p, _i64, _i32, _p, _p, _p, _p, _sz, _p],
     "mv_row_sqnorms": [_p, _i64, _i64, _p, _p],
     "mv_knn_set_timing": [_i32],
+    "mv_knn_set_symmetric": [_i32],
+    "mv_knn_last_screen_ms": [_p, _p],
     "mv_knn_last_phase_ms": [_p],
     "mv_umap_graph_ws_bytes": [_i64, _i32, C.POINTER(_sz)],
     "mv_umap_fuzzy_graph": [_p, _p, _i64, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _sz, _p],

@@ -95,6 +95,33 @@ def test_c0_shape_bit_exact():
     np.testing.assert_allclose(dist[:1000], np.sqrt(g["dist2"]), rtol=RTOL, atol=1e-6)
 
 
+@pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp8"])
+@pytest.mark.parametrize("n,d,k", [(4096, 784, 15), (5003, 96, 30), (4500, 17, 5)])
+def test_symmetric_screen_matches_fused_screen(dtype, n, d, k):
+    """Self kNN from the upper-triangle distance matrix (the default for
+    n >= 4096) against the fused screen (symmetric screen off): identical
+    indices and distances, both bit-exact against the oracle; duplicated rows
+    give exact ties."""
+    from mlx_vis_amd import _capi
+
+    print(f"case dtype={dtype} n={n} d={d} k={k}", flush=True)
+    rng = np.random.default_rng(n + d)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    X[n // 2: n // 2 + 50] = X[:50]
+    lib = _capi.lib()
+    idx_s, dist_s, st_s = _run(X, k, dtype=dtype)
+    try:
+        lib.mv_knn_set_symmetric(0)
+        idx_f, dist_f, st_f = _run(X, k, dtype=dtype)
+    finally:
+        lib.mv_knn_set_symmetric(1)
+    print("stats sym", st_s, "fused", st_f)
+    np.testing.assert_array_equal(idx_s, idx_f)
+    np.testing.assert_array_equal(dist_s, dist_f)
+    oi, od = oknn.exact_knn(X, X, k, self_base=0)
+    _check(idx_s, dist_s, oi, od)
+
+
 @pytest.mark.parametrize("dtype", ["bf16", "fp8", "bf16x3"])
 @pytest.mark.parametrize("n,d,k", [(1000, 128, 15), (777, 3, 5), (300, 784, 15), (2000, 129, 32)])
 def test_precision_screens_give_identical_indices(dtype, n, d, k):
