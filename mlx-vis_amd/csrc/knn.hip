@@ -776,7 +776,27 @@ struct SymArgs {
   int64_t per_xcd;     // slots per XCD (ceil(n_slots / 8))
   int64_t ldd;         // n_pad
   float* D;            // [n_pad, n_pad]
+  float* bmin;         // [n_pad, nb]: per row, the smallest value of each 128-column block
+  int n;
+  int diag_self;       // the diagonal is each row's excluded self column
 };
+
+// Minimum over each 32-lane half of the wavefront, left in lanes 31 and 63:
+// row_shr:1, 2, 4, 8 (lanes without a source keep their value) give each
+// 16-lane row's minimum in its lane 15, row_bcast:15 folds row 0 into row 1
+// and row 2 into row 3.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_keep_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+__device__ __forceinline__ float min32_to_lane31(float v) {
+  v = fminf(v, dpp_keep_f32<0x111>(v));
+  v = fminf(v, dpp_keep_f32<0x112>(v));
+  v = fminf(v, dpp_keep_f32<0x114>(v));
+  v = fminf(v, dpp_keep_f32<0x118>(v));
+  v = fminf(v, dpp_keep_f32<0x142, 0xA>(v));
+  return v;
+}
 
 constexpr int SYM_G = 8;   // tiles per super-block side
 
@@ -959,14 +979,25 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
   }
   __syncthreads();
   const int64_t r0 = static_cast<int64_t>(I) * BM, c0 = static_cast<int64_t>(J) * BN;
+  // Each row of the block (and, off the diagonal, of its transpose) is
+  // stored by 32 lanes, which also reduce its minimum for the row selection
+  // (bmin[row][column block]); padding columns and each row's self column are
+  // left out of the minimum.
 #pragma unroll
   for (int q = 0; q < BM * BN / 4 / NTS; ++q) {
     const int e = tid + q * NTS;
     const int row = e >> 5, c4 = (e & 31) * 4;
     v4f x;
+    float mn = INFINITY;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) x[k] = Dt[row * TS + c4 + k];
+    for (int k = 0; k < 4; ++k) {
+      x[k] = Dt[row * TS + c4 + k];
+      const bool ok = c0 + c4 + k < a.n && !(a.diag_self && I == J && c4 + k == row);
+      mn = ok ? fminf(mn, x[k]) : mn;
+    }
     *reinterpret_cast<v4f*>(a.D + (r0 + row) * a.ldd + c0 + c4) = x;
+    mn = min32_to_lane31(mn);
+    if ((e & 31) == 31) a.bmin[(r0 + row) * a.nb + J] = mn;
   }
   if (I != J) {
 #pragma unroll
@@ -974,9 +1005,15 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
       const int e = tid + q * NTS;
       const int col = e >> 5, r4 = (e & 31) * 4;
       v4f x;
+      float mn = INFINITY;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) x[k] = Dt[(r4 + k) * TS + col];
+      for (int k = 0; k < 4; ++k) {
+        x[k] = Dt[(r4 + k) * TS + col];
+        mn = r0 + r4 + k < a.n ? fminf(mn, x[k]) : mn;
+      }
       *reinterpret_cast<v4f*>(a.D + (c0 + col) * a.ldd + r0 + r4) = x;
+      mn = min32_to_lane31(mn);
+      if ((e & 31) == 31) a.bmin[(c0 + col) * a.nb + I] = mn;
     }
   }
 }
@@ -987,12 +1024,38 @@ struct SelectArgs {
   int n, nq, C;
   const int32_t* q_self;  // nullable
   int64_t self_base;      // used when q_self == NULL; < 0: no exclusion
+  const float* bmin;      // [n_pad, nb] block minima (nullptr: read every block)
+  int nb;
   uint64_t* cand;         // [nq_pad, C] keys (split 0 of the screen's layout)
 };
 
+// Offers the key of each lane (KEY_NONE: none) to the row's list: keys below
+// the list's C-th key wait in the LDS buffer, which is merged into the list
+// when it would overflow.
+__device__ __forceinline__ void select_offer(uint64_t key, uint64_t& lk, uint64_t& tk, float& tf, int& cnt,
+                                             uint64_t* pend, int C, int l) {
+  key = key < tk ? key : KEY_NONE;
+  uint64_t m = ballot(key != KEY_NONE);
+  if (m == 0) return;
+  if (cnt + __popcll(m) > 64) {
+    tk = list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
+    if (tk != KEY_NONE) tf = key_dist(tk);
+    cnt = 0;
+    key = key < tk ? key : KEY_NONE;
+    m = ballot(key != KEY_NONE);
+  }
+  const int pos = cnt + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+  if (key != KEY_NONE) pend[pos] = key;
+  cnt += __popcll(m);
+}
+
 // The C smallest keys (distance, index) of each row of D, one wavefront per
-// row: a float threshold (the list's C-th distance) rejects four columns per
-// lane per step; keys that pass wait in an LDS buffer of 64 and are merged
+// row. With block minima, the C-th smallest of the row's block minima, tau,
+// bounds its C-th smallest value (C blocks hold a value <= tau), so only the
+// blocks whose minimum is <= tau are read: every value of another block is
+// > tau. Within the blocks read, a float threshold (the list's C-th distance)
+// rejects values; keys that pass wait in an LDS buffer of 64 and are merged
 // into the sorted list 64 at a time.
 __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
   __shared__ uint64_t s_pend[4][64];
@@ -1009,33 +1072,51 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
   uint64_t tk = KEY_NONE;
   float tf = INFINITY;
   int cnt = 0;
-  // the loop bound is uniform: every lane takes part in the merges
-  for (int64_t base = 0; base < a.n; base += 256) {
-    const int64_t j0 = base + l * 4;
-    const v4f v = j0 < a.ldd ? *reinterpret_cast<const v4f*>(Dr + j0) : v4f{INFINITY, INFINITY, INFINITY, INFINITY};
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) any |= v[q] <= tf;
-    if (ballot(any) == 0) continue;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t j = j0 + q;
-      const bool ok = j < a.n && j != self && v[q] <= tf;
-      uint64_t key = ok ? dkey(v[q], static_cast<uint32_t>(j)) : KEY_NONE;
-      key = key < tk ? key : KEY_NONE;
-      uint64_t m = ballot(key != KEY_NONE);
-      if (m == 0) continue;
-      if (cnt + __popcll(m) > 64) {
-        tk = list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
-        if (tk != KEY_NONE) tf = key_dist(tk);
-        cnt = 0;
-        key = key < tk ? key : KEY_NONE;
-        m = ballot(key != KEY_NONE);
+  float tau = INFINITY;
+  if (a.bmin && a.nb > C) {
+    // tau: the C-th smallest block minimum (its key list reuses the row list's machinery)
+    const float* bm = a.bmin + row * a.nb;
+    for (int b0 = 0; b0 < a.nb; b0 += 64) {
+      const int b = b0 + l;
+      const float v = b < a.nb ? bm[b] : INFINITY;
+      select_offer(v < INFINITY ? dkey(v, static_cast<uint32_t>(b)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
+    }
+    if (cnt > 0) tk = list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
+    if (tk != KEY_NONE) tau = key_dist(tk);
+    lk = KEY_NONE;
+    tk = KEY_NONE;
+    tf = INFINITY;
+    cnt = 0;
+    // the blocks to read, 64 at a time; each block: 2 columns per lane
+    for (int b0 = 0; b0 < a.nb; b0 += 64) {
+      const int b = b0 + l;
+      uint64_t need = ballot(b < a.nb && bm[b] <= tau);
+      while (need) {
+        const int jb = (b0 + __builtin_ctzll(need)) * BN;
+        need &= need - 1;
+        const int64_t j = jb + 2 * l;
+        const float2 v = *reinterpret_cast<const float2*>(Dr + j);
+        const bool ok0 = j < a.n && j != self && v.x <= tf;
+        const bool ok1 = j + 1 < a.n && j + 1 != self && v.y <= tf;
+        select_offer(ok0 ? dkey(v.x, static_cast<uint32_t>(j)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
+        select_offer(ok1 ? dkey(v.y, static_cast<uint32_t>(j + 1)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
       }
-      const int pos = cnt + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                                static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
-      if (key != KEY_NONE) pend[pos] = key;
-      cnt += __popcll(m);
+    }
+  } else {
+    // the loop bound is uniform: every lane takes part in the merges
+    for (int64_t base = 0; base < a.n; base += 256) {
+      const int64_t j0 = base + l * 4;
+      const v4f v = j0 < a.ldd ? *reinterpret_cast<const v4f*>(Dr + j0) : v4f{INFINITY, INFINITY, INFINITY, INFINITY};
+      bool any = false;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) any |= v[q] <= tf;
+      if (ballot(any) == 0) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t j = j0 + q;
+        const bool ok = j < a.n && j != self && v[q] <= tf;
+        select_offer(ok ? dkey(v[q], static_cast<uint32_t>(j)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
+      }
     }
   }
   if (cnt > 0) list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
@@ -1071,7 +1152,22 @@ struct RerankArgs {
 __device__ __forceinline__ double dist64_seq(const float* __restrict__ q, const float* __restrict__ x,
                                              int64_t d) {
   double acc = 0.0;
-  for (int64_t t = 0; t < d; ++t) {
+  int64_t t = 0;
+  // sixteen loads in flight per step, then the terms in order
+  for (; t + 16 <= d; t += 16) {
+    float qv[16], xv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      qv[u] = q[t + u];
+      xv[u] = x[t + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const double df = static_cast<double>(qv[u]) - static_cast<double>(xv[u]);
+      acc = __dadd_rn(acc, __dmul_rn(df, df));
+    }
+  }
+  for (; t < d; ++t) {
     const double df = static_cast<double>(q[t]) - static_cast<double>(x[t]);
     acc = __dadd_rn(acc, __dmul_rn(df, df));
   }
@@ -1555,6 +1651,7 @@ struct Buffers {
   char *Xs, *Qs2;                               // split BF16 copies for the re-screen
   int32_t* self2;
   float* Dfull;                                 // symmetric screen: [n_pad, n_pad] screen values
+  float* bmin;                                  // symmetric screen: [n_pad, n_pad / 128] block minima
 };
 
 static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L, size_t* total) {
@@ -1606,7 +1703,10 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
   const int64_t srows = nq < SCAN_ROWS ? (nq > 0 ? nq : 1) : SCAN_ROWS;
   b.scan_d = c.take<uint64_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
   b.scan_i = c.take<uint32_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
-  if (L.sym) b.Dfull = c.take<float>(static_cast<size_t>(L.n_pad) * L.n_pad);
+  if (L.sym) {
+    b.Dfull = c.take<float>(static_cast<size_t>(L.n_pad) * L.n_pad);
+    b.bmin = c.take<float>(static_cast<size_t>(L.n_pad) * (L.n_pad / BN));
+  }
   *total = c.off + 256;
   return b;
 }
@@ -1875,6 +1975,9 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     ya.per_xcd = ceil_div(ya.n_slots, 8);
     ya.ldd = L.n_pad;
     ya.D = B.Dfull;
+    ya.bmin = B.bmin;
+    ya.n = static_cast<int>(n);
+    ya.diag_self = q_self == nullptr && self_b == 0;
     const unsigned sgrid = static_cast<unsigned>(ya.per_xcd * 8);
     auto ks = dtype == MV_F32 ? k_dist_sym<DT_F32, SYM_W>
               : dtype == MV_BF16 ? k_dist_sym<DT_BF16, SYM_W>
@@ -1890,6 +1993,10 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     xa.C = L.C;
     xa.q_self = q_self;
     xa.self_base = self_b;
+    // block pruning needs the diagonal to be the excluded self column (as the
+    // block minima leave it out) or no exclusion at all
+    xa.bmin = (q_self == nullptr && self_b == 0) ? B.bmin : nullptr;
+    xa.nb = static_cast<int>(col_tiles);
     xa.cand = B.cand;
     hipLaunchKernelGGL(k_select, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
     MV_LAUNCHED("k_select");
