@@ -70,6 +70,8 @@ def parse():
                     help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8", "bf16x3"])
     ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
+    ap.add_argument("--no-symmetric", action="store_true",
+                    help="self kNN with the fused screen (k_screen) instead of the symmetric distance kernel")
     return ap.parse_args()
 
 
@@ -354,8 +356,11 @@ def main():
     import numpy as np
     import torch
 
-    from mlx_vis_amd import UMAP
+    from mlx_vis_amd import UMAP, _capi
     from mlx_vis_amd.datasets import gaussian_mixture_784
+
+    if args.no_symmetric:
+        _capi.lib().mv_knn_set_symmetric(0)
 
     if args.only == "copy":
         src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 1 GiB
@@ -467,8 +472,6 @@ def main():
                               "rerank": round(phases[2], 3), "exact_scan": round(phases[3], 3)},
                 "ms_screen_split": {"distance_kernel": round(scr["kernel_ms"], 3),
                                     "row_select": round(scr["select_ms"], 3)},
-                "row_select_gbs": (round(4.0 * ((n + 127) // 128 * 128) ** 2 / (scr["select_ms"] * 1e-3) / 1e9, 1)
-                                   if scr["symmetric"] and scr["select_ms"] > 0 else None),
                 "rows_fp64_scan": stats[0], "rows_fp64_tiecheck": stats[1],
                 "tflops_whole_knn": round(flops / (knn_total_ms * 1e-3) / 1e12, 2)},
         "phases_s": {k: round(v, 4) for k, v in ph.items()},

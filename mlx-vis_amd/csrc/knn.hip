@@ -4,7 +4,9 @@
 // reference forms the full (chunk x n) matrix max(|x_i|^2 + |x_j|^2 - 2 x_i.x_j, 0)
 // (_knn.py:60-63), masks the diagonal with +1e30 (_knn.py:64-69), argsorts every
 // row and keeps k columns (_knn.py:71-72), then takes sqrt (_knn.py:86-87).
-// Here nothing of size n x n is ever materialised:
+// Here no row is ever sorted, and the n x n matrix is materialised only by the
+// symmetric screen (self kNN up to 48 GiB of matrix), where it halves the
+// multiplies:
 //
 //   1. prepass     column means (FP64 partial sums), then a centred, zero-padded
 //                  copy of X (and of Q when Q != X) with FP32 row norms and the
@@ -17,6 +19,15 @@
 //                  ballot/shuffle insertion. Columns are split S ways; the
 //                  blockIdx -> (row block, split) map keeps one split per XCD so
 //                  the 32 CUs of an XCD stream the same X tiles through its L2.
+//   2'. symmetric  self kNN (Q = X): MFMA tiles on and above the diagonal only
+//                  (k_dist_sym, 8 wavefronts per 128x128 tile), each stored with
+//                  its transpose into the n_pad x n_pad matrix together with
+//                  per-row minima of every 128-column block; k_select then keeps
+//                  each row's C smallest keys, reading only the blocks whose
+//                  minimum can hold one. The screen values are the fused
+//                  screen's (FP8: up to the order of the two row scales in
+//                  the transposed half), and the re-rank makes the results
+//                  identical.
 //   3. rerank      one wavefront per query row: merge the S lists into the global
 //                  top-C, FP32 direct-difference distances of the C candidates,
 //                  FP64 (sequential, unfused) only where two FP32 values cannot be
