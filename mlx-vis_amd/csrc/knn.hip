@@ -2004,9 +2004,10 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     xa.C = L.C;
     xa.q_self = q_self;
     xa.self_base = self_b;
-    // block pruning needs the diagonal to be the excluded self column (as the
-    // block minima leave it out) or no exclusion at all
-    xa.bmin = (q_self == nullptr && self_b == 0) ? B.bmin : nullptr;
+    // block pruning needs the block minima to range over exactly the
+    // selectable columns: the diagonal excluded (self_base 0, as the minima
+    // leave it out) or no exclusion at all; an arbitrary q_self reads every block
+    xa.bmin = (q_self == nullptr && self_b <= 0) ? B.bmin : nullptr;
     xa.nb = static_cast<int>(col_tiles);
     xa.cand = B.cand;
     hipLaunchKernelGGL(k_select, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);

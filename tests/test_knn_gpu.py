@@ -164,6 +164,29 @@ def test_edge_shapes(n, d, k, dtype):
     _check(idx, dist, oi, od)
 
 
+@pytest.mark.parametrize("mode", ["no_exclusion", "q_self"])
+def test_symmetric_screen_other_self_rules(mode):
+    """Self kNN through the symmetric screen without self exclusion (block
+    minima then include the diagonal) and with an arbitrary excluded row per
+    query (every block read), against the oracle."""
+    from mlx_vis_amd import knn_brute_device
+
+    n, d, k = 4600, 40, 12
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    X[3000:3020] = X[:20]                                   # duplicates: exact ties
+    Xd = torch.from_numpy(X).cuda()
+    if mode == "no_exclusion":
+        idx, dist = knn_brute_device(Xd, k, exclude_self=False)
+        oi, od = oknn.exact_knn(X, X, k)
+    else:
+        qs = ((np.arange(n) * 7 + 3) % n).astype(np.int32)
+        idx, dist = knn_brute_device(Xd, k, q_self=torch.from_numpy(qs))
+        oi, od = oknn.exact_knn(X, X, k, q_self=qs)
+    torch.cuda.synchronize()
+    _check(idx.cpu().numpy(), dist.cpu().numpy(), oi, od)
+
+
 def test_single_query_and_invalid_arguments():
     from mlx_vis_amd import knn_brute_device
 
