@@ -106,7 +106,8 @@ int mv_knn_last_screen_ms(float* out2, int32_t* out_sym);
  * candidate list is selected from it; the returned neighbours and distances
  * are those of the fused screen.
  * Off: every self kNN uses the fused screen. Process-wide; set it before
- * mv_knn_ws_bytes and keep it until the matching mv_knn_brute has run. */
+ * mv_knn_ws_bytes and keep it until the matching mv_knn_brute has run.
+ * Returns the previous setting (1 on, 0 off), not a status. */
 int mv_knn_set_symmetric(int on);
 
 /* Row squared norms: out[i] = sum_t X[i,t]^2 in FP32 (mx.sum(X*X, 1) at

@@ -2159,8 +2159,9 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
 }
 
 int mv_knn_set_symmetric(int on) {
+  const int prev = g_symmetric ? 1 : 0;
   g_symmetric = on != 0;
-  return MV_OK;
+  return prev;
 }
 
 int mv_knn_set_timing(int on) {

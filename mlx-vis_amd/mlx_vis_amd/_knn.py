@@ -62,21 +62,38 @@ def knn_brute_device(
     lib = _capi.lib()
     import ctypes
 
-    nbytes = ctypes.c_size_t(0)
-    _capi.check(
-        lib.mv_knn_ws_bytes(nq, n, d, int(k), _capi.DTYPES[dtype], int(rerank), int(same), ctypes.byref(nbytes)),
-        "mv_knn_ws_bytes",
-    )
-    ws = _device.workspace(nbytes.value)
-    _capi.check(
-        lib.mv_knn_brute(
-            Q.data_ptr(), nq, X.data_ptr(), n, d, int(k), _capi.DTYPES[dtype], int(rerank),
-            _capi.ptr(q_self), self_base, int(bool(euclid)),
-            out_idx.data_ptr(), out_dist.data_ptr(), _capi.ptr(stats),
-            ws.data_ptr(), ws.numel(), _device.stream(),
-        ),
-        "mv_knn_brute",
-    )
+    def ws_bytes():
+        nbytes = ctypes.c_size_t(0)
+        _capi.check(
+            lib.mv_knn_ws_bytes(nq, n, d, int(k), _capi.DTYPES[dtype], int(rerank), int(same), ctypes.byref(nbytes)),
+            "mv_knn_ws_bytes",
+        )
+        return nbytes.value
+
+    nbytes = ws_bytes()
+    # The symmetric self-kNN screen keeps the n x n matrix in its workspace;
+    # when that would take more than half of the free device memory, this call
+    # uses the fused screen instead (same results, no matrix).
+    sym_prev = None
+    if same and n >= 4096:
+        free, _ = torch.cuda.mem_get_info(X.device)
+        if nbytes > free // 2:
+            sym_prev = lib.mv_knn_set_symmetric(0)
+            nbytes = ws_bytes()
+    try:
+        ws = _device.workspace(nbytes)
+        _capi.check(
+            lib.mv_knn_brute(
+                Q.data_ptr(), nq, X.data_ptr(), n, d, int(k), _capi.DTYPES[dtype], int(rerank),
+                _capi.ptr(q_self), self_base, int(bool(euclid)),
+                out_idx.data_ptr(), out_dist.data_ptr(), _capi.ptr(stats),
+                ws.data_ptr(), ws.numel(), _device.stream(),
+            ),
+            "mv_knn_brute",
+        )
+    finally:
+        if sym_prev is not None:
+            lib.mv_knn_set_symmetric(sym_prev)
     return out_idx, out_dist
 
 

@@ -110,11 +110,12 @@ def test_symmetric_screen_matches_fused_screen(dtype, n, d, k):
     X[n // 2: n // 2 + 50] = X[:50]
     lib = _capi.lib()
     idx_s, dist_s, st_s = _run(X, k, dtype=dtype)
+    prev = lib.mv_knn_set_symmetric(0)
     try:
-        lib.mv_knn_set_symmetric(0)
+        assert prev == 1
         idx_f, dist_f, st_f = _run(X, k, dtype=dtype)
     finally:
-        lib.mv_knn_set_symmetric(1)
+        lib.mv_knn_set_symmetric(prev)
     print("stats sym", st_s, "fused", st_f)
     np.testing.assert_array_equal(idx_s, idx_f)
     np.testing.assert_array_equal(dist_s, dist_f)
