@@ -96,7 +96,7 @@ def test_c0_shape_bit_exact():
 
 
 @pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp8"])
-@pytest.mark.parametrize("n,d,k", [(4096, 784, 15), (5003, 96, 30), (4500, 17, 5)])
+@pytest.mark.parametrize("n,d,k", [(4096, 784, 15), (5003, 96, 30), (4500, 17, 5), (4099, 2, 64)])
 def test_symmetric_screen_matches_fused_screen(dtype, n, d, k):
     """Self kNN from the upper-triangle distance matrix (the default for
     n >= 4096) against the fused screen (symmetric screen off): identical
