@@ -507,6 +507,22 @@ def main():
             "s": round((time.perf_counter() - t) / max(args.steps, 1), 4),
             "identical_knn_indices": bool(torch.equal(um3.knn_[0], um.knn_[0]))}
         line["precision"] = prec
+        log("small plain-method workload: 10,000 x 128 N(0,1), 200 epochs, random init")
+        import numpy as np_
+        Xc0 = torch.from_numpy(np_.random.default_rng(0).standard_normal((10000, 128), dtype=np_.float32)).cuda()
+
+        def step_c0():
+            return UMAP(n_neighbors=15, min_dist=0.1, n_epochs=200, negative_sample_rate=5, init="random",
+                        knn_method="brute", random_state=0).fit_transform(Xc0)
+        step_c0()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            step_c0()
+        torch.cuda.synchronize()
+        line["small_workload"] = {"workload": "UMAP fit_transform 10000x128 N(0,1) seed 0, exact brute kNN k=15, "
+                                              "random-uniform init in [-10, 10], 200 epochs, 5 negatives",
+                                  "s": round((time.perf_counter() - t) / 3, 4)}
         log("NNDescent k=15, 12 iterations, max_candidates=30")
         from mlx_vis_amd import NNDescent
 

@@ -141,6 +141,38 @@ def test_epochs_without_negatives_match_oracle(n, k, dim, hubs, masked):
     np.testing.assert_allclose(got[hub], Yo[hub], rtol=1e-4, atol=1e-3)
 
 
+def test_c1_single_step_on_its_graph():
+    """The full-shape single-step check: the 70,000 x 784 mixture (seed 1), its
+    exact 15-NN graph and schedule, Y ~ U[-10, 10] (seed 0), the first 4,096
+    slots (the reference's row-major forward-edge order) active at epoch 5,
+    five fixed negatives per edge (heads tiled as umap.py:434-436), one
+    _sgd_step (umap.py:359-387) against the FP64 oracle, tolerance 1e-5."""
+    from mlx_vis_amd import find_ab_params, knn_brute_device
+    from mlx_vis_amd._umap import fuzzy_graph_device, sgd_step_device
+    from mlx_vis_amd.datasets import gaussian_mixture_784
+
+    n, k, n_epochs, epoch = 70_000, 15, 500, 5
+    X = gaussian_mixture_784(n, seed=1)
+    idx, dist = knn_brute_device(torch.from_numpy(X).cuda(), k)
+    g = fuzzy_graph_device(idx, dist, n_epochs)
+    eps = g.eps.cpu().numpy().ravel()
+    active = np.flatnonzero((eps > 0) & (eps <= epoch))[:4096]   # epochs_per_next starts at eps
+    assert len(active) == 4096
+    ef = (active // k).astype(np.int32)
+    et = g.knn_idx.cpu().numpy().ravel()[active].astype(np.int32)
+    A = len(active)
+    rng = np.random.default_rng(0)
+    Y = rng.uniform(-10, 10, (n, 2)).astype(np.float32)
+    nf = ef[np.arange(5 * A) % A]
+    nt = rng.integers(0, n, 5 * A).astype(np.int32)
+    a, b = find_ab_params(1.0, 0.1)
+    alpha = 1.0 * (1.0 - epoch / n_epochs)
+    got = sgd_step_device(torch.from_numpy(Y).cuda(), *(torch.from_numpy(v).cuda() for v in (ef, et, nf, nt)),
+                          alpha, a, b).cpu().numpy()
+    want = ou.sgd_step(Y, ef, et, nf, nt, np.float32(alpha), np.float32(a), np.float32(b), dtype=np.float64)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
+
+
 def test_layout_quality_matches_oracle_run():
     """Same graph, a/b and init: 15-NN preservation and trustworthiness of the
     GPU layout within 0.01 of the oracle's epoch loop."""
