@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ Z, 
 }
 
 constexpr int HMAX = 1024;        // widest hidden layer
-constexpr int HEAD_PPW = 8;       // points per wavefront in k_head_bwd
+constexpr int HEAD_PPW = 4;       // points per wavefront in k_head_bwd
 
 // Output-layer backward for DOUT outputs. Per point p:
 //   dA2[p, c] = (sum_o dZ[p, o] W3[o, c]) * gelu'(A2pre[p, c])   (stored BF16)
@@ -555,6 +555,14 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, 
   }
 #pragma unroll
   for (int o = 0; o < DOUT; ++o) pb3[o] = 0.f;
+  // W3 columns of this lane, read once
+  float w3r[DOUT][JM];
+#pragma unroll
+  for (int j = 0; j < JM; ++j) {
+    const int c = l + 64 * j;
+#pragma unroll
+    for (int o = 0; o < DOUT; ++o) w3r[o][j] = c < h2 ? W3[o * h2 + c] : 0.f;
+  }
   const int p0 = (blockIdx.x * 4 + w) * HEAD_PPW;
   for (int i = 0; i < HEAD_PPW; ++i) {
     const int p = p0 + i;
@@ -571,7 +579,7 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, 
       if (c < h2) {
         float s = 0.f;
 #pragma unroll
-        for (int o = 0; o < DOUT; ++o) s = fmaf(dz[o], W3[o * h2 + c], s);
+        for (int o = 0; o < DOUT; ++o) s = fmaf(dz[o], w3r[o][j], s);
         const int64_t at = static_cast<int64_t>(p) * h2 + c;
         const bf16_t g = f2bf(s * gelu_grad(bf2f(A2pre[at])));
         dA2[at] = g;
