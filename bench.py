@@ -508,8 +508,7 @@ def main():
             "identical_knn_indices": bool(torch.equal(um3.knn_[0], um.knn_[0]))}
         line["precision"] = prec
         log("small plain-method workload: 10,000 x 128 N(0,1), 200 epochs, random init")
-        import numpy as np_
-        Xc0 = torch.from_numpy(np_.random.default_rng(0).standard_normal((10000, 128), dtype=np_.float32)).cuda()
+        Xc0 = torch.from_numpy(np.random.default_rng(0).standard_normal((10000, 128), dtype=np.float32)).cuda()
 
         def step_c0():
             return UMAP(n_neighbors=15, min_dist=0.1, n_epochs=200, negative_sample_rate=5, init="random",
