@@ -45,6 +45,33 @@ def test_ws_query_and_invalid_arguments_map_to_valueerror():
     assert ei.value.status == _capi.MV_EUNSUPPORTED
 
 
+def test_symmetric_screen_workspace_and_toggle():
+    """The symmetric self-kNN screen adds the n_pad^2 FP32 matrix (and block
+    minima) to the workspace, only for self kNN within its size range; the
+    toggle returns its previous setting."""
+    from mlx_vis_amd import _capi
+
+    lib = _capi.lib()
+
+    def ws(nq, n, same):
+        nb = ctypes.c_size_t(0)
+        _capi.check(lib.mv_knn_ws_bytes(nq, n, 784, 15, 0, 2, same, ctypes.byref(nb)))
+        return nb.value
+
+    n_pad = 70_016
+    on = ws(70_000, 70_000, 1)
+    assert lib.mv_knn_set_symmetric(0) == 1
+    try:
+        off = ws(70_000, 70_000, 1)
+        assert lib.mv_knn_set_symmetric(0) == 0
+    finally:
+        assert lib.mv_knn_set_symmetric(1) == 0
+    assert on - off >= n_pad * n_pad * 4 + n_pad * (n_pad // 128) * 4
+    assert ws(70_000, 70_000, 0) < n_pad * n_pad * 4           # queries are not the data
+    assert ws(4_000, 4_000, 1) < 4_096 * 4_096 * 4              # below 4,096 rows: fused screen
+    assert ws(200_000, 200_000, 1) < 200_064 * 200_064 * 4      # beyond 48 GiB: fused screen
+
+
 def test_fault_injection_maps_to_mivis_error(monkeypatch):
     from mlx_vis_amd import _capi
 
