@@ -436,6 +436,16 @@ def main():
     run_epochs(Y0, None, g, 1, args.epochs, args.epochs, 5, 1.0, a, b, 1)
     torch.cuda.synchronize()
     ph["epochs"] = time.perf_counter() - t
+    # Layout roofline: every kept edge is active floor((n_epochs - 1) / eps)
+    # times; algorithmic bytes per epoch 12 E (slot scan) + 80 per active edge
+    # (ids, head/tail and 5 negative gathers, head/tail updates), against HBM.
+    eps_k = g.eps[g.eps > 0].double()
+    samples = float(torch.floor((float(args.epochs - 1) / eps_k) + 1e-9).sum().item())
+    lay_bytes = 12.0 * g.n_edges * (args.epochs - 1) + 80.0 * samples
+    layout = {"epochs": args.epochs - 1, "kept_edges": g.n_edges, "edge_samples": int(samples),
+              "edge_samples_per_s": round(samples / ph["epochs"], 1),
+              "algorithmic_gbs": round(lay_bytes / ph["epochs"] / 1e9, 1),
+              "frac_hbm": round(lay_bytes / ph["epochs"] / 1e9 / HBM_PEAK_GBS, 4)}
 
     log("kNN phase timing")
     phases, stats, ref_idx, scr = knn_phase_timing(Xd, args.knn_reps)
@@ -475,6 +485,7 @@ def main():
                 "rows_fp64_scan": stats[0], "rows_fp64_tiecheck": stats[1],
                 "tflops_whole_knn": round(flops / (knn_total_ms * 1e-3) / 1e12, 2)},
         "phases_s": {k: round(v, 4) for k, v in ph.items()},
+        "layout": layout,
         "quality": {"knn15_preservation": round(quality, 4), "n_edges": g.n_edges},
         "roofline": {"kernel": kern, "bound": "mfma", "achieved": round(screen_tflops, 2),
                      "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
