@@ -427,27 +427,56 @@ __global__ __launch_bounds__(256) void k_gather_x(const bf16_t* __restrict__ src
 }
 
 // Z[p] = H2[p] W3^T + b3 (one wavefront per point; dout <= 4; W3 FP32).
+// Output layer forward, Z[p, o] = sum_c H2[p, c] W3[o, c] + b3[o]: four
+// points per wavefront, whose rows are all loaded before any sum (one memory
+// latency per four points); lane l takes columns l + 64 j, and each (point,
+// output) sum is reduced over the wavefront with DPP (lane 63 ends with it).
+constexpr int HEADF_PPW = 4;
+__device__ __forceinline__ float wave_sum_to_lane63(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));   // row_shr:1
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, false));   // row_shr:2
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, false));   // row_shr:4
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, false));   // row_shr:8
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));   // row_bcast:15
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));   // row_bcast:31
+  return v;
+}
+template <int DOUT, int JM>   // JM: 64-column groups of the hidden layer (h2 <= 64 JM)
 __global__ __launch_bounds__(256) void k_head_fwd(const bf16_t* __restrict__ H2, int P, int h2,
-                                                  const float* __restrict__ W3, const float* __restrict__ b3, int dout,
+                                                  const float* __restrict__ W3, const float* __restrict__ b3,
                                                   float* __restrict__ Z) {
-  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  constexpr int dout = DOUT;
+  const int p0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * HEADF_PPW;
   const int l = threadIdx.x & 63;
-  if (p >= P) return;
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int c = l; c < h2; c += 64) {
-    const float h = bf2f(H2[static_cast<int64_t>(p) * h2 + c]);
+  if (p0 >= P) return;
+  float w3r[DOUT][JM];
 #pragma unroll
-    for (int o = 0; o < 4; ++o)
-      if (o < dout) acc[o] = fmaf(h, W3[o * h2 + c], acc[o]);
+  for (int j = 0; j < JM; ++j) {
+    const int c = l + 64 * j;
+#pragma unroll
+    for (int o = 0; o < DOUT; ++o) w3r[o][j] = c < h2 ? W3[o * h2 + c] : 0.f;
+  }
+  float h[HEADF_PPW][JM];
+#pragma unroll
+  for (int i = 0; i < HEADF_PPW; ++i) {
+    const int p = min(p0 + i, P - 1);
+#pragma unroll
+    for (int j = 0; j < JM; ++j) {
+      const int c = l + 64 * j;
+      h[i][j] = c < h2 ? bf2f(H2[static_cast<int64_t>(p) * h2 + c]) : 0.f;
+    }
   }
 #pragma unroll
-  for (int o = 0; o < 4; ++o) acc[o] = wave_sum_f32(acc[o]);
-  if (l < dout) {
-    float v = acc[0];
-    if (l == 1) v = acc[1];
-    if (l == 2) v = acc[2];
-    if (l == 3) v = acc[3];
-    Z[static_cast<int64_t>(p) * dout + l] = v + b3[l];
+  for (int i = 0; i < HEADF_PPW; ++i) {
+    const int p = p0 + i;
+#pragma unroll
+    for (int o = 0; o < DOUT; ++o) {
+      float acc = 0.f;
+#pragma unroll
+      for (int j = 0; j < JM; ++j) acc = fmaf(h[i][j], w3r[o][j], acc);
+      acc = wave_sum_to_lane63(acc);
+      if (l == 63 && p < P) Z[static_cast<int64_t>(p) * dout + o] = acc + b3[o];
+    }
   }
 }
 
@@ -862,8 +891,20 @@ static int forward(const Params& W, const Dims& D, const bf16_t* Xb, int P, bf16
   g.pre_out = A2;
   g.ldx = D.h2;
   if (int rc = gemm<EPI_BIAS_GELU>(g, s)) return rc;
-  hipLaunchKernelGGL(k_head_fwd, dim3(static_cast<unsigned>(ceil_div(P, 4))), dim3(256), 0, s, H2, P, D.h2, W.W3,
-                     W.b3, D.dout, Z);
+  {
+    const dim3 hg(static_cast<unsigned>(ceil_div(P, 4 * HEADF_PPW)));
+    const bool narrow = D.h2 <= 512;
+#define MV_HEADF(DO)                                                                                 \
+  if (narrow) hipLaunchKernelGGL((k_head_fwd<DO, 8>), hg, dim3(256), 0, s, H2, P, D.h2, W.W3, W.b3, Z); \
+  else hipLaunchKernelGGL((k_head_fwd<DO, 16>), hg, dim3(256), 0, s, H2, P, D.h2, W.W3, W.b3, Z)
+    switch (D.dout) {
+      case 1: MV_HEADF(1); break;
+      case 2: MV_HEADF(2); break;
+      case 3: MV_HEADF(3); break;
+      default: MV_HEADF(4); break;
+    }
+#undef MV_HEADF
+  }
   MV_LAUNCHED("k_head_fwd");
   return MV_OK;
 }
