@@ -387,10 +387,40 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
 }
 
 // out[c, r] = in[r, c] for a rows x cols BF16 matrix (64 x 64 tiles through LDS).
+// 64 x 64 tiles through LDS. Where the rows are whole 16-byte groups (cols,
+// ldi and ldo multiples of 8, the MLP's padded widths) each thread moves
+// 8-element vectors both ways; otherwise 2-byte elements.
 __global__ __launch_bounds__(256) void k_transpose_bf16(const bf16_t* __restrict__ in, int rows, int cols,
                                                         int64_t ldi, bf16_t* __restrict__ out, int64_t ldo) {
   __shared__ bf16_t tile[64][66];
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const bool vec = (cols % 8) == 0 && (ldi % 8) == 0 && (ldo % 8) == 0 && (rows % 8) == 0;
+  if (vec) {
+    // load: thread t takes row t / 8 (+ 32), columns 8 (t % 8) .. + 7
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int y = (threadIdx.x >> 3) + 32 * h, x8 = (threadIdx.x & 7) * 8;
+      const int r = r0 + y, c = c0 + x8;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (r < rows && c < cols) v = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(r) * ldi + c);
+      const bf16_t* e = reinterpret_cast<const bf16_t*>(&v);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) tile[y][x8 + q] = e[q];
+    }
+    __syncthreads();
+    // store: thread t writes transposed row t / 8 (+ 32), entries 8 (t % 8) .. + 7
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int y = (threadIdx.x >> 3) + 32 * h, x8 = (threadIdx.x & 7) * 8;
+      const int c = c0 + y, r = r0 + x8;
+      uint4 v;
+      bf16_t* e = reinterpret_cast<bf16_t*>(&v);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) e[q] = tile[x8 + q][y];
+      if (c < cols && r < rows) *reinterpret_cast<uint4*>(out + static_cast<int64_t>(c) * ldo + r) = v;
+    }
+    return;
+  }
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int y = ty; y < 64; y += 4) {
     const int r = r0 + y, c = c0 + tx;
