@@ -19,7 +19,7 @@
 //                    FP32 store, bias + GELU (stores pre-activation and
 //                    activation in BF16), GELU-gradient product (BF16).
 //   k_transpose_bf16, k_gather_x, k_to_bf16, k_head_fwd, k_loss_grad,
-//   k_head_bwd, k_adam, k_sample_edges.
+//   k_head_bwd, k_adam_blob, k_sample_edges.
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
@@ -668,28 +668,41 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, 
   if (tid < DOUT) atomicAdd(&gb3[tid], s_b3[tid]);
 }
 
-// Adam on FP32 master weights (bias-corrected), writing a BF16 copy and,
-// when `bt` is set, the BF16 transpose (rows x cols -> cols x rows).
-__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ gsum,
-                                              float* __restrict__ m, float* __restrict__ v, int64_t count, float lr,
-                                              float b1, float b2, float eps, float c1, float c2,
-                                              bf16_t* __restrict__ bcopy, bf16_t* __restrict__ bt, int rows,
-                                              int cols) {
+// Adam on the FP32 master weights (bias corrections c1 = 1 - b1^t,
+// c2 = 1 - b2^t) over the whole parameter blob in one launch: W1 (with its
+// BF16 copy), b1, W2 (BF16 copy and BF16 transpose), then b2, W3, b3.
+struct AdamBlob {
+  float* p;
+  const float* g;
+  float* m;
+  float* v;
+  int64_t np, o_b1, o_W2, o_b2;
+  int h1, h2;
+  bf16_t* W1b;
+  bf16_t* W2b;
+  bf16_t* W2tb;
+};
+__global__ __launch_bounds__(256) void k_adam_blob(AdamBlob a, float lr, float b1, float b2, float eps, float c1,
+                                                   float c2) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const float g = gsum[t];
-  const float mt = b1 * m[t] + (1.f - b1) * g;
-  const float vt = b2 * v[t] + (1.f - b2) * g * g;
-  m[t] = mt;
-  v[t] = vt;
-  const float nv = p[t] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
-  p[t] = nv;
-  if (bcopy) bcopy[t] = f2bf(nv);
-  if (bt) {
-    const int r = static_cast<int>(t / cols), c = static_cast<int>(t % cols);
-    bt[static_cast<int64_t>(c) * rows + r] = f2bf(nv);
+  if (t >= a.np) return;
+  const float g = a.g[t];
+  const float mt = b1 * a.m[t] + (1.f - b1) * g;
+  const float vt = b2 * a.v[t] + (1.f - b2) * g * g;
+  a.m[t] = mt;
+  a.v[t] = vt;
+  const float nv = a.p[t] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+  a.p[t] = nv;
+  if (t < a.o_b1) {
+    a.W1b[t] = f2bf(nv);
+  } else if (t >= a.o_W2 && t < a.o_b2) {
+    const int64_t e = t - a.o_W2;
+    a.W2b[e] = f2bf(nv);
+    const int r = static_cast<int>(e / a.h1), c = static_cast<int>(e % a.h1);
+    a.W2tb[static_cast<int64_t>(c) * a.h2 + r] = f2bf(nv);
   }
 }
+
 
 // Edge batches drawn from a fuzzy graph in its dense slot layout (slot e =
 // i * k + c, head i, tail tail[e], eps[e] = epochs per sample = w_max / w,
@@ -1063,16 +1076,23 @@ static int adam_all(const Params& W, const Dims& D, const float* grad, int64_t n
   const int64_t nW1 = static_cast<int64_t>(D.h1) * D.dp;
   const int64_t nW2 = static_cast<int64_t>(D.h2) * D.h1;
   const int64_t o_b1 = nW1, o_W2 = o_b1 + D.h1, o_b2 = o_W2 + nW2, o_W3 = o_b2 + D.h2;
-  auto launch = [&](int64_t off, int64_t cnt, bf16_t* bc, bf16_t* bt, int rows, int cols) -> int {
-    hipLaunchKernelGGL(k_adam, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0, s, W.W1 + off,
-                       grad + off, W.m + off, W.v + off, cnt, lr, b1, b2, eps, c1, c2, bc, bt, rows, cols);
-    MV_LAUNCHED("k_adam");
-    return MV_OK;
-  };
-  if (int rc = launch(0, nW1, W.W1b, nullptr, D.h1, D.dp)) return rc;
-  if (int rc = launch(o_b1, D.h1, nullptr, nullptr, 1, D.h1)) return rc;
-  if (int rc = launch(o_W2, nW2, W.W2b, W.W2tb, D.h2, D.h1)) return rc;
-  if (int rc = launch(o_b2, np - o_b2, nullptr, nullptr, 1, static_cast<int>(np - o_b2))) return rc;
+  AdamBlob ab;
+  ab.p = W.W1;
+  ab.g = grad;
+  ab.m = W.m;
+  ab.v = W.v;
+  ab.np = np;
+  ab.o_b1 = o_b1;
+  ab.o_W2 = o_W2;
+  ab.o_b2 = o_b2;
+  ab.h1 = D.h1;
+  ab.h2 = D.h2;
+  ab.W1b = W.W1b;
+  ab.W2b = W.W2b;
+  ab.W2tb = W.W2tb;
+  hipLaunchKernelGGL(k_adam_blob, dim3(static_cast<unsigned>(ceil_div(np, 256))), dim3(256), 0, s, ab, lr, b1, b2,
+                     eps, c1, c2);
+  MV_LAUNCHED("k_adam_blob");
   (void)o_W3;
   return MV_OK;
 }
