@@ -590,7 +590,7 @@ constexpr int HEAD_PPW = 4;       // points per wavefront in k_head_bwd
 // and the block's partial sums of dW3 = dZ^T H2, db3 = sum dZ and
 // db2 = sum dA2 (of the BF16-rounded values, as the GEMMs see them), reduced
 // in LDS and added with one atomic per block and parameter.
-template <int DOUT>
+template <int DOUT, int JM>   // JM: 64-column groups of the hidden layer (h2 <= 64 JM)
 __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, const bf16_t* __restrict__ H2,
                                                   const bf16_t* __restrict__ A2pre, int P, int h2,
                                                   const float* __restrict__ W3, bf16_t* __restrict__ dA2,
@@ -604,7 +604,6 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, 
   for (int t = tid; t < h2; t += 256) s_b2[t] = 0.f;
   if (tid < DOUT) s_b3[tid] = 0.f;
   __syncthreads();
-  constexpr int JM = HMAX / 64;
   float pw3[DOUT][JM], pb2[JM], pb3[DOUT];
 #pragma unroll
   for (int j = 0; j < JM; ++j) {
@@ -622,30 +621,42 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, 
 #pragma unroll
     for (int o = 0; o < DOUT; ++o) w3r[o][j] = c < h2 ? W3[o * h2 + c] : 0.f;
   }
+  // the wavefront's HEAD_PPW points: every load first (one memory latency),
+  // then the arithmetic
   const int p0 = (blockIdx.x * 4 + w) * HEAD_PPW;
+  float dzs[HEAD_PPW][DOUT], a2v[HEAD_PPW][JM], hv[HEAD_PPW][JM];
+#pragma unroll
+  for (int i = 0; i < HEAD_PPW; ++i) {
+    const int p = min(p0 + i, P - 1);
+#pragma unroll
+    for (int o = 0; o < DOUT; ++o) dzs[i][o] = dZ[static_cast<int64_t>(p) * DOUT + o];
+#pragma unroll
+    for (int j = 0; j < JM; ++j) {
+      const int c = l + 64 * j;
+      const int64_t at = static_cast<int64_t>(p) * h2 + (c < h2 ? c : 0);
+      a2v[i][j] = bf2f(A2pre[at]);
+      hv[i][j] = bf2f(H2[at]);
+    }
+  }
+#pragma unroll
   for (int i = 0; i < HEAD_PPW; ++i) {
     const int p = p0 + i;
     if (p >= P) break;
-    float dz[DOUT];
 #pragma unroll
-    for (int o = 0; o < DOUT; ++o) {
-      dz[o] = dZ[static_cast<int64_t>(p) * DOUT + o];
-      pb3[o] += dz[o];
-    }
+    for (int o = 0; o < DOUT; ++o) pb3[o] += dzs[i][o];
 #pragma unroll
     for (int j = 0; j < JM; ++j) {
       const int c = l + 64 * j;
       if (c < h2) {
         float s = 0.f;
 #pragma unroll
-        for (int o = 0; o < DOUT; ++o) s = fmaf(dz[o], w3r[o][j], s);
+        for (int o = 0; o < DOUT; ++o) s = fmaf(dzs[i][o], w3r[o][j], s);
         const int64_t at = static_cast<int64_t>(p) * h2 + c;
-        const bf16_t g = f2bf(s * gelu_grad(bf2f(A2pre[at])));
+        const bf16_t g = f2bf(s * gelu_grad(a2v[i][j]));
         dA2[at] = g;
         pb2[j] += bf2f(g);
-        const float h = bf2f(H2[at]);
 #pragma unroll
-        for (int o = 0; o < DOUT; ++o) pw3[o][j] = fmaf(dz[o], h, pw3[o][j]);
+        for (int o = 0; o < DOUT; ++o) pw3[o][j] = fmaf(dzs[i][o], hv[i][j], pw3[o][j]);
       }
     }
   }
@@ -1015,11 +1026,20 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   MV_LAUNCHED("k_fill_f32");
   {
     const dim3 grid(static_cast<unsigned>(ceil_div(P, 4 * HEAD_PPW)));
+    const bool narrow = D.h2 <= 512;
     switch (D.dout) {
-      case 1: hipLaunchKernelGGL(k_head_bwd<1>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
-      case 2: hipLaunchKernelGGL(k_head_bwd<2>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
-      case 3: hipLaunchKernelGGL(k_head_bwd<3>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
-      default: hipLaunchKernelGGL(k_head_bwd<4>, grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2); break;
+      case 1: if (narrow) hipLaunchKernelGGL((k_head_bwd<1, 8>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+              else hipLaunchKernelGGL((k_head_bwd<1, 16>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+              break;
+      case 2: if (narrow) hipLaunchKernelGGL((k_head_bwd<2, 8>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+              else hipLaunchKernelGGL((k_head_bwd<2, 16>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+              break;
+      case 3: if (narrow) hipLaunchKernelGGL((k_head_bwd<3, 8>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+              else hipLaunchKernelGGL((k_head_bwd<3, 16>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+              break;
+      default: if (narrow) hipLaunchKernelGGL((k_head_bwd<4, 8>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+               else hipLaunchKernelGGL((k_head_bwd<4, 16>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
+               break;
     }
     MV_LAUNCHED("k_head_bwd");
   }
