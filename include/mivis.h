@@ -270,6 +270,18 @@ int mv_mlp_sample_edges(const float* eps, const int32_t* tail, int64_t n, int k,
 int mv_knn_overlap(const int32_t* a, int64_t lda, int ka, const int32_t* b, int64_t ldb, int kb, int64_t n,
                    int min_count, int32_t* out_count, uint8_t* out_flag, void* stream);
 
+/* Trustworthiness (Venna & Kaski 2001) ranks; no counterpart in the reference,
+ * which defines no metric. X (n, d) FP32 input space; rows (m) the query
+ * points (NULL: rows 0 .. m-1); lo (m, ldl) the embedding's k neighbours of
+ * each query. out_excess[q] = sum_t max(r(i, lo[q, t]) - k, 0), r the rank of
+ * the neighbour among the n - 1 other points ordered by (FP32 squared
+ * distance, index), 1 = nearest; a neighbour equal to the query point has
+ * rank n; entries outside [0, n) count 0. T = 1 - 2 / (m k (2n - 3k - 1)) *
+ * sum_q out_excess[q]. 1 <= k <= 64. */
+int mv_trust_ws_bytes(int64_t m, int k, size_t* out);
+int mv_trust_excess(const float* X, int64_t n, int64_t d, const int32_t* rows, int64_t m, const int32_t* lo,
+                    int64_t ldl, int k, int64_t* out_excess, void* ws, size_t ws_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Edge-gradient family beyond UMAP: the contrastive losses of CNE and the
  * pair force laws of PaCMAP / LocalMAP / t-SNE, on the same embedding layout

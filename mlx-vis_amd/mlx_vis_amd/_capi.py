@@ -81,6 +81,8 @@ _SIGNATURES: dict[str, list] = {
     "mv_mlp_forward": [_p, _i32, _i32, _i32, _i32, _p, _i64, _p, _i32, _p, _sz, _p],
     "mv_mlp_sample_edges": [_p, _p, _i64, _i32, _u32, _i32, _i32, _i32, _p, _p],
     "mv_knn_overlap": [_p, _i64, _i32, _p, _i64, _i32, _i64, _i32, _p, _p, _p],
+    "mv_trust_ws_bytes": [_i64, _i32, C.POINTER(_sz)],
+    "mv_trust_excess": [_p, _i64, _i64, _p, _i64, _p, _i64, _i32, _p, _p, _sz, _p],
     "mv_random_normal": [_p, _i64, _u64, _p],
     "mv_umap_spectral_embed": [_p, _i64, _i32, _u64, _p, _p, _p, _sz, _p],
     "mv_cne_grad": [_p, _i64, _i32, _p, _p, _i64, _p, _i32, _i32, _f32, _u64, _i32, _i64, _p, _p],

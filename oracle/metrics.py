@@ -41,6 +41,31 @@ def knn_preservation(X_high, Y_low, k: int = 15, rows=None) -> float:
     return recall(lo, hi)
 
 
+def trust_excess(X_high, lo_idx, rows=None) -> np.ndarray:
+    """Per-row sum of max(r(i, j) - k, 0) over the given low-dimensional
+    neighbour lists lo_idx (m, k): r(i, j) is the rank of j among all other
+    points ordered by (input-space squared distance, index), 1 = nearest; the
+    query point itself sorts last (rank n). FP64 expansion distances (exact
+    for small-integer or dyadic inputs)."""
+    X = np.asarray(X_high, dtype=np.float64)
+    lo = np.asarray(lo_idx, dtype=np.int64)
+    n = X.shape[0]
+    rows = np.arange(lo.shape[0]) if rows is None else np.asarray(rows)
+    k = lo.shape[1]
+    sqx = (X * X).sum(1)
+    out = np.zeros(rows.size, dtype=np.int64)
+    for s in range(0, rows.size, 256):
+        r = rows[s:s + 256]
+        Dh = sqx[r, None] + sqx[None, :] - 2.0 * X[r] @ X.T
+        Dh[np.arange(r.size), r] = np.inf
+        order_h = np.argsort(Dh, axis=1, kind="stable")
+        ranks = np.empty_like(order_h)
+        ranks[np.arange(r.size)[:, None], order_h] = np.arange(1, n + 1)[None, :]
+        rk = np.take_along_axis(ranks, lo[s:s + 256], axis=1)
+        out[s:s + r.size] = np.maximum(rk - k, 0).sum(axis=1)
+    return out
+
+
 def trustworthiness(X_high, Y_low, k: int = 15, rows=None) -> float:
     """Trustworthiness (Venna & Kaski 2001), optionally over a row sample.
 
@@ -49,24 +74,16 @@ def trustworthiness(X_high, Y_low, k: int = 15, rows=None) -> float:
     neighbours; r(i, j): rank of j among i's input neighbours (1 = nearest).
     m = number of rows averaged (n for the full statistic).
     """
-    X = np.asarray(X_high, dtype=np.float64)
     Y = np.asarray(Y_low, dtype=np.float64)
-    n = X.shape[0]
+    n = Y.shape[0]
     rows = np.arange(n) if rows is None else np.asarray(rows)
-    sqx = (X * X).sum(1)
     sqy = (Y * Y).sum(1)
-    total = 0.0
+    nn_l = np.empty((rows.size, k), dtype=np.int64)
     for s in range(0, rows.size, 256):
         r = rows[s:s + 256]
-        Dh = sqx[r, None] + sqx[None, :] - 2.0 * X[r] @ X.T
         Dl = sqy[r, None] + sqy[None, :] - 2.0 * Y[r] @ Y.T
-        Dh[np.arange(r.size), r] = np.inf
         Dl[np.arange(r.size), r] = np.inf
-        order_h = np.argsort(Dh, axis=1, kind="stable")
-        ranks = np.empty_like(order_h)
-        ranks[np.arange(r.size)[:, None], order_h] = np.arange(1, n + 1)[None, :]
-        nn_l = np.argsort(Dl, axis=1, kind="stable")[:, :k]
-        rk = np.take_along_axis(ranks, nn_l, axis=1)
-        total += np.maximum(rk - k, 0).sum()
+        nn_l[s:s + r.size] = np.argsort(Dl, axis=1, kind="stable")[:, :k]
+    total = int(trust_excess(X_high, nn_l, rows).sum())
     m = rows.size
     return float(1.0 - 2.0 / (m * k * (2.0 * n - 3.0 * k - 1.0)) * total)

@@ -161,3 +161,25 @@ def test_knn_preservation_bounds():
     X = rng.standard_normal((200, 5)).astype(np.float32)
     assert om.knn_preservation(X, X, 10) == 1.0
     assert om.knn_preservation(X, rng.standard_normal((200, 2)), 10) < 0.3
+
+
+def test_trust_excess_brute_force_ranks_with_ties():
+    """oracle trust_excess against ranks counted pair by pair: r(i, j) = 1 +
+    #{l != i : (D(i, l), l) < (D(i, j), j)}, the query itself ranked n, on
+    small-integer points with many equal distances."""
+    rng = np.random.default_rng(11)
+    n, d, k = 40, 3, 6
+    X = rng.integers(0, 3, size=(n, d)).astype(np.float64)
+    rows = np.array([5, 0, 39, 17, 5, 22])
+    lo = rng.integers(0, n, size=(rows.size, k))
+    lo[1, 2] = rows[1]                      # the query itself: rank n
+    want = np.zeros(rows.size, dtype=np.int64)
+    for q, i in enumerate(rows):
+        D = [float(((X[i] - X[l]) ** 2).sum()) for l in range(n)]
+        for j in lo[q]:
+            if j == i:
+                r = n
+            else:
+                r = 1 + sum(1 for l in range(n) if l != i and (D[l], l) < (D[j], j))
+            want[q] += max(r - k, 0)
+    np.testing.assert_array_equal(om.trust_excess(X, lo, rows), want)
