@@ -458,6 +458,14 @@ def main():
     screen_tflops = kern_flops / (scr["kernel_ms"] * 1e-3) / 1e12
     knn_total_ms = sum(phases)
     quality = knn_preservation_gpu(um.knn_[0], Y)
+    # trustworthiness of the embedding over all 70K points: exact ranks in the
+    # input space for each embedding neighbour (m x n x d distance work, VALU)
+    from mlx_vis_amd import trustworthiness
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    trust = trustworthiness(Xd, torch.as_tensor(Y).cuda(), 15)
+    torch.cuda.synchronize()
+    t_trust = time.perf_counter() - t
 
     line = {
         "metric": "fit_transform s, 70K x 784 kNN+UMAP; kNN TFLOP/s; bf16/fp8 vs fp32 at exact kNN",
@@ -486,7 +494,8 @@ def main():
                 "tflops_whole_knn": round(flops / (knn_total_ms * 1e-3) / 1e12, 2)},
         "phases_s": {k: round(v, 4) for k, v in ph.items()},
         "layout": layout,
-        "quality": {"knn15_preservation": round(quality, 4), "n_edges": g.n_edges},
+        "quality": {"knn15_preservation": round(quality, 4), "trustworthiness_15": round(trust, 5),
+                    "trustworthiness_s": round(t_trust, 4), "n_edges": g.n_edges},
         "roofline": {"kernel": kern, "bound": "mfma", "achieved": round(screen_tflops, 2),
                      "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": screen_traffic(n, d, kern)},
