@@ -85,6 +85,8 @@ struct GemmArgs {
   float* colsum;           // EPI_GELU_GRAD (nullable): colsum[col] += sum over rows of the output
   const float* W3;         // EPI_HEAD: output layer (dout x N, FP32)
   int dout;                // EPI_HEAD: outputs (<= 4); C is Z (M x dout FP32), pre-filled with b3
+  bf16_t* ct;              // BF16 epilogues (nullable): transposed copy, ct[col * ldct + row]; C may then be null
+  int64_t ldct;
 };
 
 // Sum over the 16 lanes of a DPP row (row_ror 8, 4, 2, 1); every lane of the
@@ -299,7 +301,8 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
   constexpr int TS = BN + 8;   // LDS row stride of the output tile (BF16 elements)
   bf16_t* s_out = reinterpret_cast<bf16_t*>(s_stage);
   bf16_t* s_pre = s_out + BM * TS;
-  const bool vec_out = BF16_OUT && g.N % 8 == 0 && g.ldc % 8 == 0 && (!g.pre_out || g.ldx % 8 == 0);
+  const bool vec_out = BF16_OUT && g.N % 8 == 0 && g.ldc % 8 == 0 && (!g.pre_out || g.ldx % 8 == 0) &&
+                       (!g.ct || (g.M % 8 == 0 && g.ldct % 8 == 0));
   float csum[4] = {0.f, 0.f, 0.f, 0.f};   // EPI_GELU_GRAD: column sums of this lane's outputs
   // VEC: BF16 outputs to the LDS tile (uniform choice, made once per tile)
 #define MV_EPI_LOOP(VEC)                                                                            \
@@ -341,7 +344,8 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
             s_out[lrow * TS + lcol] = o;                                                          \
             if (EPI == EPI_BIAS_GELU && g.pre_out) s_pre[lrow * TS + lcol] = pb;                  \
           } else if (ok) {                                                                        \
-            static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = o;               \
+            if (g.C) static_cast<bf16_t*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = o;      \
+            if (g.ct) g.ct[static_cast<int64_t>(col) * g.ldct + row] = o;                         \
             if (EPI == EPI_BIAS_GELU && g.pre_out) g.pre_out[static_cast<int64_t>(row) * g.ldx + col] = pb; \
           }                                                                                       \
         }                                                                                         \
@@ -363,11 +367,30 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
       const int lrow = sgi >> 4, lcol = (sgi & 15) * 8;
       const int row = bm + lrow, col = bn + lcol;
       if (row < g.M && col < g.N) {
-        *reinterpret_cast<float4*>(static_cast<bf16_t*>(g.C) + static_cast<int64_t>(row) * g.ldc + col) =
-            *reinterpret_cast<const float4*>(s_out + lrow * TS + lcol);
+        if (g.C)
+          *reinterpret_cast<float4*>(static_cast<bf16_t*>(g.C) + static_cast<int64_t>(row) * g.ldc + col) =
+              *reinterpret_cast<const float4*>(s_out + lrow * TS + lcol);
         if (EPI == EPI_BIAS_GELU && g.pre_out)
           *reinterpret_cast<float4*>(g.pre_out + static_cast<int64_t>(row) * g.ldx + col) =
               *reinterpret_cast<const float4*>(s_pre + lrow * TS + lcol);
+      }
+    }
+    if (g.ct) {
+      // transposed copy: 128 columns x 16 segments of 8 rows; lanes take
+      // consecutive columns (conflict-free LDS reads down a column)
+#pragma unroll
+      for (int j = 0; j < (BM * BN / 8) / NT; ++j) {
+        const int sgi = tid + NT * j;
+        const int lcol = sgi & (BN - 1), lrow = (sgi / BN) * 8;
+        const int row = bm + lrow, col = bn + lcol;
+        if (row < g.M && col < g.N) {
+          uint32_t w[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            w[e] = static_cast<uint32_t>(s_out[(lrow + 2 * e) * TS + lcol]) |
+                   (static_cast<uint32_t>(s_out[(lrow + 2 * e + 1) * TS + lcol]) << 16);
+          *reinterpret_cast<uint4*>(g.ct + static_cast<int64_t>(col) * g.ldct + row) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
       }
     }
     __syncthreads();   // the next tile's staging overwrites the output tile
@@ -841,7 +864,7 @@ static Params carve_params(void* blob, const Dims& D, size_t* total) {
 
 // Per-batch buffers for P points.
 struct Batch {
-  bf16_t *Xb, *A1, *H1, *A2, *H2, *dA2, *dA2t, *H1t, *dA1, *dA1t, *Xbt;
+  bf16_t *Xb, *A1, *H1, *A2, *H2, *dA2, *dA2t, *H1t, *dA1t, *Xbt;
   float *Z, *Yb, *dZ, *grad, *loss;
   int32_t* idx;
 };
@@ -857,7 +880,6 @@ static Batch carve_batch(void* ws, const Dims& D, int P, int64_t np, size_t* tot
   b.dA2 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h2);
   b.dA2t = c.take<bf16_t>(static_cast<int64_t>(P) * D.h2);
   b.H1t = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
-  b.dA1 = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
   b.dA1t = c.take<bf16_t>(static_cast<int64_t>(P) * D.h1);
   b.Xbt = c.take<bf16_t>(static_cast<int64_t>(P) * D.dp);
   b.Z = c.take<float>(static_cast<int64_t>(P) * D.dout);
@@ -917,8 +939,8 @@ static int transpose(const bf16_t* in, int rows, int cols, int64_t ldi, bf16_t* 
 }
 
 // Forward of P rows already in Xb (BF16, stride dp): A1/H1, A2/H2, Z.
-static int forward(const Params& W, const Dims& D, const bf16_t* Xb, int P, bf16_t* A1, bf16_t* H1, bf16_t* A2,
-                   bf16_t* H2, float* Z, hipStream_t s) {
+static int forward(const Params& W, const Dims& D, const bf16_t* Xb, int P, bf16_t* A1, bf16_t* H1, bf16_t* H1t,
+                   bf16_t* A2, bf16_t* H2, float* Z, hipStream_t s) {
   GemmArgs g{};
   g.A = Xb;
   g.B = W.W1b;
@@ -932,7 +954,10 @@ static int forward(const Params& W, const Dims& D, const bf16_t* Xb, int P, bf16
   g.bias = W.b1;
   g.pre_out = A1;
   g.ldx = D.h1;
+  g.ct = H1t;   // H1^T for the weight gradient of layer 2 (rows of P)
+  g.ldct = P;
   if (int rc = gemm<EPI_BIAS_GELU>(g, s)) return rc;
+  g.ct = nullptr;
   g.A = H1;
   g.B = W.W2b;
   g.N = D.h2;
@@ -1044,7 +1069,6 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
     MV_LAUNCHED("k_head_bwd");
   }
   if (int rc = transpose(b.dA2, P, D.h2, D.h2, b.dA2t, P, s)) return rc;
-  if (int rc = transpose(b.H1, P, D.h1, D.h1, b.H1t, P, s)) return rc;
   GemmArgs g{};
   // gW2 (h2 x h1) = dA2^T H1
   g.A = b.dA2t;
@@ -1066,13 +1090,14 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   g.K = D.h2;
   g.lda = D.h2;
   g.ldb = D.h2;
-  g.C = b.dA1;
+  g.C = nullptr;   // only dA1^T is needed (the weight gradient of layer 1)
   g.ldc = D.h1;
+  g.ct = b.dA1t;
+  g.ldct = P;
   g.pre_in = b.A1;
   g.ldx = D.h1;
   g.colsum = gb1;
   if (int rc = gemm<EPI_GELU_GRAD>(g, s)) return rc;
-  if (int rc = transpose(b.dA1, P, D.h1, D.h1, b.dA1t, P, s)) return rc;
   if (int rc = transpose(b.Xb, P, D.dp, D.dp, b.Xbt, P, s)) return rc;
   // gW1 (h1 x dp) = dA1^T Xb
   g = GemmArgs{};
@@ -1228,7 +1253,7 @@ int mv_mlp_grad(const void* blob, int din, int h1, int h2, int dout, const float
   hipLaunchKernelGGL(k_gather_x, dim3(static_cast<unsigned>(ceil_div(P, 4))), dim3(256), 0, s, Xbf, D.dp, idx, P,
                      bt.Xb, Yp, dout, bt.Yb);
   MV_LAUNCHED("k_gather_x");
-  if (int rc = forward(W, D, bt.Xb, P, bt.A1, bt.H1, bt.A2, bt.H2, bt.Z, s)) return rc;
+  if (int rc = forward(W, D, bt.Xb, P, bt.A1, bt.H1, bt.H1t, bt.A2, bt.H2, bt.Z, s)) return rc;
   hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, bt.loss, static_cast<int64_t>(2), 0.f);
   MV_LAUNCHED("k_fill_f32");
   hipLaunchKernelGGL(k_loss_grad, dim3(static_cast<unsigned>(ceil_div(B, 256))), dim3(256), 0, s, bt.Z, bt.Yb, B, dout,
@@ -1271,7 +1296,7 @@ int mv_mlp_train(void* blob, int din, int h1, int h2, int dout, const float* X, 
     hipLaunchKernelGGL(k_gather_x, dim3(static_cast<unsigned>(ceil_div(P, 4))), dim3(256), 0, s, Xbf, D.dp,
                        idx + static_cast<int64_t>(t) * P, P, bt.Xb, Yp, dout, bt.Yb);
     MV_LAUNCHED("k_gather_x");
-    if (int rc = forward(W, D, bt.Xb, P, bt.A1, bt.H1, bt.A2, bt.H2, bt.Z, s)) return rc;
+    if (int rc = forward(W, D, bt.Xb, P, bt.A1, bt.H1, bt.H1t, bt.A2, bt.H2, bt.Z, s)) return rc;
     float* lo = loss_trace ? loss_trace + 2 * static_cast<int64_t>(t) : nullptr;
     if (lo) {
       hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, lo, static_cast<int64_t>(2), 0.f);
