@@ -87,6 +87,7 @@ struct GemmArgs {
   int dout;                // EPI_HEAD: outputs (<= 4); C is Z (M x dout FP32), pre-filled with b3
   bf16_t* ct;              // BF16 epilogues (nullable): transposed copy, ct[col * ldct + row]; C may then be null
   int64_t ldct;
+  int64_t split_stride;    // EPI_F32: K split s stores its partial product to C + s * split_stride
 };
 
 // Sum over the 16 lanes of a DPP row (row_ror 8, 4, 2, 1); every lane of the
@@ -138,7 +139,8 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
   if (mt >= g.m_tiles) continue;
   const int cc = (t >> 3) % ncols;
   const int bm = mt * BM, bn = (cc % g.n_tiles) * BN;
-  const int kc0 = (cc / g.n_tiles) * g.kchunks;
+  const int ks = cc / g.n_tiles;   // K split
+  const int kc0 = ks * g.kchunks;
   const int kc1 = min(nk_all, kc0 + g.kchunks);
   if (kc0 >= kc1) continue;
 
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
         const bool ok = row < g.M && col < g.N;                                                   \
         const float v = acc[mi][ni][r];                                                           \
         if constexpr (EPI == EPI_F32) {                                                           \
-          if (ok) static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col] = v;          \
+          if (ok) static_cast<float*>(g.C)[ks * g.split_stride + static_cast<int64_t>(row) * g.ldc + col] = v; \
         } else if constexpr (EPI == EPI_F32_ADD) {                                                \
           if (ok) atomicAdd(&static_cast<float*>(g.C)[static_cast<int64_t>(row) * g.ldc + col], v); \
         } else {                                                                                  \
@@ -707,7 +709,11 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, 
 // BF16 copy), b1, W2 (BF16 copy and BF16 transpose), then b2, W3, b3.
 struct AdamBlob {
   float* p;
-  const float* g;
+  float* g;               // gradient blob; entries outside W1 / W2 are cleared after use
+  const float* part1;     // W1 gradient: s1 partial slices of stride st1
+  const float* part2;     // W2 gradient: s2 partial slices of stride st2
+  int64_t st1, st2;
+  int s1, s2;
   float* m;
   float* v;
   int64_t np, o_b1, o_W2, o_b2;
@@ -720,7 +726,16 @@ __global__ __launch_bounds__(256) void k_adam_blob(AdamBlob a, float lr, float b
                                                    float c2) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= a.np) return;
-  const float g = a.g[t];
+  float g = 0.f;
+  if (t < a.o_b1) {
+    for (int q = 0; q < a.s1; ++q) g += a.part1[q * a.st1 + t];
+  } else if (t >= a.o_W2 && t < a.o_b2) {
+    const int64_t e = t - a.o_W2;
+    for (int q = 0; q < a.s2; ++q) g += a.part2[q * a.st2 + e];
+  } else {
+    g = a.g[t];
+    a.g[t] = 0.f;   // ready for the next step's atomic sums
+  }
   const float mt = b1 * a.m[t] + (1.f - b1) * g;
   const float vt = b2 * a.v[t] + (1.f - b2) * g * g;
   a.m[t] = mt;
@@ -737,6 +752,22 @@ __global__ __launch_bounds__(256) void k_adam_blob(AdamBlob a, float lr, float b
   }
 }
 
+// The W1 / W2 entries of the gradient blob from their partial slices (split
+// order, as k_adam_blob sums them).
+__global__ __launch_bounds__(256) void k_reduce_parts(AdamBlob a) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nW2 = a.o_b2 - a.o_W2;
+  if (t < a.o_b1) {
+    float g = 0.f;
+    for (int q = 0; q < a.s1; ++q) g += a.part1[q * a.st1 + t];
+    a.g[t] = g;
+  } else if (t - a.o_b1 < nW2) {
+    const int64_t e = t - a.o_b1;
+    float g = 0.f;
+    for (int q = 0; q < a.s2; ++q) g += a.part2[q * a.st2 + e];
+    a.g[a.o_W2 + e] = g;
+  }
+}
 
 // Edge batches drawn from a fuzzy graph in its dense slot layout (slot e =
 // i * k + c, head i, tail tail[e], eps[e] = epochs per sample = w_max / w,
@@ -862,10 +893,29 @@ static Params carve_params(void* blob, const Dims& D, size_t* total) {
   return P;
 }
 
+// K splits actually used for a request of `req` over K (every split non-empty).
+static int split_count(int K, int req) {
+  const int nk = K / 64;
+  if (nk < 1) return 1;
+  if (req > nk) req = nk;
+  if (req < 1) req = 1;
+  const int kch = static_cast<int>(ceil_div(nk, req));
+  return static_cast<int>(ceil_div(nk, kch));
+}
+
+// K splits for a weight-gradient GEMM: about 256 workgroups in total.
+static int grad_splits(int M, int N, int K) {
+  const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
+  int sp = static_cast<int>(256 / (tiles > 0 ? tiles : 1));
+  return sp < 1 ? 1 : sp;
+}
+
 // Per-batch buffers for P points.
 struct Batch {
   bf16_t *Xb, *A1, *H1, *A2, *H2, *dA2, *dA2t, *H1t, *dA1t, *Xbt;
   float *Z, *Yb, *dZ, *grad, *loss;
+  float *part1, *part2;   // K-split partial slices of the W1 / W2 gradients
+  int s1, s2;             // their split counts
   int32_t* idx;
 };
 
@@ -886,6 +936,10 @@ static Batch carve_batch(void* ws, const Dims& D, int P, int64_t np, size_t* tot
   b.Yb = c.take<float>(static_cast<int64_t>(P) * D.dout);
   b.dZ = c.take<float>(static_cast<int64_t>(P) * D.dout);
   b.grad = c.take<float>(np);
+  b.s1 = split_count(P, grad_splits(D.h1, D.dp, P));
+  b.s2 = split_count(P, grad_splits(D.h2, D.h1, P));
+  b.part1 = c.take<float>(static_cast<int64_t>(b.s1) * D.h1 * D.dp);
+  b.part2 = c.take<float>(static_cast<int64_t>(b.s2) * D.h2 * D.h1);
   b.loss = c.take<float>(2);
   b.idx = c.take<int32_t>(P);
   if (total) *total = align_up(c.off, 256) + 256;   // a multiple of 256: data placed after it stays aligned
@@ -905,14 +959,13 @@ static int cu_count() {
 }
 
 // Launches k_gemm_nt over ceil(N/128) x ceil(M/128) tiles and `splits` K
-// ranges (splits > 1 only with EPI_F32_ADD) on a persistent grid.
+// ranges (splits > 1 with EPI_F32_ADD, or EPI_F32 into split_stride-spaced
+// partial slices) on a persistent grid.
 template <int EPI, bool AF32 = false>
 static int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
   const int nk = g.K / 64;
-  if (splits < 1) splits = 1;
-  if (splits > nk) splits = nk;
+  splits = split_count(g.K, splits);
   g.kchunks = static_cast<int>(ceil_div(nk, splits));
-  splits = static_cast<int>(ceil_div(nk, g.kchunks));
   g.m_tiles = static_cast<int>(ceil_div(g.M, BM));
   g.n_tiles = static_cast<int>(ceil_div(g.N, BN));
   g.splits = splits;
@@ -922,13 +975,6 @@ static int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
   hipLaunchKernelGGL((k_gemm_nt<EPI, AF32>), dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, g);
   MV_LAUNCHED("k_gemm_nt");
   return MV_OK;
-}
-
-// K splits for a weight-gradient GEMM: about 256 workgroups in total.
-static int grad_splits(int M, int N, int K) {
-  const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
-  int sp = static_cast<int>(256 / (tiles > 0 ? tiles : 1));
-  return sp < 1 ? 1 : sp;
 }
 
 static int transpose(const bf16_t* in, int rows, int cols, int64_t ldi, bf16_t* out, int64_t ldo, hipStream_t s) {
@@ -1047,8 +1093,10 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   float* gb2 = gW2 + static_cast<int64_t>(D.h2) * D.h1;
   float* gW3 = gb2 + D.h2;
   float* gb3 = gW3 + static_cast<int64_t>(D.dout) * D.h2;
-  hipLaunchKernelGGL(k_fill_f32, dim3(static_cast<unsigned>(ceil_div(np, 256))), dim3(256), 0, s, b.grad, np, 0.f);
-  MV_LAUNCHED("k_fill_f32");
+  // b.grad's bias and output-layer entries receive atomic sums and must be
+  // zero here (the caller clears them); the W1 / W2 gradients go to the
+  // K-split partial slices b.part1 / b.part2 (summed by k_adam_blob or
+  // k_reduce_parts in split order).
   {
     const dim3 grid(static_cast<unsigned>(ceil_div(P, 4 * HEAD_PPW)));
     const bool narrow = D.h2 <= 512;
@@ -1078,9 +1126,11 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   g.K = P;
   g.lda = P;
   g.ldb = P;
-  g.C = gW2;
+  g.C = b.part2;
   g.ldc = D.h1;
-  if (int rc = gemm<EPI_F32_ADD>(g, s, grad_splits(g.M, g.N, g.K))) return rc;
+  g.split_stride = static_cast<int64_t>(D.h2) * D.h1;
+  if (int rc = gemm<EPI_F32>(g, s, b.s2)) return rc;
+  (void)gW2;
   // dA1 (P x h1) = (dA2 W2) * gelu'(A1) = dA2 (W2^T)^T
   g = GemmArgs{};
   g.A = b.dA2;
@@ -1108,37 +1158,46 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   g.K = P;
   g.lda = P;
   g.ldb = P;
-  g.C = gW1;
+  g.C = b.part1;
   g.ldc = D.dp;
-  return gemm<EPI_F32_ADD>(g, s, grad_splits(g.M, g.N, g.K));
+  g.split_stride = static_cast<int64_t>(D.h1) * D.dp;
+  return gemm<EPI_F32>(g, s, b.s1);
 }
 
-static int adam_all(const Params& W, const Dims& D, const float* grad, int64_t np, float lr, int step, hipStream_t s) {
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float c1 = 1.f - powf(b1, static_cast<float>(step));
-  const float c2 = 1.f - powf(b2, static_cast<float>(step));
-  // W1, b1 (BF16 copy of W1)
+static AdamBlob adam_blob(const Params& W, const Dims& D, const Batch& b, int64_t np) {
   const int64_t nW1 = static_cast<int64_t>(D.h1) * D.dp;
   const int64_t nW2 = static_cast<int64_t>(D.h2) * D.h1;
-  const int64_t o_b1 = nW1, o_W2 = o_b1 + D.h1, o_b2 = o_W2 + nW2, o_W3 = o_b2 + D.h2;
   AdamBlob ab;
   ab.p = W.W1;
-  ab.g = grad;
+  ab.g = b.grad;
+  ab.part1 = b.part1;
+  ab.part2 = b.part2;
+  ab.st1 = nW1;
+  ab.st2 = nW2;
+  ab.s1 = b.s1;
+  ab.s2 = b.s2;
   ab.m = W.m;
   ab.v = W.v;
   ab.np = np;
-  ab.o_b1 = o_b1;
-  ab.o_W2 = o_W2;
-  ab.o_b2 = o_b2;
+  ab.o_b1 = nW1;
+  ab.o_W2 = nW1 + D.h1;
+  ab.o_b2 = ab.o_W2 + nW2;
   ab.h1 = D.h1;
   ab.h2 = D.h2;
   ab.W1b = W.W1b;
   ab.W2b = W.W2b;
   ab.W2tb = W.W2tb;
+  return ab;
+}
+
+static int adam_all(const Params& W, const Dims& D, const Batch& bt, int64_t np, float lr, int step, hipStream_t s) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.f - powf(b1, static_cast<float>(step));
+  const float c2 = 1.f - powf(b2, static_cast<float>(step));
+  const AdamBlob ab = adam_blob(W, D, bt, np);
   hipLaunchKernelGGL(k_adam_blob, dim3(static_cast<unsigned>(ceil_div(np, 256))), dim3(256), 0, s, ab, lr, b1, b2,
                      eps, c1, c2);
   MV_LAUNCHED("k_adam_blob");
-  (void)o_W3;
   return MV_OK;
 }
 
@@ -1259,7 +1318,15 @@ int mv_mlp_grad(const void* blob, int din, int h1, int h2, int dout, const float
   hipLaunchKernelGGL(k_loss_grad, dim3(static_cast<unsigned>(ceil_div(B, 256))), dim3(256), 0, s, bt.Z, bt.Yb, B, dout,
                      neg, a, b, lam, bt.dZ, bt.loss);
   MV_LAUNCHED("k_loss_grad");
+  hipLaunchKernelGGL(k_fill_f32, dim3(static_cast<unsigned>(ceil_div(np, 256))), dim3(256), 0, s, bt.grad, np, 0.f);
+  MV_LAUNCHED("k_fill_f32");
   if (int rc = backward(W, D, bt, P, np, s)) return rc;
+  {
+    const AdamBlob ab = adam_blob(W, D, bt, np);
+    const int64_t nw = ab.o_b1 + (ab.o_b2 - ab.o_W2);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(static_cast<unsigned>(ceil_div(nw, 256))), dim3(256), 0, s, ab);
+    MV_LAUNCHED("k_reduce_parts");
+  }
   if (grad_out) MV_TRY_HIP(hipMemcpyAsync(grad_out, bt.grad, sizeof(float) * np, hipMemcpyDeviceToDevice, s));
   if (loss_out) MV_TRY_HIP(hipMemcpyAsync(loss_out, bt.loss, sizeof(float) * 2, hipMemcpyDeviceToDevice, s));
   return MV_OK;
@@ -1292,21 +1359,25 @@ int mv_mlp_train(void* blob, int din, int h1, int h2, int dout, const float* X, 
   bf16_t* Xbf = reinterpret_cast<bf16_t*>(static_cast<char*>(ws) + need);
   hipLaunchKernelGGL(k_to_bf16, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, X, n, din, D.dp, Xbf);
   MV_LAUNCHED("k_to_bf16");
+  // the gradient blob starts at zero; k_adam_blob clears what it consumed
+  hipLaunchKernelGGL(k_fill_f32, dim3(static_cast<unsigned>(ceil_div(np, 256))), dim3(256), 0, s, bt.grad, np, 0.f);
+  MV_LAUNCHED("k_fill_f32");
+  if (loss_trace && steps > 0) {
+    const int64_t nl = 2 * static_cast<int64_t>(steps);
+    hipLaunchKernelGGL(k_fill_f32, dim3(static_cast<unsigned>(ceil_div(nl, 256))), dim3(256), 0, s, loss_trace, nl, 0.f);
+    MV_LAUNCHED("k_fill_f32");
+  }
   for (int t = 0; t < steps; ++t) {
     hipLaunchKernelGGL(k_gather_x, dim3(static_cast<unsigned>(ceil_div(P, 4))), dim3(256), 0, s, Xbf, D.dp,
                        idx + static_cast<int64_t>(t) * P, P, bt.Xb, Yp, dout, bt.Yb);
     MV_LAUNCHED("k_gather_x");
     if (int rc = forward(W, D, bt.Xb, P, bt.A1, bt.H1, bt.H1t, bt.A2, bt.H2, bt.Z, s)) return rc;
     float* lo = loss_trace ? loss_trace + 2 * static_cast<int64_t>(t) : nullptr;
-    if (lo) {
-      hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, lo, static_cast<int64_t>(2), 0.f);
-      MV_LAUNCHED("k_fill_f32");
-    }
     hipLaunchKernelGGL(k_loss_grad, dim3(static_cast<unsigned>(ceil_div(B, 256))), dim3(256), 0, s, bt.Z, bt.Yb, B,
                        dout, neg, a, b, lam, bt.dZ, lo);
     MV_LAUNCHED("k_loss_grad");
     if (int rc = backward(W, D, bt, P, np, s)) return rc;
-    if (int rc = adam_all(W, D, bt.grad, np, lr, step0 + t + 1, s)) return rc;
+    if (int rc = adam_all(W, D, bt, np, lr, step0 + t + 1, s)) return rc;
   }
   return MV_OK;
 }
