@@ -130,7 +130,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 def learned_form(Xd, um, fresh_n, reps=5):
     """Trains the MLP on the fitted UMAP (graph + embedding of the 70K set), maps
     `fresh_n` fresh points of the same mixture, times a plain layout of them,
-    and judges both against the exact 15-NN of the fresh points; points below
+    and scores both against the exact 15-NN of the fresh points; points below
     0.8 preservation get 50 plain epochs."""
     import numpy as np
     import torch
@@ -189,7 +189,7 @@ def learned_form(Xd, um, fresh_n, reps=5):
     t = time.perf_counter()
     hi_idx, hi_dist = knn_brute_device(Xfd, 15)
     torch.cuda.synchronize()
-    out["judge_exact_knn_s"] = round(time.perf_counter() - t, 3)
+    out["eval_exact_knn_s"] = round(time.perf_counter() - t, 3)
 
     def pres(Zd):
         lo, _ = knn_brute_device(Zd.contiguous(), 15)
@@ -199,7 +199,7 @@ def learned_form(Xd, um, fresh_n, reps=5):
     t = time.perf_counter()
     c_mlp = pres(Z)
     torch.cuda.synchronize()
-    out["judge_lowd_knn_s"] = round(time.perf_counter() - t, 3)
+    out["eval_lowd_knn_s"] = round(time.perf_counter() - t, 3)
     c_plain = pres(torch.from_numpy(z_plain).cuda())
     p_mlp = float(c_mlp.double().mean().item()) / 15
     p_plain = float(c_plain.double().mean().item()) / 15

@@ -12,7 +12,7 @@ proportion to their weights: the UMAP edge cross-entropy (the layout's
 gradient law, umap.py:367-380, without the clip; in-batch tails as
 negatives) plus a regression of both ends to the plain embedding.
 ``transform`` maps new points in one pass. ``refine`` is the plain layout as
-judge and fallback: points whose 15-NN preservation against the exact high-D
+check and fallback: points whose 15-NN preservation against the exact high-D
 kNN falls below a threshold are moved by plain SGD epochs (others fixed).
 
 All arithmetic runs in libmivis (mlp.hip, umap.hip, knn.hip); torch only
@@ -233,10 +233,10 @@ class ParametricUMAP:
         self.fit(X, umap)
         return self.transform(X)
 
-    # -- judge and fallback ------------------------------------------------------
+    # -- check and fallback ------------------------------------------------------
     def refine_device(self, Xd: torch.Tensor, Zd: torch.Tensor, threshold: float = 0.8, n_epochs: int = 50,
                       knn=None, learning_rate: float = 1.0, knn_dtype: str = "fp32"):
-        """Judges each point of the embedding Zd of Xd by its k-NN preservation
+        """Scores each point of the embedding Zd of Xd by its k-NN preservation
         (k = n_neighbors) against the exact high-D kNN and moves the points below
         `threshold` by `n_epochs` plain layout epochs on the exact-kNN graph; the
         others stay fixed. Returns (refined Z, per-point counts before)."""

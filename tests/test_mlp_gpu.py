@@ -1,5 +1,5 @@
 """Learned form on the GPU: forward, gradients and Adam against the CPU oracle
-(oracle/mlp.py), edge sampling, the preservation judge, masked refinement, and
+(oracle/mlp.py), edge sampling, the preservation check, masked refinement, and
 what the trained network gains."""
 
 import ctypes
