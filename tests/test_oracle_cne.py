@@ -69,3 +69,22 @@ def test_build_edges_is_the_symmetric_edge_set():
     assert len(e) == len(want)
     keys = e[:, 0].astype(np.int64) * n + e[:, 1]
     assert (np.diff(keys) > 0).all(), "sorted by (head, tail)"
+
+
+def test_adam_cne_matches_torch_adam():
+    """oracle adam_cne (cne.py:355-360) against torch.optim.Adam on the same
+    gradient sequence (bias-corrected, eps outside the square root)."""
+    import torch
+
+    rng = np.random.default_rng(21)
+    Y0 = rng.standard_normal((30, 2))
+    grads = [rng.standard_normal((30, 2)) for _ in range(6)]
+    p = torch.nn.Parameter(torch.tensor(Y0, dtype=torch.float64))
+    opt = torch.optim.Adam([p], lr=0.05, betas=(0.9, 0.999), eps=1e-8)
+    Y, m, v = Y0.copy(), 0.0, 0.0
+    for it, g in enumerate(grads, start=1):
+        opt.zero_grad()
+        p.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        Y, m, v = oc.adam_cne(Y, g, m, v, it, 0.05)
+    np.testing.assert_allclose(Y, p.detach().numpy(), rtol=0, atol=1e-12)

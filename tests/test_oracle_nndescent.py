@@ -47,3 +47,28 @@ def test_oracle_invariants_and_recall():
     # when it leaves the full path before its iteration-4 FP32 recompute.
     d2 = ((X[:, None, :] - X[idx]) ** 2).sum(-1)
     np.testing.assert_allclose(dist, np.sqrt(d2), rtol=2e-2, atol=0.1)
+
+
+def test_gather_dists_exact_on_integer_points():
+    """oracle gather_dists (nndescent.py:573-641) against pairwise squared
+    distances computed in exact integer arithmetic: with small-integer
+    coordinates every dot product is exact in FP32 and in FP16, so both the
+    FP32 and the FP16-dot variant must return the exact values; on general
+    floats the FP32 variant stays within FP32 rounding of the FP64 values."""
+    rng = np.random.default_rng(13)
+    n, d, c = 300, 8, 7
+    Xi = rng.integers(0, 4, size=(n, d))
+    X = Xi.astype(np.float32)
+    sq = (X * X).sum(1).astype(np.float32)
+    ids = rng.integers(0, n, size=(n, c))
+    exact = ((Xi[:, None, :] - Xi[ids]) ** 2).sum(-1).astype(np.float32)
+    np.testing.assert_array_equal(onn.gather_dists(X, sq, ids), exact)
+    np.testing.assert_array_equal(onn.gather_dists(X, sq, ids, half=True), exact)
+    rows = np.array([5, 0, 299, 17])
+    want_rows = ((Xi[rows][:, None, :] - Xi[ids[:4]]) ** 2).sum(-1).astype(np.float32)
+    np.testing.assert_array_equal(onn.gather_dists(X, sq, ids[:4], rows=rows), want_rows)
+    Xf = rng.standard_normal((n, 16)).astype(np.float32)
+    sqf = (Xf * Xf).sum(1).astype(np.float32)
+    ref = ((Xf[:, None, :].astype(np.float64) - Xf[ids].astype(np.float64)) ** 2).sum(-1)
+    got = onn.gather_dists(Xf, sqf, ids)
+    assert np.all(np.abs(got - ref) <= 1e-5 * (sqf[:, None] + sqf[ids]) + 1e-6)
