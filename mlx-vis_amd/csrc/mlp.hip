@@ -608,7 +608,8 @@ __global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ Z, 
 }
 
 constexpr int HMAX = 1024;        // widest hidden layer
-constexpr int HEAD_PPW = 4;       // points per wavefront in k_head_bwd
+constexpr int HEAD_PPW = 4;       // points per wavefront group in k_head_bwd
+constexpr int HEAD_G = 2;         // groups per wavefront (fewer blocks: fewer gradient atomics)
 
 // Output-layer backward for DOUT outputs. Per point p:
 //   dA2[p, c] = (sum_o dZ[p, o] W3[o, c]) * gelu'(A2pre[p, c])   (stored BF16)
@@ -646,42 +647,44 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dZ, 
 #pragma unroll
     for (int o = 0; o < DOUT; ++o) w3r[o][j] = c < h2 ? W3[o * h2 + c] : 0.f;
   }
-  // the wavefront's HEAD_PPW points: every load first (one memory latency),
-  // then the arithmetic
-  const int p0 = (blockIdx.x * 4 + w) * HEAD_PPW;
-  float dzs[HEAD_PPW][DOUT], a2v[HEAD_PPW][JM], hv[HEAD_PPW][JM];
-#pragma unroll
-  for (int i = 0; i < HEAD_PPW; ++i) {
-    const int p = min(p0 + i, P - 1);
-#pragma unroll
-    for (int o = 0; o < DOUT; ++o) dzs[i][o] = dZ[static_cast<int64_t>(p) * DOUT + o];
-#pragma unroll
-    for (int j = 0; j < JM; ++j) {
-      const int c = l + 64 * j;
-      const int64_t at = static_cast<int64_t>(p) * h2 + (c < h2 ? c : 0);
-      a2v[i][j] = bf2f(A2pre[at]);
-      hv[i][j] = bf2f(H2[at]);
+  for (int gi = 0; gi < HEAD_G; ++gi) {
+    // the wavefront's HEAD_PPW points: every load first (one memory latency),
+    // then the arithmetic
+    const int p0 = ((blockIdx.x * HEAD_G + gi) * 4 + w) * HEAD_PPW;
+    float dzs[HEAD_PPW][DOUT], a2v[HEAD_PPW][JM], hv[HEAD_PPW][JM];
+  #pragma unroll
+    for (int i = 0; i < HEAD_PPW; ++i) {
+      const int p = min(p0 + i, P - 1);
+  #pragma unroll
+      for (int o = 0; o < DOUT; ++o) dzs[i][o] = dZ[static_cast<int64_t>(p) * DOUT + o];
+  #pragma unroll
+      for (int j = 0; j < JM; ++j) {
+        const int c = l + 64 * j;
+        const int64_t at = static_cast<int64_t>(p) * h2 + (c < h2 ? c : 0);
+        a2v[i][j] = bf2f(A2pre[at]);
+        hv[i][j] = bf2f(H2[at]);
+      }
     }
-  }
-#pragma unroll
-  for (int i = 0; i < HEAD_PPW; ++i) {
-    const int p = p0 + i;
-    if (p >= P) break;
-#pragma unroll
-    for (int o = 0; o < DOUT; ++o) pb3[o] += dzs[i][o];
-#pragma unroll
-    for (int j = 0; j < JM; ++j) {
-      const int c = l + 64 * j;
-      if (c < h2) {
-        float s = 0.f;
-#pragma unroll
-        for (int o = 0; o < DOUT; ++o) s = fmaf(dzs[i][o], w3r[o][j], s);
-        const int64_t at = static_cast<int64_t>(p) * h2 + c;
-        const bf16_t g = f2bf(s * gelu_grad(a2v[i][j]));
-        dA2[at] = g;
-        pb2[j] += bf2f(g);
-#pragma unroll
-        for (int o = 0; o < DOUT; ++o) pw3[o][j] = fmaf(dzs[i][o], hv[i][j], pw3[o][j]);
+  #pragma unroll
+    for (int i = 0; i < HEAD_PPW; ++i) {
+      const int p = p0 + i;
+      if (p >= P) break;
+  #pragma unroll
+      for (int o = 0; o < DOUT; ++o) pb3[o] += dzs[i][o];
+  #pragma unroll
+      for (int j = 0; j < JM; ++j) {
+        const int c = l + 64 * j;
+        if (c < h2) {
+          float s = 0.f;
+  #pragma unroll
+          for (int o = 0; o < DOUT; ++o) s = fmaf(dzs[i][o], w3r[o][j], s);
+          const int64_t at = static_cast<int64_t>(p) * h2 + c;
+          const bf16_t g = f2bf(s * gelu_grad(a2v[i][j]));
+          dA2[at] = g;
+          pb2[j] += bf2f(g);
+  #pragma unroll
+          for (int o = 0; o < DOUT; ++o) pw3[o][j] = fmaf(dzs[i][o], hv[i][j], pw3[o][j]);
+        }
       }
     }
   }
@@ -1098,7 +1101,7 @@ static int backward(const Params& W, const Dims& D, const Batch& b, int P, int64
   // K-split partial slices b.part1 / b.part2 (summed by k_adam_blob or
   // k_reduce_parts in split order).
   {
-    const dim3 grid(static_cast<unsigned>(ceil_div(P, 4 * HEAD_PPW)));
+    const dim3 grid(static_cast<unsigned>(ceil_div(P, 4 * HEAD_PPW * HEAD_G)));
     const bool narrow = D.h2 <= 512;
     switch (D.dout) {
       case 1: if (narrow) hipLaunchKernelGGL((k_head_bwd<1, 8>), grid, dim3(256), 0, s, b.dZ, b.H2, b.A2, P, D.h2, W.W3, b.dA2, gW3, gb3, gb2);
