@@ -501,7 +501,22 @@ def main():
                      "frac": round(screen_tflops / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": screen_traffic(n, d, kern)},
     }
     if not args.no_extras:
-        prec = {}
+        from mlx_vis_amd import knn_brute_device
+        # relative error of the returned Euclidean distances against FP64
+        # distances of the same pairs, on a fixed sample of 2,000 rows
+        err_rows = np.random.default_rng(5).choice(n, size=min(2000, n), replace=False)
+        X64 = Xd.index_select(0, torch.as_tensor(err_rows, device=Xd.device)).double().cpu().numpy()
+        Xall64 = Xd.double().cpu().numpy()
+
+        def dist_err(dtype, rerank):
+            idx_e, dist_e = knn_brute_device(Xd, 15, dtype=dtype, rerank=rerank)
+            ii = idx_e.cpu().numpy()[err_rows]
+            dd = dist_e.cpu().numpy()[err_rows].astype(np.float64)
+            d64 = np.sqrt(((X64[:, None, :] - Xall64[ii]) ** 2).sum(-1))
+            rel = np.abs(dd - d64) / np.maximum(d64, 1e-30)
+            return {"max": float(f"{rel.max():.3e}"), "mean": float(f"{rel.mean():.3e}"), "rows": int(err_rows.size)}
+
+        prec = {"fp32_dist_rel_err_vs_fp64": dist_err("fp32", 2)}
         for dt, rr in (("bf16x3", 2), ("bf16", 2), ("bf16", 4), ("fp8", 2), ("fp8", 4)):
             log(f"kNN {dt} screen, rerank {rr}")
             ph_dt, st_dt, idx_dt, _ = knn_phase_timing(Xd, args.knn_reps, dt, rr)
@@ -510,7 +525,7 @@ def main():
                         "speedup_vs_fp32": round(knn_total_ms / tot, 3),
                         "identical_indices": bool(torch.equal(idx_dt, ref_idx)),
                         "rows_uncertified_first_pass": st_dt[2], "rows_uncertified_after_rescreen": st_dt[3],
-                        "rows_fp64_scan": st_dt[0]}
+                        "rows_fp64_scan": st_dt[0], "dist_rel_err_vs_fp64": dist_err(dt, rr)}
         # the whole fit_transform with the split-BF16 screen (same certified kNN)
         def step_x3():
             um3 = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=args.epochs, negative_sample_rate=5,
