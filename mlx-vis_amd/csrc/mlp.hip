@@ -378,20 +378,27 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
       }
     }
     if (g.ct) {
-      // transposed copy: 128 columns x 16 segments of 8 rows; lanes take
-      // consecutive columns (conflict-free LDS reads down a column)
+      // transposed copy: 64 column pairs x 16 segments of 8 rows; a lane
+      // reads its pair of columns 4 bytes per row (consecutive lanes,
+      // consecutive words: conflict-free) and splits them into the two
+      // columns' 16-byte row segments (N is even here, so col + 1 < N)
 #pragma unroll
-      for (int j = 0; j < (BM * BN / 8) / NT; ++j) {
+      for (int j = 0; j < (BM * BN / 16) / NT; ++j) {
         const int sgi = tid + NT * j;
-        const int lcol = sgi & (BN - 1), lrow = (sgi / BN) * 8;
+        const int lcol = (sgi & (BN / 2 - 1)) * 2, lrow = (sgi / (BN / 2)) * 8;
         const int row = bm + lrow, col = bn + lcol;
         if (row < g.M && col < g.N) {
-          uint32_t w[4];
+          uint32_t x[8];
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            w[e] = static_cast<uint32_t>(s_out[(lrow + 2 * e) * TS + lcol]) |
-                   (static_cast<uint32_t>(s_out[(lrow + 2 * e + 1) * TS + lcol]) << 16);
-          *reinterpret_cast<uint4*>(g.ct + static_cast<int64_t>(col) * g.ldct + row) = make_uint4(w[0], w[1], w[2], w[3]);
+          for (int e = 0; e < 8; ++e) x[e] = *reinterpret_cast<const uint32_t*>(s_out + (lrow + e) * TS + lcol);
+          uint32_t w0[4], w1[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            w0[e] = (x[2 * e] & 0xFFFFu) | (x[2 * e + 1] << 16);
+            w1[e] = (x[2 * e] >> 16) | (x[2 * e + 1] & 0xFFFF0000u);
+          }
+          *reinterpret_cast<uint4*>(g.ct + static_cast<int64_t>(col) * g.ldct + row) = make_uint4(w0[0], w0[1], w0[2], w0[3]);
+          *reinterpret_cast<uint4*>(g.ct + static_cast<int64_t>(col + 1) * g.ldct + row) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
         }
       }
     }
