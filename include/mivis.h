@@ -170,8 +170,9 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
 
 /* In-edge lists of the kept slots (eps > 0), grouped by tail node: entries
  * t < (number of kept slots) hold the slot in_slots[t] and its tail
- * in_node[t]; nodes appear in ascending order, slots of one node in no fixed
- * order. Used by mv_umap_epochs to sum tail updates without per-edge atomics.
+ * in_node[t]; nodes appear in ascending order, the slots of one node in
+ * ascending order. Used by mv_umap_epochs to sum tail updates without per-edge
+ * atomics.
  * ws: 3 * align256(4n) + align256(4 * ceil(n / 1024)) bytes. */
 int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, int32_t* in_slots,
                      int32_t* in_node, void* ws, size_t ws_bytes, void* stream);

@@ -126,21 +126,27 @@ __global__ __launch_bounds__(256) void k_cne_grad(CneArgs a) {
     }
 #undef MV_NEG
   }
-  // head terms: lanes with equal heads (adjacent in a head-sorted list) are
-  // summed first, one atomic per run of equal heads
+  // head terms: each run of adjacent lanes with the same head (a head-sorted
+  // list makes runs long; mini-batch draws leave mostly runs of one) is summed
+  // by a segmented suffix scan and added with one atomic by the run's first
+  // lane. A lane only takes a farther partial sum while no run ends between
+  // the two lanes, so equal heads in separate runs are never merged.
   const int l = threadIdx.x & 63;
   const int32_t key = live ? i : -1 - l;   // dead lanes never match
   const int32_t prev = __shfl_up(key, 1);
+  const int32_t next = __shfl_down(key, 1);
   const bool start = l == 0 || prev != key;
+  const bool run_end = l == 63 || next != key;
 #pragma unroll
   for (int q = 0; q < D; ++q) {
     float v = gh[q];
-    // inclusive segmented suffix sum towards the run's first lane
+    bool closed = run_end;   // a run ends within [l, l + off)
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const float o = __shfl_down(v, off);
-      const int32_t ok = __shfl_down(key, off);
-      if (l + off < 64 && ok == key) v += o;
+      const bool oc = __shfl_down(static_cast<int>(closed), off) != 0;
+      if (l + off < 64 && !closed) v += o;
+      if (l + off < 64) closed = closed || oc;
     }
     gh[q] = v;
   }

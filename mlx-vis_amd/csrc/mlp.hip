@@ -813,10 +813,24 @@ __global__ __launch_bounds__(256) void k_sample_edges(const float* __restrict__ 
           best = e;
         }
       }
-      if (best < 0) best = e;
     }
   }
   if (pick < 0) pick = best;
+  if (pick < 0) {
+    // none of the 64 draws hit a kept slot (a heavily pruned graph): probe
+    // forward from a drawn slot to the next kept one. The maximum-weight slot
+    // is always kept, so the probe ends within nk steps unless the graph has
+    // no kept slot at all (then slot 0 is returned).
+    const uint4 r = Philox::gen(make_uint4(static_cast<uint32_t>(j), step0 + static_cast<uint32_t>(t), 0xFFFFu,
+                                           0x4D4C5030u),
+                                seed, ~seed);
+    const int64_t e0 = uniform_below(r.x, static_cast<uint32_t>(nk));
+    for (int64_t q = 0; q < nk && pick < 0; ++q) {
+      const int64_t e = e0 + q < nk ? e0 + q : e0 + q - nk;
+      if (eps[e] >= 1.f) pick = e;
+    }
+    if (pick < 0) pick = 0;
+  }
   int32_t* o = out + static_cast<int64_t>(t) * 2 * B;
   o[j] = static_cast<int32_t>(pick / k);
   o[B + j] = tail[pick];

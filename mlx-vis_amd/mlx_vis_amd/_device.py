@@ -36,6 +36,23 @@ def require_device(what: str, *tensors) -> None:
             raise ValueError(f"{what}: expected device (cuda) tensors, got {getattr(t, 'device', type(t))}")
 
 
+def f32_rows(t: torch.Tensor) -> torch.Tensor:
+    """The float32, row-major contiguous form of a device tensor (a copy only
+    when the dtype or the strides differ): the kernels read raw row-major
+    float32 memory, so a half-precision tensor or a strided view must never
+    reach them as it is."""
+    if t.dtype != torch.float32:
+        t = t.float()
+    return t.contiguous()
+
+
+def i32_vec(t: torch.Tensor | None) -> torch.Tensor | None:
+    """The int32 contiguous form of an index tensor (None passes through)."""
+    if t is None:
+        return None
+    return t.to(torch.int32).contiguous()
+
+
 def workspace(nbytes: int) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
 

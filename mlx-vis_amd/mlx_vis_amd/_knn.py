@@ -34,7 +34,8 @@ def knn_brute_device(
 ):
     """Device-level brute kNN.
 
-    X: (n, d) float32 CUDA tensor. Q: (nq, d) queries (default X itself).
+    X: (n, d) CUDA tensor. Q: (nq, d) queries (default X itself). Other
+    dtypes and strided views are converted to contiguous float32 first.
     Self exclusion: with ``Q is None`` the query row i never returns i (the
     +1e30 diagonal of _knn.py:64-69); ``q_self`` (nq,) int32 names the row of X
     each query must not return (-1 for none).
@@ -44,10 +45,10 @@ def knn_brute_device(
         raise ValueError(f"unknown kNN dtype {dtype!r}; expected one of {sorted(_capi.DTYPES)}")
     _device.require_device("knn_brute_device", X, Q, stats)
     same = Q is None
-    if same:
-        Q = X
-    if X.dim() != 2 or Q.dim() != 2 or X.shape[1] != Q.shape[1]:
-        raise ValueError(f"bad shapes X {tuple(X.shape)} Q {tuple(Q.shape)}")
+    if X.dim() != 2 or (not same and (Q.dim() != 2 or X.shape[1] != Q.shape[1])):
+        raise ValueError(f"bad shapes X {tuple(X.shape)} Q {None if same else tuple(Q.shape)}")
+    X = _device.f32_rows(X)
+    Q = X if same else _device.f32_rows(Q)
     n, d = X.shape
     nq = Q.shape[0]
     if exclude_self is None:

@@ -54,8 +54,9 @@ def trust_excess_device(X: torch.Tensor, lo: torch.Tensor, rows: torch.Tensor | 
         raise ValueError("lo has more rows than X")
     if not 1 <= k <= 64:
         raise ValueError("trustworthiness supports 1 <= k <= 64")
-    X = X.float().contiguous()
-    lo = lo.to(torch.int32).contiguous()
+    X = _device.f32_rows(X)
+    lo = _device.i32_vec(lo)
+    rows = _device.i32_vec(rows)
     out = torch.empty(m, dtype=torch.int64, device=X.device)
     need = ctypes.c_size_t()
     _capi.call("mv_trust_ws_bytes", m, k, ctypes.byref(need))

@@ -65,6 +65,9 @@ class NNDescent:
     def build_device(self, Xd: torch.Tensor):
         """Device-level build: returns (indices int32, Euclidean distances float32) CUDA tensors."""
         _device.require_device("NNDescent.build_device", Xd)
+        if Xd.dim() != 2:
+            raise ValueError(f"X must be (n, d), got shape {tuple(Xd.shape)}")
+        Xd = _device.f32_rows(Xd)
         n, d = Xd.shape
         k = min(self.k, n - 1)
         if k < 1:

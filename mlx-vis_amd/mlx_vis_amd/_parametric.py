@@ -210,6 +210,9 @@ class ParametricUMAP:
         if self.blob_ is None:
             raise RuntimeError("ParametricUMAP is not fitted")
         _device.require_device("ParametricUMAP.transform_device", Xd, ws)
+        if Xd.dim() != 2:
+            raise ValueError(f"X must be (m, din), got shape {tuple(Xd.shape)}")
+        Xd = _device.f32_rows(Xd)
         m, din = Xd.shape
         if din != self.din_:
             raise ValueError(f"expected {self.din_} features, got {din}")
@@ -241,6 +244,7 @@ class ParametricUMAP:
         `threshold` by `n_epochs` plain layout epochs on the exact-kNN graph; the
         others stay fixed. Returns (refined Z, per-point counts before)."""
         _device.require_device("ParametricUMAP.refine_device", Xd, Zd)
+        Xd, Zd = _device.f32_rows(Xd), _device.f32_rows(Zd)
         k = self.n_neighbors
         if knn is None:
             knn = knn_brute_device(Xd, k, dtype=knn_dtype, euclid=True)

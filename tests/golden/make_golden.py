@@ -11,16 +11,40 @@ libraries independent of this repository's oracle and kernels:
 - umap_ab.json: a, b of the UMAP curve 1 / (1 + a x^(2b)) for (spread,
   min_dist) pairs, fitted by scipy.optimize.curve_fit on the 300 points of
   umap.py:272-296.
+- knn_c1_sample2000.npz: exact k=15 neighbours (indices, squared Euclidean
+  distances, self excluded) of 2,000 sampled rows of the 70,000 x 784
+  10-Gaussian mixture (mlx_vis_amd.datasets.gaussian_mixture_784, seed 1;
+  rows = sorted default_rng(123).choice(70000, 2000)) against all 70,000
+  rows, from scikit-learn's brute-force NearestNeighbors in float64.
+- nnd_c1.npz, nnd_c3.npz: the NNDescent oracle (oracle/nndescent.py, the
+  NumPy restatement of mlx_vis/_nndescent/nndescent.py with the reference's
+  NumPy random stream) on the 70,000 x 784 mixture (k=15, n_iters=12,
+  max_candidates=30, random_state=0) and on the 1,000,000 x 128 mixture
+  (gaussian_mixture_128, seed 2; k=15, n_iters=12, random_state=42): its
+  per-iteration changed counts, its neighbours on 10,000 sampled rows (sorted
+  default_rng(0).choice(n, 10000)), the exact 15-NN of those rows
+  (oracle.knn.exact_knn) and its recall on them (plus, at 70,000 rows, its
+  recall over all rows). These are the "reference recall" figures an
+  NNDescent on the device must reach.
+
+Usage: ``python tests/golden/make_golden.py [knn_c0 ab knn_c1 nnd_c1 nnd_c3]``
+(no argument: the fast fixtures knn_c0 and ab).
 """
 
 from __future__ import annotations
 
 import json
+import sys
+import time
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+for _p in (ROOT, ROOT / "mlx-vis_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
 
 
 def c0_input() -> np.ndarray:
@@ -51,7 +75,77 @@ def make_ab():
     (HERE / "umap_ab.json").write_text(json.dumps(out, indent=1) + "\n")
 
 
+C1_SAMPLE_SEED = 123
+
+
+def c1_sample_rows(n: int = 70_000, m: int = 2000) -> np.ndarray:
+    return np.sort(np.random.default_rng(C1_SAMPLE_SEED).choice(n, m, replace=False)).astype(np.int64)
+
+
+def make_knn_c1():
+    from sklearn.neighbors import NearestNeighbors
+
+    from mlx_vis_amd.datasets import gaussian_mixture_784
+    from oracle import knn as oknn
+
+    X = gaussian_mixture_784(70_000, seed=1)
+    rows = c1_sample_rows()
+    t = time.time()
+    nn = NearestNeighbors(n_neighbors=16, algorithm="brute", metric="euclidean").fit(X.astype(np.float64))
+    dist, idx = nn.kneighbors(X[rows].astype(np.float64))
+    assert (idx[:, 0] == rows).all() and (dist[:, 0] < 1e-5).all()   # self (zero up to expansion rounding)
+    idx, dist2 = idx[:, 1:].astype(np.int32), (dist[:, 1:] ** 2).astype(np.float64)
+    print(f"knn_c1: scikit-learn {time.time() - t:.1f} s", flush=True)
+    # cross-check: the canonical order of the oracle gives the same neighbours
+    oi, od = oknn.exact_knn(X[rows], X, 15, q_self=rows)
+    assert np.array_equal(oi, idx), int((oi != idx).sum())
+    np.testing.assert_allclose(od, dist2, rtol=1e-9)
+    np.savez_compressed(HERE / "knn_c1_sample2000.npz", rows=rows.astype(np.int32), idx=idx, dist2=dist2)
+
+
+def _nnd_fixture(name, X, seed, n_iters, max_candidates, full_recall):
+    from oracle import knn as oknn
+    from oracle import metrics as om
+    from oracle import nndescent as onn
+
+    n = X.shape[0]
+    rows = np.sort(np.random.default_rng(0).choice(n, 10_000, replace=False)).astype(np.int64)
+    t = time.time()
+    idx, _, trace = onn.nndescent(X, k=15, n_iters=n_iters, max_candidates=max_candidates, random_state=seed,
+                                  return_trace=True)
+    print(f"{name}: oracle NNDescent {time.time() - t:.1f} s, trace {trace}", flush=True)
+    t = time.time()
+    ex_rows, _ = oknn.exact_knn(X[rows], X, 15, q_self=rows)
+    print(f"{name}: exact kNN of the sampled rows {time.time() - t:.1f} s", flush=True)
+    out = dict(rows=rows.astype(np.int32), oracle_idx=idx[rows].astype(np.int32), exact_idx=ex_rows.astype(np.int32),
+               trace=np.asarray(trace, dtype=np.int64), seed=np.int64(seed),
+               recall_rows=np.float64(om.recall(idx[rows], ex_rows)))
+    if full_recall:
+        t = time.time()
+        ex_all, _ = oknn.exact_knn(X, X, 15, self_base=0)
+        out["recall_all"] = np.float64(om.recall(idx, ex_all))
+        print(f"{name}: exact kNN of all rows {time.time() - t:.1f} s", flush=True)
+    print(f"{name}: recall on the sampled rows {float(out['recall_rows']):.5f}"
+          + (f", all rows {float(out['recall_all']):.5f}" if full_recall else ""), flush=True)
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+
+
+def make_nnd_c1():
+    from mlx_vis_amd.datasets import gaussian_mixture_784
+
+    _nnd_fixture("nnd_c1", gaussian_mixture_784(70_000, seed=1), 0, 12, 30, True)
+
+
+def make_nnd_c3():
+    from mlx_vis_amd.datasets import gaussian_mixture_128
+
+    _nnd_fixture("nnd_c3", gaussian_mixture_128(1_000_000, seed=2), 42, 12, None, False)
+
+
+MAKERS = {"knn_c0": make_knn, "ab": make_ab, "knn_c1": make_knn_c1, "nnd_c1": make_nnd_c1, "nnd_c3": make_nnd_c3}
+
+
 if __name__ == "__main__":
-    make_knn()
-    make_ab()
-    print("wrote", sorted(p.name for p in HERE.iterdir() if p.suffix in (".npz", ".json")))
+    for name in sys.argv[1:] or ["knn_c0", "ab"]:
+        MAKERS[name]()
+    print("fixtures:", sorted(p.name for p in HERE.iterdir() if p.suffix in (".npz", ".json")))

@@ -53,6 +53,38 @@ def test_cne_gradient_matches_oracle(loss, dim):
     np.testing.assert_allclose(got, want, rtol=1e-3, atol=1e-5 * np.abs(want).max())
 
 
+@pytest.mark.parametrize("loss", ["infonce", "nce", "neg"])
+def test_cne_minibatch_gradient_matches_oracle(loss):
+    """Mini-batch mode (cne.py:337-339): each lane draws its edge uniformly, so
+    equal heads land in non-adjacent lanes. The host reproduces the kernel's
+    Philox draws (edge: counter (b, iter, 0xFFFF, 0xC0E5), negative s:
+    (b, iter, s, 0xC0E5), first word mapped by floor(r * N / 2^32)) and the
+    oracle's gradient of that edge list must match, scaled by E / batch."""
+    from _philox import philox4x32_10, uniform_below
+
+    capi = _lib()
+    print(f"case minibatch loss={loss}", flush=True)
+    X, edges = _graph(2000, 10, 10, 8)
+    n, E, m, B = X.shape[0], edges.shape[0], 5, 6400
+    seed, it = 0x1234ABCD5678, 7
+    k0, k1 = seed & 0xFFFFFFFF, seed >> 32
+    bidx = np.arange(B)
+    e = uniform_below(philox4x32_10(bidx, it, 0xFFFF, 0xC0E5, k0, k1)[0], E)
+    neg = np.stack([uniform_below(philox4x32_10(bidx, it, s, 0xC0E5, k0, k1)[0], n) for s in range(m)], axis=1)
+    # a batch this size repeats heads in non-adjacent lanes of one wavefront
+    heads = edges[e, 0].reshape(-1, 64)
+    assert any(len(set(w)) < 64 for w in heads)
+    Y = np.random.default_rng(9).normal(0, 2, (n, 2)).astype(np.float32)
+    Yd, eid, ejd = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in (Y, edges[:, 0], edges[:, 1]))
+    g = torch.zeros((n, 2), dtype=torch.float32, device="cuda")
+    scale = E / B
+    capi.check(capi.lib().mv_cne_grad(Yd.data_ptr(), n, 2, eid.data_ptr(), ejd.data_ptr(), E, None, m, LOSSES[loss],
+                                      scale, seed, it, B, g.data_ptr(), None))
+    want = scale * oc.cne_grad(Y, edges[e, 0], edges[e, 1], neg, loss)
+    got = g.cpu().numpy()
+    assert _rel(got, want) < 2e-5, _rel(got, want)
+
+
 @pytest.mark.parametrize("law", list(LAWS))
 def test_pair_laws_match_oracle(law):
     capi = _lib()

@@ -32,3 +32,19 @@ def test_ab_params_match_golden():
         for fn in (ou.find_ab_params, find_ab_params):
             a, b = fn(row["spread"], row["min_dist"])
             np.testing.assert_allclose([a, b], [row["a"], row["b"]], rtol=1e-5)
+
+
+def test_philox_known_answers():
+    """The NumPy Philox-4x32-10 the GPU tests use to reproduce device draws,
+    against the Random123 known-answer vectors (philox4x32, 10 rounds)."""
+    from _philox import philox4x32_10
+
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in kat:
+        got = [int(v) for v in philox4x32_10(*ctr, *key)]
+        assert got == list(want), (ctr, key, [hex(v) for v in got])

@@ -204,3 +204,38 @@ def test_single_query_and_invalid_arguments():
         knn_brute_device(Xd, 5, dtype="int8")
     with pytest.raises(ValueError):
         knn_brute_device(Xd[:3], 3)               # k exceeds the selectable rows
+
+
+def test_device_inputs_of_other_dtype_or_strides_are_normalised():
+    """A float16 tensor, a float64 tensor and non-contiguous views (column
+    slice, every second row) give the neighbours of their contiguous float32
+    form: the device-level entry points convert before any kernel reads raw
+    row-major float32 memory."""
+    from mlx_vis_amd import knn_brute_device
+
+    rng = np.random.default_rng(21)
+    base = rng.standard_normal((1400, 48)).astype(np.float32)
+    full = torch.from_numpy(base).cuda()
+    cases = {
+        "float16": full.half(),
+        "float64": full.double(),
+        "column_slice": full[:, :40],
+        "row_stride": full[::2],
+        "transposed_twice": full.t().contiguous().t(),
+    }
+    for name, t in cases.items():
+        print(f"case {name}", flush=True)
+        ref_in = t.float().contiguous()
+        want_i, want_d = knn_brute_device(ref_in, 10)
+        got_i, got_d = knn_brute_device(t, 10)
+        torch.cuda.synchronize()
+        assert torch.equal(got_i, want_i), name
+        assert torch.equal(got_d, want_d), name
+        Xh = ref_in.cpu().numpy()
+        oi, od = oknn.exact_knn(Xh, Xh, 10, self_base=0)
+        _check(got_i.cpu().numpy(), got_d.cpu().numpy(), oi, od)
+    # queries as a strided view
+    Q = full[1::3, :]
+    got_i, _ = knn_brute_device(full, 7, Q)
+    oi, _ = oknn.exact_knn(base[1::3], base, 7)
+    assert np.array_equal(got_i.cpu().numpy(), oi)
