@@ -60,8 +60,77 @@ def dedup_topk(cands, dists, k):
     return np.take_along_axis(sc, sel, 1).astype(np.int32), np.take_along_axis(sd, sel, 1)
 
 
+def schedule_params(k, mc, uf):
+    """(mc_nn, j_nn) of an iteration — nndescent.py:100-101: how many of a
+    row's neighbours are expanded and how many of each one's neighbours are
+    taken."""
+    sc = min(1.0, uf ** 0.5 * 3)
+    mc_nn = max(3, int(min(k, mc) * sc))
+    j_nn = max(k // 2, min(k, int(k * sc)))
+    return mc_nn, j_nn
+
+
+def converging_step(X, sq, idx, dists, flags, uf, k, mc):
+    """One iteration of the converging path (update_frac < 0.5) —
+    nndescent.py:103-220: NEW neighbours first (stable), mc_new = max(3,
+    int(k * uf)) of them expanded by j_nn of their neighbours, reverse
+    candidates while uf >= 0.10, only rows with a NEW flag (the others keep
+    their row), FP32 full-d distances of the new candidates merged with the
+    STORED distances of the current ones. Returns (idx, dists)."""
+    n = idx.shape[0]
+    _, j_nn = schedule_params(k, mc, uf)
+    new_order = np.argsort((~flags).astype(np.int32), axis=1, kind="stable")
+    new_first = np.take_along_axis(idx, new_order, 1)
+    mc_new = max(3, int(k * uf))
+    active = np.where(flags.any(axis=1))[0]
+    if active.size == 0:
+        # no row changed (reachable only with delta <= 0; the reference's row
+        # loop then has a zero step, nndescent.py:135-136): nothing to update
+        return idx.copy(), dists.copy()
+    use_rev = uf >= 0.10
+    rev = reverse_candidates(idx) if use_rev else None
+    nn = idx[new_first[active, :mc_new].reshape(-1)].reshape(active.size, -1, k)[:, :, :j_nn]
+    nc = nn.reshape(active.size, -1)
+    if use_rev:
+        nc = np.concatenate([nc, rev[active]], axis=1)
+    nd = gather_dists(X, sq, nc, rows=active)
+    ac = np.concatenate([idx[active], nc], axis=1)
+    ad = np.concatenate([dists[active], nd], axis=1)
+    ad = np.where(ac == active[:, None], F32(1e30), ad)
+    ri, rd = dedup_topk(ac, ad, k)
+    new_idx = idx.copy()
+    new_d = dists.copy()
+    new_idx[active] = ri
+    new_d[active] = rd
+    return new_idx, new_d
+
+
+def full_step(Xd, sqd, idx, dists, uf, it, k, mc):
+    """One iteration of the full path (update_frac >= 0.5) —
+    nndescent.py:221-378 without the iteration-4 recompute: every row expands
+    its first mc_nn neighbours by j_nn of theirs, plus reverse candidates
+    while uf >= 0.10; distances in the space (Xd, sqd) given for this
+    iteration, FP16 dots in iterations 1-4, the current neighbours' distances
+    recomputed in iterations 0-4 (stored ones reused after). Returns (idx,
+    dists)."""
+    n = idx.shape[0]
+    mc_nn, j_nn = schedule_params(k, mc, uf)
+    use_half = it in (1, 2, 3, 4)
+    recompute = it == 0 or use_half
+    use_rev = uf >= 0.10
+    nn = idx[idx[:, :mc_nn].reshape(-1)].reshape(n, -1, k)[:, :, :j_nn].reshape(n, -1)
+    nc = np.concatenate([nn, reverse_candidates(idx)], axis=1) if use_rev else nn
+    nd = gather_dists(Xd, sqd, nc, half=use_half)
+    cd = gather_dists(Xd, sqd, idx, half=use_half) if recompute else dists
+    ac = np.concatenate([idx, nc], axis=1)
+    ad = np.concatenate([cd, nd], axis=1)
+    ad = np.where(ac == np.arange(n)[:, None], F32(1e30), ad)
+    return dedup_topk(ac, ad, k)
+
+
 def nndescent(X, k=15, n_iters=20, max_candidates=None, delta=0.015, random_state=None, return_trace=False):
-    """NNDescent.build — nndescent.py:34-403. Returns (idx, Euclidean dists)."""
+    """NNDescent.build — nndescent.py:34-403. Returns (idx, Euclidean dists)
+    (and the changed count of every iteration run with return_trace)."""
     X = np.asarray(X, dtype=F32)
     n, d = X.shape
     k = min(k, n - 1)
@@ -84,34 +153,11 @@ def nndescent(X, k=15, n_iters=20, max_candidates=None, delta=0.015, random_stat
     P2 = (rs.randn(d, p2).astype(F32) / np.sqrt(p2)).astype(F32) if p2 < d else None
     uf = 1.0
     flags = np.ones((n, k), dtype=bool)
-    rows_all = np.arange(n)
     trace = []
     for it in range(n_iters):
-        sc = min(1.0, uf ** 0.5 * 3)
-        mc_nn = max(3, int(min(k, mc) * sc))
-        j_nn = max(k // 2, min(k, int(k * sc)))
         if uf < 0.5:
-            new_order = np.argsort((~flags).astype(np.int32), axis=1, kind="stable")
-            new_first = np.take_along_axis(idx, new_order, 1)
-            mc_new = max(3, int(k * uf))
-            active = np.where(flags.any(axis=1))[0]
-            use_rev = uf >= 0.10
-            rev = reverse_candidates(idx) if use_rev else None
-            nn = idx[new_first[active, :mc_new].reshape(-1)].reshape(active.size, -1, k)[:, :, :j_nn]
-            nc = nn.reshape(active.size, -1)
-            if use_rev:
-                nc = np.concatenate([nc, rev[active]], axis=1)
-            nd = gather_dists(X, sq, nc, rows=active)
-            ac = np.concatenate([idx[active], nc], axis=1)
-            ad = np.concatenate([dists[active], nd], axis=1)
-            ad = np.where(ac == active[:, None], F32(1e30), ad)
-            ri, rd = dedup_topk(ac, ad, k)
-            new_idx = idx.copy()
-            new_d = dists.copy()
-            new_idx[active] = ri
-            new_d[active] = rd
+            new_idx, new_d = converging_step(X, sq, idx, dists, flags, uf, k, mc)
         else:
-            mc_new = mc_nn
             if it == 0:
                 Xd, sqd = X_low, sq_low
             elif it == 1 and P2 is not None:
@@ -119,17 +165,7 @@ def nndescent(X, k=15, n_iters=20, max_candidates=None, delta=0.015, random_stat
                 sqd = np.einsum("ij,ij->i", Xd, Xd).astype(F32)
             else:
                 Xd, sqd = X, sq
-            use_half = it in (1, 2, 3, 4)
-            recompute = it == 0 or use_half
-            use_rev = uf >= 0.10
-            nn = idx[idx[:, :mc_new].reshape(-1)].reshape(n, -1, k)[:, :, :j_nn].reshape(n, -1)
-            nc = np.concatenate([nn, reverse_candidates(idx)], axis=1) if use_rev else nn
-            nd = gather_dists(Xd, sqd, nc, half=use_half)
-            cd = gather_dists(Xd, sqd, idx, half=use_half) if recompute else dists
-            ac = np.concatenate([idx, nc], axis=1)
-            ad = np.concatenate([cd, nd], axis=1)
-            ad = np.where(ac == rows_all[:, None], F32(1e30), ad)
-            new_idx, new_d = dedup_topk(ac, ad, k)
+            new_idx, new_d = full_step(Xd, sqd, idx, dists, uf, it, k, mc)
             if it == 4:
                 new_d = gather_dists(X, sq, new_idx)
         flags = new_idx != idx

@@ -27,7 +27,10 @@ libraries independent of this repository's oracle and kernels:
   recall over all rows). These are the "reference recall" figures an
   NNDescent on the device must reach.
 
-Usage: ``python tests/golden/make_golden.py [knn_c0 ab knn_c1 nnd_c1 nnd_c3]``
+- umap_c0_oracle.json: the oracle's layout of the 10,000 x 128 workload
+  (make_umap_c0): kept edges, 15-NN preservation, trustworthiness.
+
+Usage: ``python tests/golden/make_golden.py [knn_c0 ab knn_c1 nnd_c1 nnd_c3 umap_c0]``
 (no argument: the fast fixtures knn_c0 and ab).
 """
 
@@ -142,7 +145,42 @@ def make_nnd_c3():
     _nnd_fixture("nnd_c3", gaussian_mixture_128(1_000_000, seed=2), 42, 12, None, False)
 
 
-MAKERS = {"knn_c0": make_knn, "ab": make_ab, "knn_c1": make_knn_c1, "nnd_c1": make_nnd_c1, "nnd_c3": make_nnd_c3}
+def make_umap_c0():
+    """The oracle's layout of the small plain-method workload: exact kNN
+    (k = 15), fuzzy set pruned for 200 epochs, a/b for (spread 1, min_dist
+    0.1), init RandomState(0).uniform(-10, 10, (n, 2)) (what UMAP(init="random",
+    random_state=0) draws), 200 epochs with 5 NumPy-drawn negatives per edge
+    (seed 0); its kept-edge count, 15-NN preservation over all rows and
+    trustworthiness (k = 15) over 2,000 sampled rows."""
+    from oracle import knn as oknn
+    from oracle import metrics as om
+    from oracle import umap as ou
+
+    X = c0_input()
+    n, n_epochs = X.shape[0], 200
+    t = time.time()
+    idx, d2 = oknn.exact_knn(X, X, 15, self_base=0)
+    dist = np.sqrt(d2).astype(np.float32)
+    rows, cols, vals = ou.fuzzy_simplicial_set(idx.astype(np.int32), dist, n_epochs)
+    a, b = ou.find_ab_params(1.0, 0.1)
+    Y0 = np.random.RandomState(0).uniform(-10.0, 10.0, size=(n, 2)).astype(np.float32)
+    Y = ou.optimize(rows, cols, vals, Y0, a, b, n_epochs, neg_rate=5, lr=1.0, seed=0)
+    print(f"umap_c0: oracle layout {time.time() - t:.1f} s", flush=True)
+    sample = np.sort(np.random.default_rng(7).choice(n, 2000, replace=False))
+    out = {
+        "workload": "10,000 x 128 N(0,1) seed 0, k=15, 200 epochs, 5 negatives, random init seed 0",
+        "n_edges": int(rows.size),
+        "knn15_preservation": om.knn_preservation(X, Y, 15),
+        "trustworthiness_15": om.trustworthiness(X, Y, 15, rows=sample),
+        "trust_rows_seed": 7,
+        "trust_rows": 2000,
+    }
+    print("umap_c0:", out, flush=True)
+    (HERE / "umap_c0_oracle.json").write_text(json.dumps(out, indent=1) + "\n")
+
+
+MAKERS = {"knn_c0": make_knn, "ab": make_ab, "knn_c1": make_knn_c1, "nnd_c1": make_nnd_c1, "nnd_c3": make_nnd_c3,
+          "umap_c0": make_umap_c0}
 
 
 if __name__ == "__main__":

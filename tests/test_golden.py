@@ -48,3 +48,16 @@ def test_philox_known_answers():
     for ctr, key, want in kat:
         got = [int(v) for v in philox4x32_10(*ctr, *key)]
         assert got == list(want), (ctr, key, [hex(v) for v in got])
+
+
+def test_oracle_knn_matches_golden_c1_sample():
+    """The oracle's canonical order on the 70,000 x 784 mixture (200 of the
+    fixture's 2,000 sampled rows, scikit-learn float64 brute force)."""
+    from mlx_vis_amd.datasets import gaussian_mixture_784
+
+    g = np.load(GOLDEN / "knn_c1_sample2000.npz")
+    X = gaussian_mixture_784(70_000, seed=1)
+    rows = g["rows"][::10].astype(np.int64)
+    oi, od = oknn.exact_knn(X[rows], X, 15, q_self=rows)
+    np.testing.assert_array_equal(oi, g["idx"][::10])
+    np.testing.assert_allclose(od, g["dist2"][::10], rtol=1e-9)

@@ -117,3 +117,45 @@ def test_build_rejects_k_beyond_the_candidate_slots():
     X = _data(500, 8, 9)
     with pytest.raises(ValueError):
         NNDescent(k=32, random_state=0).build(X)   # 32 + 32 * 32 + 32 > 1024 slots
+
+
+def _fixture(name):
+    from pathlib import Path
+
+    return np.load(Path(__file__).resolve().parent / "golden" / f"{name}.npz")
+
+
+@pytest.mark.parametrize("case", ["c1", "c3"])
+def test_full_size_recall_not_below_the_oracle(case):
+    """NNDescent at full size against the oracle's run of the same seed
+    (tests/golden/nnd_{c1,c3}.npz, written by make_golden.py): the 70,000 x
+    784 mixture (k = 15, 12 iterations, max_candidates 30, seed 0) and the
+    1,000,000 x 128 mixture (k = 15, 12 iterations, seed 42). On the 10,000
+    sampled rows the recall against the exact 15-NN must be at least the
+    oracle's minus 0.002, the iteration count within one of the oracle's, and
+    most rows identical to the oracle's."""
+    from mlx_vis_amd import NNDescent
+    from mlx_vis_amd.datasets import gaussian_mixture_128, gaussian_mixture_784
+
+    print(f"case nndescent full size {case}", flush=True)
+    fx = _fixture(f"nnd_{case}")
+    if case == "c1":
+        X = gaussian_mixture_784(70_000, seed=1)
+        nn = NNDescent(k=15, n_iters=12, max_candidates=30, random_state=int(fx["seed"]))
+    else:
+        X = gaussian_mixture_128(1_000_000, seed=2)
+        nn = NNDescent(k=15, n_iters=12, random_state=int(fx["seed"]))
+    gi, gd = nn.build_device(torch.from_numpy(X).cuda())
+    rows = fx["rows"].astype(np.int64)
+    g_rows = gi.index_select(0, torch.from_numpy(rows).cuda()).cpu().numpy()
+    rg = om.recall(g_rows, fx["exact_idx"])
+    ro = float(fx["recall_rows"])
+    agree = float((g_rows == fx["oracle_idx"]).all(axis=1).mean())
+    trace = [int(c) for c in fx["trace"]]
+    print(f"recall gpu={rg:.5f} oracle={ro:.5f}; rows identical to the oracle {agree:.4f}; "
+          f"iterations gpu={nn.iterations_} {nn.changed_} oracle={len(trace)} {trace}")
+    assert rg >= ro - 0.002
+    assert abs(nn.iterations_ - len(trace)) <= 1
+    assert agree > 0.6   # 0.73 at 70,000 x 784: FP16 tie orders differ (MLX vs NumPy vs device)
+    assert (g_rows != rows[:, None]).all()
+    assert all(len(set(r)) == 15 for r in g_rows)

@@ -331,3 +331,59 @@ def test_fit_transform_nndescent_and_precision_screens_agree():
     c = UMAP(n_epochs=30, random_state=2, knn_method="nndescent")
     Yc = c.fit_transform(X)
     assert np.isfinite(Yc).all()
+
+
+def test_c0_end_to_end_layout_quality_matches_oracle():
+    """The small plain-method workload through the public API
+    (UMAP(n_epochs=200, init="random", knn_method="brute", random_state=0) on
+    10,000 x 128 N(0,1), seed 0) against the oracle's layout of the same
+    graph, a/b and init (tests/golden/umap_c0_oracle.json, written by
+    make_golden.py umap_c0): the kept-edge count up to threshold rounding,
+    15-NN preservation and trustworthiness within 0.01 (negatives differ:
+    Philox here, NumPy there)."""
+    import json
+    from pathlib import Path
+
+    from mlx_vis_amd import UMAP
+
+    fx = json.loads((Path(__file__).resolve().parent / "golden" / "umap_c0_oracle.json").read_text())
+    X = np.random.default_rng(0).standard_normal((10000, 128), dtype=np.float32)
+    um = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=200, negative_sample_rate=5, init="random",
+              knn_method="brute", random_state=0)
+    Y = um.fit_transform(X)
+    assert Y.shape == (10000, 2) and Y.dtype == np.float32 and np.isfinite(Y).all()
+    # prune decisions may differ only for weights within FP32 rounding of the threshold
+    assert abs(um.graph_.n_edges - fx["n_edges"]) <= max(2, fx["n_edges"] // 10000)
+    sample = np.sort(np.random.default_rng(fx["trust_rows_seed"]).choice(10000, fx["trust_rows"], replace=False))
+    p = om.knn_preservation(X, Y, 15)
+    t = om.trustworthiness(X, Y, 15, rows=sample)
+    print(f"preservation gpu={p:.4f} oracle={fx['knn15_preservation']:.4f}; "
+          f"trust gpu={t:.4f} oracle={fx['trustworthiness_15']:.4f}")
+    assert abs(p - fx["knn15_preservation"]) < 0.01
+    assert abs(t - fx["trustworthiness_15"]) < 0.01
+
+
+def test_umap_normalize_minmax_is_applied_before_the_knn():
+    """UMAP(normalize="minmax") gives the embedding of the min-max scaled input
+    (mlx_vis/_normalize.py:28-31 applied in fit_transform, umap.py:86-88)."""
+    from mlx_vis_amd import UMAP
+
+    rng = np.random.default_rng(4)
+    X = (rng.standard_normal((3000, 20)) * rng.uniform(0.5, 30, 20) + 5).astype(np.float32)
+    kw = dict(n_neighbors=10, n_epochs=60, init="random", knn_method="brute", random_state=3)
+    ua, ub = UMAP(normalize="minmax", **kw), UMAP(**kw)
+    Ya = ua.fit_transform(X)
+    Xs = ((X - X.min(0)) / (X.max(0) - X.min(0) + 1e-8)).astype(np.float32)
+    Yb = ub.fit_transform(Xs)
+    # the same graph (the kNN of the scaled input, bit for bit) ...
+    assert torch.equal(ua.knn_[0], ub.knn_[0]) and torch.equal(ua.knn_[1], ub.knn_[1])
+    oi, _ = oknn.exact_knn(Xs, Xs, 10, self_base=0)
+    np.testing.assert_array_equal(ua.knn_[0].cpu().numpy(), oi)
+    # ... and layouts of the same quality (not bitwise: tail sums at nodes with
+    # more than 64 in-edges reach them by atomics in arrival order, and the
+    # SGD amplifies such rounding differences over the epochs)
+    assert Ya.shape == Yb.shape == (3000, 2) and np.isfinite(Ya).all()
+    sample = np.arange(0, 3000, 5)
+    ta, tb = om.trustworthiness(Xs, Ya, 10, rows=sample), om.trustworthiness(Xs, Yb, 10, rows=sample)
+    print(f"trustworthiness normalize=minmax {ta:.4f}, pre-scaled input {tb:.4f}")
+    assert abs(ta - tb) < 0.02
