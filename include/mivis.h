@@ -87,6 +87,22 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
                  int32_t* out_idx, float* out_dist, int32_t* out_stats,
                  void* ws, size_t ws_bytes, void* stream);
 
+/* mv_knn_brute without a workspace argument and without any allocation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, results and errors as mv_knn_brute,
+ * always with the fused screen (the symmetric screen's n x n matrix does not
+ * fit). The workspace is a static 128 MiB device array of the library, one per
+ * GPU, held until the process ends: a call needing more (mv_knn_ws_bytes with
+ * mv_knn_set_symmetric(0) > 134217728, e.g. nq = n = 8192 at d = 784 fits,
+ * n = 100,000 at d = 784 does not) fails with MV_EUNSUPPORTED and a message.
+ * Calls on one GPU share the array: calls that overlap in time on the same
+ * GPU (other streams, other host threads) race. Phase timing is not
+ * recorded. */
+int mv_knn_brute_noalloc(const float* Q, int64_t nq, const float* X, int64_t n, int64_t d, int k,
+                         int dtype, int rerank, const int32_t* q_self, int64_t self_base, int euclid,
+                         int32_t* out_idx, float* out_dist, int32_t* out_stats, void* stream);
+
 /* Phase timing of mv_knn_brute on the calling thread: with timing on, each
  * call records HIP events on its stream; mv_knn_last_phase_ms waits for the
  * last timed call and writes milliseconds of [prepass, screen, rerank,

@@ -37,6 +37,8 @@
 //                  FP64 (rare; guarantees the canonical result).
 #include <math.h>
 
+#include <mutex>
+
 #include "common.hpp"
 
 namespace mv {
@@ -1725,7 +1727,42 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
 // Symmetric screen for self kNN (mv_knn_set_symmetric; on by default).
 static bool g_symmetric = true;
 
-static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, bool same, int dtype) {
+// Static scratch of mv_knn_brute_noalloc: the workspace of its calls, for
+// callers that cannot allocate. Each GPU the library runs on holds one copy
+// (128 MiB) until the process ends; every call on a GPU shares it, so calls
+// that overlap in time on one GPU race. Host code never passes `scratch`
+// itself to a kernel (in host code the name is only a stand-in): it uses the
+// device address that hipGetSymbolAddress returns for the current device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(128) << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+static int noalloc_scratch(void** out) {
+  static std::mutex mu;
+  static void* addr[64] = {nullptr};
+  int dev = 0;
+  MV_TRY_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    mv::set_error("mv_knn_brute_noalloc: device ordinal %d outside [0, 64)", dev);
+    return MV_EUNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    MV_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      mv::set_error("mv_knn_brute_noalloc: hipGetSymbolAddress returned no address for the scratch array");
+      return MV_EHIP;
+    }
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return MV_OK;
+}
+
+static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, bool same, int dtype,
+                          bool sym_ok) {
   Layout L;
   L.nq_pad = static_cast<int64_t>(align_up(nq > 0 ? nq : 1, BM));
   L.n_pad = static_cast<int64_t>(align_up(n, BN));
@@ -1739,7 +1776,7 @@ static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, b
   L.C = C;
   L.dtype = dtype;
   L.same = same;
-  L.sym = same && g_symmetric && n >= SYM_MIN_N &&
+  L.sym = same && sym_ok && n >= SYM_MIN_N &&
           static_cast<size_t>(L.n_pad) * L.n_pad * 4 <= SYM_MAX_BYTES;
   size_t total = 0;
   carve(nullptr, nq, n, d, L, &total);
@@ -1816,63 +1853,17 @@ int launch_screen(const ScreenArgs& sa, int64_t grid, hipStream_t s) {
   return MV_OK;
 }
 
-}  // namespace knn
-}  // namespace mv
-
-using namespace mv;
-using namespace mv::knn;
-
-extern "C" {
-
-int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rerank, int same_qx,
-                    size_t* out_bytes) {
-  MV_REQUIRE(out_bytes, "mv_knn_ws_bytes: out_bytes is NULL");
-  MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "mv_knn_ws_bytes: bad shape nq=%lld n=%lld d=%lld",
-             (long long)nq, (long long)n, (long long)d);
-  MV_REQUIRE(k >= 1 && k <= CMAX_K, "mv_knn_ws_bytes: k=%d outside [1, %d]", k, CMAX_K);
-  MV_REQUIRE(rerank >= 1, "mv_knn_ws_bytes: rerank=%d < 1", rerank);
-  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
-    mv::set_error("mv_knn_ws_bytes: dtype %d not supported", dtype);
-    return MV_EUNSUPPORTED;
-  }
-  Layout L = make_layout(nq, n, d, k, rerank, same_qx != 0, dtype);
-  *out_bytes = L.bytes;
-  return MV_OK;
-}
-
-int mv_row_sqnorms(const float* X, int64_t n, int64_t d, float* out, void* stream) {
-  MV_FAULT_POINT("launch");
-  MV_REQUIRE(X && out, "mv_row_sqnorms: null pointer");
-  MV_REQUIRE(n >= 0 && d >= 1, "mv_row_sqnorms: bad shape");
-  if (n == 0) return MV_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_row_sqnorms, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, X, n, d,
-                     out);
-  MV_LAUNCHED("k_row_sqnorms");
-  return MV_OK;
-}
-
-int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t d, int k, int dtype,
-                 int rerank, const int32_t* q_self, int64_t self_base, int euclid, int32_t* out_idx,
-                 float* out_dist, int32_t* out_stats, void* ws, size_t ws_bytes, void* stream) {
-  MV_REQUIRE(Q && X && out_idx && out_dist, "mv_knn_brute: null pointer argument");
-  MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "mv_knn_brute: bad shape nq=%lld n=%lld d=%lld", (long long)nq,
-             (long long)n, (long long)d);
-  MV_REQUIRE(n < (1ll << 31) - 256 && nq < (1ll << 31) - 256, "mv_knn_brute: n or nq too large");
-  MV_REQUIRE(k >= 1 && k <= CMAX_K, "mv_knn_brute: k=%d outside [1, %d]", k, CMAX_K);
-  MV_REQUIRE(rerank >= 1, "mv_knn_brute: rerank=%d < 1", rerank);
-  const bool excl = q_self != nullptr || self_base >= 0;
-  MV_REQUIRE(k <= n - (excl ? 1 : 0), "mv_knn_brute: k=%d exceeds the %lld selectable rows", k,
-             (long long)(n - (excl ? 1 : 0)));
-  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
-    mv::set_error("mv_knn_brute: dtype %d not supported", dtype);
-    return MV_EUNSUPPORTED;
-  }
-  MV_FAULT_POINT("launch");
+// The whole brute-force kNN of mv_knn_brute on a given workspace. `sym_ok`
+// permits the symmetric screen (self kNN), `timed` the phase events of
+// mv_knn_set_timing; `who` names the entry point in error messages.
+static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, int64_t n, int64_t d, int k,
+                   int dtype, int rerank, const int32_t* q_self, int64_t self_base, int euclid, int32_t* out_idx,
+                   float* out_dist, int32_t* out_stats, void* ws, size_t ws_bytes, void* stream, bool sym_ok,
+                   bool timed) {
   const bool same = (Q == X) && nq == n;
-  Layout L = make_layout(nq, n, d, k, rerank, same, dtype);
+  Layout L = make_layout(nq, n, d, k, rerank, same, dtype, sym_ok);
   if (ws_bytes < L.bytes || !ws) {
-    mv::set_error("mv_knn_brute: workspace of %zu bytes < required %zu", ws_bytes, L.bytes);
+    mv::set_error("%s: workspace of %zu bytes < required %zu", who, ws_bytes, L.bytes);
     return MV_EWORKSPACE;
   }
   if (nq == 0) return MV_OK;
@@ -1887,11 +1878,13 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   int32_t* stats = out_stats ? out_stats : B.ctr + 8;
   const int64_t self_b = q_self ? -1 : self_base;
 
-  if (g_timer.on) {
-    if (int rc = g_timer.ensure()) return rc;
-    g_timer.valid = false;
+  PhaseTimer quiet;                              // records nothing
+  PhaseTimer& tm = timed ? g_timer : quiet;
+  if (tm.on) {
+    if (int rc = tm.ensure()) return rc;
+    tm.valid = false;
   }
-  if (int rc = g_timer.mark(0, s)) return rc;
+  if (int rc = tm.mark(0, s)) return rc;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, B.ctr, 16);
   MV_LAUNCHED("k_zero_i32");
   if (out_stats) {
@@ -1971,7 +1964,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     sa.invx = B.invx;
     sa.row_bytes = L.dq_pad * elem_bytes(dtype);
   }
-  if (int rc = g_timer.mark(1, s)) return rc;
+  if (int rc = tm.mark(1, s)) return rc;
   if (L.sym) {
     // symmetric screen: upper-triangle tiles into the full matrix, then one
     // list per row (a single split)
@@ -1995,7 +1988,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
               : dtype == MV_BF16X3 ? k_dist_sym<DT_BF16X3, SYM_W> : k_dist_sym<DT_FP8, SYM_W>;
     hipLaunchKernelGGL(ks, dim3(sgrid), dim3(64 * SYM_W), 0, s, ya);
     MV_LAUNCHED("k_dist_sym");
-    if (int rc = g_timer.mark(5, s)) return rc;
+    if (int rc = tm.mark(5, s)) return rc;
     SelectArgs xa;
     xa.D = B.Dfull;
     xa.ldd = L.n_pad;
@@ -2019,7 +2012,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
                                   : launch_screen<DT_FP8>(sa, grid, s);
     if (rc) return rc;
   }
-  if (int rc2 = g_timer.mark(2, s)) return rc2;
+  if (int rc2 = tm.mark(2, s)) return rc2;
 
   // Screen arithmetic bound, in units of (|q|^2 + max |x|^2). FP32: two
   // roundings per product term plus the norms and the final combination.
@@ -2059,7 +2052,7 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
   ra.stats = stats;
   hipLaunchKernelGGL(k_rerank, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, ra);
   MV_LAUNCHED("k_rerank");
-  if (int rc2 = g_timer.mark(3, s)) return rc2;
+  if (int rc2 = tm.mark(3, s)) return rc2;
 
   const int32_t* scan_count = fb1_count;
   const int32_t* scan_rows = B.fb1;
@@ -2152,10 +2145,95 @@ int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t 
     hipLaunchKernelGGL(k_exact_scan, dim3(static_cast<unsigned>(sgrid)), dim3(256), 0, s, xa);
     MV_LAUNCHED("k_exact_scan");
   }
-  if (int rc2 = g_timer.mark(4, s)) return rc2;
-  g_timer.valid = g_timer.on;
-  g_timer.last_sym = L.sym;
+  if (int rc2 = tm.mark(4, s)) return rc2;
+  tm.valid = tm.on;
+  tm.last_sym = L.sym;
   return MV_OK;
+}
+
+}  // namespace knn
+}  // namespace mv
+
+using namespace mv;
+using namespace mv::knn;
+
+extern "C" {
+
+int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rerank, int same_qx,
+                    size_t* out_bytes) {
+  MV_REQUIRE(out_bytes, "mv_knn_ws_bytes: out_bytes is NULL");
+  MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "mv_knn_ws_bytes: bad shape nq=%lld n=%lld d=%lld",
+             (long long)nq, (long long)n, (long long)d);
+  MV_REQUIRE(k >= 1 && k <= CMAX_K, "mv_knn_ws_bytes: k=%d outside [1, %d]", k, CMAX_K);
+  MV_REQUIRE(rerank >= 1, "mv_knn_ws_bytes: rerank=%d < 1", rerank);
+  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
+    mv::set_error("mv_knn_ws_bytes: dtype %d not supported", dtype);
+    return MV_EUNSUPPORTED;
+  }
+  Layout L = make_layout(nq, n, d, k, rerank, same_qx != 0, dtype, g_symmetric);
+  *out_bytes = L.bytes;
+  return MV_OK;
+}
+
+int mv_row_sqnorms(const float* X, int64_t n, int64_t d, float* out, void* stream) {
+  MV_FAULT_POINT("launch");
+  MV_REQUIRE(X && out, "mv_row_sqnorms: null pointer");
+  MV_REQUIRE(n >= 0 && d >= 1, "mv_row_sqnorms: bad shape");
+  if (n == 0) return MV_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_row_sqnorms, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, X, n, d,
+                     out);
+  MV_LAUNCHED("k_row_sqnorms");
+  return MV_OK;
+}
+
+// Argument checks shared by mv_knn_brute and mv_knn_brute_noalloc.
+static int knn_check(const char* who, const float* Q, int64_t nq, const float* X, int64_t n, int64_t d, int k,
+                     int dtype, int rerank, const int32_t* q_self, int64_t self_base, int32_t* out_idx,
+                     float* out_dist) {
+  MV_REQUIRE(Q && X && out_idx && out_dist, "%s: null pointer argument", who);
+  MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "%s: bad shape nq=%lld n=%lld d=%lld", who, (long long)nq,
+             (long long)n, (long long)d);
+  MV_REQUIRE(n < (1ll << 31) - 256 && nq < (1ll << 31) - 256, "%s: n or nq too large", who);
+  MV_REQUIRE(k >= 1 && k <= CMAX_K, "%s: k=%d outside [1, %d]", who, k, CMAX_K);
+  MV_REQUIRE(rerank >= 1, "%s: rerank=%d < 1", who, rerank);
+  const bool excl = q_self != nullptr || self_base >= 0;
+  MV_REQUIRE(k <= n - (excl ? 1 : 0), "%s: k=%d exceeds the %lld selectable rows", who, k,
+             (long long)(n - (excl ? 1 : 0)));
+  if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
+    mv::set_error("%s: dtype %d not supported", who, dtype);
+    return MV_EUNSUPPORTED;
+  }
+  return MV_OK;
+}
+
+int mv_knn_brute(const float* Q, int64_t nq, const float* X, int64_t n, int64_t d, int k, int dtype,
+                 int rerank, const int32_t* q_self, int64_t self_base, int euclid, int32_t* out_idx,
+                 float* out_dist, int32_t* out_stats, void* ws, size_t ws_bytes, void* stream) {
+  if (int rc = knn_check("mv_knn_brute", Q, nq, X, n, d, k, dtype, rerank, q_self, self_base, out_idx, out_dist))
+    return rc;
+  MV_FAULT_POINT("launch");
+  return knn_run("mv_knn_brute", Q, nq, X, n, d, k, dtype, rerank, q_self, self_base, euclid, out_idx, out_dist,
+                 out_stats, ws, ws_bytes, stream, g_symmetric, true);
+}
+
+int mv_knn_brute_noalloc(const float* Q, int64_t nq, const float* X, int64_t n, int64_t d, int k, int dtype,
+                         int rerank, const int32_t* q_self, int64_t self_base, int euclid, int32_t* out_idx,
+                         float* out_dist, int32_t* out_stats, void* stream) {
+  static const char* who = "mv_knn_brute_noalloc";
+  if (int rc = knn_check(who, Q, nq, X, n, d, k, dtype, rerank, q_self, self_base, out_idx, out_dist)) return rc;
+  const Layout L = make_layout(nq, n, d, k, rerank, (Q == X) && nq == n, dtype, false);
+  if (L.bytes > NOALLOC_SCRATCH_BYTES) {
+    mv::set_error("%s: nq=%lld n=%lld d=%lld k=%d needs %zu bytes of scratch, more than the %zu-byte static "
+                  "scratch of this entry point; use mv_knn_brute with a workspace",
+                  who, (long long)nq, (long long)n, (long long)d, k, L.bytes, NOALLOC_SCRATCH_BYTES);
+    return MV_EUNSUPPORTED;
+  }
+  MV_FAULT_POINT("launch");
+  void* ws = nullptr;
+  if (int rc = noalloc_scratch(&ws)) return rc;
+  return knn_run(who, Q, nq, X, n, d, k, dtype, rerank, q_self, self_base, euclid, out_idx, out_dist, out_stats,
+                 ws, NOALLOC_SCRATCH_BYTES, stream, false, false);
 }
 
 int mv_knn_set_symmetric(int on) {

@@ -50,6 +50,7 @@ _SIGNATURES: dict[str, list] = {
     "mv_device_info": [_p, _p, _p],
     "mv_knn_ws_bytes": [_i64, _i64, _i64, _i32, _i32, _i32, _i32, C.POINTER(_sz)],
     "mv_knn_brute": [_p, _i64, _p, _i64, _i64, _i32, _i32, _i32, _p, _i64, _i32, _p, _p, _p, _p, _sz, _p],
+    "mv_knn_brute_noalloc": [_p, _i64, _p, _i64, _i64, _i32, _i32, _i32, _p, _i64, _i32, _p, _p, _p, _p],
     "mv_row_sqnorms": [_p, _i64, _i64, _p, _p],
     "mv_knn_set_timing": [_i32],
     "mv_knn_set_symmetric": [_i32],
