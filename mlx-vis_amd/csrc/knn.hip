@@ -40,6 +40,7 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "lowd.hpp"
 
 namespace mv {
 namespace knn {
@@ -1645,6 +1646,7 @@ struct Layout {
   int C, dtype;
   bool same;
   bool sym;                                // symmetric screen: full distance matrix in the workspace
+  bool lowd;                               // d <= LOWD_MAX_D: exact grid search (lowd.hip), nothing else
   size_t bytes;
 };
 
@@ -1763,7 +1765,12 @@ static int noalloc_scratch(void** out) {
 
 static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, bool same, int dtype,
                           bool sym_ok) {
-  Layout L;
+  Layout L{};
+  if (d <= LOWD_MAX_D) {
+    L.lowd = true;
+    L.bytes = lowd_knn_ws_bytes(n);
+    return L;
+  }
   L.nq_pad = static_cast<int64_t>(align_up(nq > 0 ? nq : 1, BM));
   L.n_pad = static_cast<int64_t>(align_up(n, BN));
   L.d_pad = static_cast<int64_t>(align_up(d, KC));
@@ -1867,6 +1874,15 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     return MV_EWORKSPACE;
   }
   if (nq == 0) return MV_OK;
+  if (L.lowd) {
+    // one to three dimensions: the exact grid search (no screen, no certificate)
+    if (out_stats) {
+      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), out_stats, 4);
+      MV_LAUNCHED("k_zero_i32");
+    }
+    return lowd_knn(Q, nq, X, n, static_cast<int>(d), k, q_self, self_base, euclid, out_idx, out_dist, ws, ws_bytes,
+                    static_cast<hipStream_t>(stream));
+  }
   size_t total = 0;
   Buffers B = carve(ws, nq, n, d, L, &total);
   hipStream_t s = static_cast<hipStream_t>(stream);

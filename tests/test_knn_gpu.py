@@ -293,3 +293,41 @@ def test_c1_mixture_8192_rows_all_match_oracle(dtype):
         lib.mv_knn_set_symmetric(prev)
     print("stats fused", st_f.tolist(), flush=True)
     _check(idx_f, dist_f, oi, od)
+
+
+@pytest.mark.parametrize("case", ["2d_200k", "2d_ties", "1d", "3d_aniso", "queries_outside", "flat_dim"])
+def test_low_dimensional_grid_search_matches_oracle(case):
+    """d <= 3 runs the exact grid search (FP64 canonical distance of every
+    visited point): embeddings of many points, integer coordinates with exact
+    ties and duplicates, one dimension, anisotropic 3-D, queries beyond the
+    data's bounding box, a dimension without extent."""
+    print(f"case lowd {case}", flush=True)
+    rng = np.random.default_rng(41)
+    Q = None
+    k = 15
+    if case == "2d_200k":
+        X = (rng.standard_normal((200_000, 2)) * [3.0, 1.0] + rng.integers(0, 5, (200_000, 1)) * 10).astype(np.float32)
+    elif case == "2d_ties":
+        X = rng.integers(0, 30, (20_000, 2)).astype(np.float32)
+        k = 40
+    elif case == "1d":
+        X = rng.standard_normal((30_000, 1)).astype(np.float32)
+        k = 9
+    elif case == "3d_aniso":
+        X = (rng.standard_normal((50_000, 3)) * [1.0, 30.0, 0.01]).astype(np.float32)
+        k = 64
+    elif case == "queries_outside":
+        X = rng.uniform(-1, 1, (40_000, 2)).astype(np.float32)
+        Q = (rng.standard_normal((3_000, 2)) * 5).astype(np.float32)
+    else:
+        X = rng.standard_normal((10_000, 3)).astype(np.float32)
+        X[:, 2] = 1.5
+    idx, dist, st = _run(X, k, Q)
+    assert st.tolist() == [0, 0, 0, 0]
+    if Q is None:
+        rows = np.sort(rng.choice(X.shape[0], min(3000, X.shape[0]), replace=False))
+        oi, od = oknn.exact_knn(X[rows], X, k, q_self=rows)
+        _check(idx[rows], dist[rows], oi, od)
+    else:
+        oi, od = oknn.exact_knn(Q, X, k)
+        _check(idx, dist, oi, od)
