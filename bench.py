@@ -184,12 +184,13 @@ def learned_form(Xd, um, fresh_n, reps=5):
     out["plain_layout_s"] = round(t_plain, 4)
     out["speedup_infer_vs_plain"] = round(t_plain / t_inf, 1)
 
-    log("learned form: exact 15-NN of the fresh points (FP32 brute force)")
+    log("learned form: exact 15-NN of the fresh points (split-BF16 screen, exact re-rank)")
     torch.cuda.synchronize()
     t = time.perf_counter()
-    hi_idx, hi_dist = knn_brute_device(Xfd, 15)
+    hi_idx, hi_dist = knn_brute_device(Xfd, 15, dtype="bf16x3")
     torch.cuda.synchronize()
     out["eval_exact_knn_s"] = round(time.perf_counter() - t, 3)
+    out["eval_exact_knn_dtype"] = "bf16x3 (indices identical to the FP32 path by the certificate)"
 
     def pres(Zd):
         lo, _ = knn_brute_device(Zd.contiguous(), 15)
@@ -219,6 +220,14 @@ def learned_form(Xd, um, fresh_n, reps=5):
                      if bool(flagged.any()) else None,
                      "flagged_after": round(float(c_ref[flagged].double().mean().item()) / 15, 4)
                      if bool(flagged.any()) else None}
+    log("learned form: trustworthiness (k = 15) on 2,000 sampled fresh points")
+    from mlx_vis_amd import trustworthiness
+
+    rows = np.sort(np.random.default_rng(5).choice(fresh_n, min(2000, fresh_n), replace=False))
+    out["trustworthiness_15"] = {
+        name: round(trustworthiness(Xfd, Zd.contiguous(), 15, rows=rows), 4)
+        for name, Zd in (("mlp", Z), ("plain_layout", torch.from_numpy(z_plain).cuda()), ("mlp_refined", z_ref))}
+    out["trustworthiness_rows"] = int(rows.size)
     out["config"] = {"mlp": "784-512-512-2 GELU, BF16 weights + FP32 master, Adam lr 1e-3, batch 4096 edges",
                      "fresh_points": fresh_n, "threshold": 0.8, "refine_epochs": 50}
     return out
@@ -518,12 +527,42 @@ def main():
             rel = np.abs(dd - d64) / np.maximum(d64, 1e-30)
             return {"max": float(f"{rel.max():.3e}"), "mean": float(f"{rel.mean():.3e}"), "rows": int(err_rows.size)}
 
-        prec = {"fp32_dist_rel_err_vs_fp64": dist_err("fp32", 2)}
-        for dt, rr in (("bf16x3", 2), ("bf16", 2), ("bf16", 4), ("fp8", 2), ("fp8", 4)):
-            log(f"kNN {dt} screen, rerank {rr}")
+        # The screens' own distance error: squared distances of the same pairs
+        # (each sampled row and its 15 exact neighbours) from the operands as
+        # each screen quantises them (centred rows; BF16 round to nearest
+        # even; FP8 e4m3 with the per-row absmax scale 448 / max|x|; split BF16
+        # hi + lo), products exact, against the FP64 distances. The FP32
+        # accumulation of the MFMA adds ~1e-7 relative on top.
+        ex_idx = ref_idx.cpu().numpy()[err_rows]
+        mu64 = Xall64.mean(axis=0)
+        Xc32 = torch.from_numpy((Xall64 - mu64).astype(np.float32))
+
+        def quantise(dtype):
+            if dtype == "bf16":
+                return Xc32.to(torch.bfloat16).double()
+            if dtype == "bf16x3":
+                hi = Xc32.to(torch.bfloat16).float()
+                return hi.double() + (Xc32 - hi).to(torch.bfloat16).double()
+            amax = Xc32.abs().max(dim=1, keepdim=True).values.clamp_min(1e-30)
+            sc = 448.0 / amax
+            return ((Xc32 * sc).to(torch.float8_e4m3fn).float() / sc).double()
+
+        ref2 = ((X64[:, None, :] - Xall64[ex_idx]) ** 2).sum(-1)
+
+        def screen_err(dtype):
+            Xq = quantise(dtype).numpy()
+            got = ((Xq[err_rows][:, None, :] - Xq[ex_idx]) ** 2).sum(-1)
+            rel = np.abs(got - ref2) / np.maximum(ref2, 1e-30)
+            return {"max": float(f"{rel.max():.3e}"), "mean": float(f"{rel.mean():.3e}"), "pairs": int(rel.size)}
+
+        prec = {"fp32_dist_rel_err_vs_fp64": dist_err("fp32", 2),
+                "screen_sq_dist_rel_err_vs_fp64": {dt: screen_err(dt) for dt in ("bf16x3", "bf16", "fp8")}}
+        for dt, rr in (("bf16x3", 2), ("bf16", 2), ("bf16", 4), ("bf16", 9), ("fp8", 2), ("fp8", 9)):
+            C = min(rr * 15, 128)
+            log(f"kNN {dt} screen, {C} candidates")
             ph_dt, st_dt, idx_dt, _ = knn_phase_timing(Xd, args.knn_reps, dt, rr)
             tot = sum(ph_dt)
-            prec[f"{dt}_c{rr * 15}"] = {"ms_total": round(tot, 3), "ms_phases": [round(v, 3) for v in ph_dt],
+            prec[f"{dt}_c{C}"] = {"ms_total": round(tot, 3), "ms_phases": [round(v, 3) for v in ph_dt],
                         "speedup_vs_fp32": round(knn_total_ms / tot, 3),
                         "identical_indices": bool(torch.equal(idx_dt, ref_idx)),
                         "rows_uncertified_first_pass": st_dt[2], "rows_uncertified_after_rescreen": st_dt[3],

@@ -76,9 +76,12 @@ int mv_device_info(int* cu_count, int* lds_per_block, int* wave_size);
  * scan, [1] rows whose order needed the FP64 tie check, [2] rows that failed
  * the first certificate, [3] reserved.
  * Limits: 1 <= k <= 64, k <= number of selectable rows. The candidate count
- * is min(rerank*k, 64) (at least k); above k = 32 it is 64, and a row whose
- * k-th neighbour is not separated from the 64th candidate by the screen's
- * error bound goes to the FP64 scan.
+ * is min(rerank*k, 128) (at least k) with the symmetric self-kNN screen (see
+ * mv_knn_set_symmetric), min(rerank*k, 64) with the fused screen; a row whose
+ * k-th neighbour is not separated from the last candidate by the screen's
+ * error bound goes to the FP64 scan (BF16 / FP8: first to a split-BF16
+ * re-screen with 64 candidates). d <= 3 uses an exact grid search instead of
+ * a screen (dtype and rerank are then ignored).
  */
 int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rerank,
                     int same_qx, size_t* out_bytes);
@@ -186,9 +189,9 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
 
 /* In-edge lists of the kept slots (eps > 0), grouped by tail node: entries
  * t < (number of kept slots) hold the slot in_slots[t] and its tail
- * in_node[t]; nodes appear in ascending order, the slots of one node in
- * ascending order. Used by mv_umap_epochs to sum tail updates without per-edge
- * atomics.
+ * in_node[t]; nodes appear in ascending order, the slots of a node with at
+ * most 64 in-edges in ascending order (of a node with more in no fixed order).
+ * Used by mv_umap_epochs to sum tail updates without per-edge atomics.
  * ws: 3 * align256(4n) + align256(4 * ceil(n / 1024)) bytes. */
 int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, int32_t* in_slots,
                      int32_t* in_node, void* ws, size_t ws_bytes, void* stream);

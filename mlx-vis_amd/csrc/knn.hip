@@ -52,7 +52,8 @@ constexpr int BN = 128;               // X rows (columns of D) per tile
 constexpr int KC = 32;                // K chunk (floats) staged per step
 constexpr int LDSA = KC + 4;          // padded LDS row stride (floats): conflict-free b128 reads
 constexpr int DTS = BN + 4;           // padded stride of the D tile in LDS
-constexpr int CMAX = 64;              // max candidates per row (one per lane)
+constexpr int CMAX = 64;              // max candidates per row of the fused screen (one per lane)
+constexpr int CMAX_SYM = 128;         // max candidates per row of the symmetric screen (two per lane)
 constexpr int NT = 256;               // threads per workgroup
 constexpr int ROW_CHUNK = 256;        // rows per column-sum partial
 constexpr int MAX_SPLIT = 8;
@@ -326,6 +327,23 @@ __device__ __forceinline__ uint64_t list_absorb(uint64_t& lk, uint64_t keys, int
   lk = list_merge64_u64(lk, keys);
   if (l >= C) lk = KEY_NONE;
   return readlane_u64(lk, C - 1);
+}
+
+// The same for a list of up to 128 keys held in two registers: lk0 the
+// ranks 0..63, lk1 the ranks 64..127, each ascending in lane order. With the
+// batch sorted descending and padded in front with 64 KEY_NONE, the position-
+// wise minimum of the list and the batch holds the 128 smallest of both and
+// is bitonic (Batcher); one compare-exchange between the registers and a
+// half-cleaner sort of each register sort it.
+__device__ __forceinline__ uint64_t list_absorb2(uint64_t& lk0, uint64_t& lk1, uint64_t keys, int C, int l) {
+  const uint64_t b = wave_sort_u64<true>(keys);
+  const uint64_t t1 = lk1 < b ? lk1 : b;
+  const uint64_t lo = lk0 < t1 ? lk0 : t1, hi = lk0 < t1 ? t1 : lk0;
+  lk0 = bitonic_clean_u64(lo);
+  lk1 = bitonic_clean_u64(hi);
+  if (l >= C) lk0 = KEY_NONE;
+  if (64 + l >= C) lk1 = KEY_NONE;
+  return C > 64 ? readlane_u64(lk1, C - 65) : readlane_u64(lk0, C - 1);
 }
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -1045,14 +1063,21 @@ struct SelectArgs {
 
 // Offers the key of each lane (KEY_NONE: none) to the row's list: keys below
 // the list's C-th key wait in the LDS buffer, which is merged into the list
-// when it would overflow.
-__device__ __forceinline__ void select_offer(uint64_t key, uint64_t& lk, uint64_t& tk, float& tf, int& cnt,
-                                             uint64_t* pend, int C, int l) {
+// when it would overflow. KPL = 2 keeps a list of up to 128 keys (lk, lk1).
+template <int KPL>
+__device__ __forceinline__ uint64_t select_absorb(uint64_t& lk, uint64_t& lk1, uint64_t keys, int C, int l) {
+  if constexpr (KPL == 1) return list_absorb(lk, keys, C, l);
+  else return list_absorb2(lk, lk1, keys, C, l);
+}
+
+template <int KPL>
+__device__ __forceinline__ void select_offer(uint64_t key, uint64_t& lk, uint64_t& lk1, uint64_t& tk, float& tf,
+                                             int& cnt, uint64_t* pend, int C, int l) {
   key = key < tk ? key : KEY_NONE;
   uint64_t m = ballot(key != KEY_NONE);
   if (m == 0) return;
   if (cnt + __popcll(m) > 64) {
-    tk = list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
+    tk = select_absorb<KPL>(lk, lk1, l < cnt ? pend[l] : KEY_NONE, C, l);
     if (tk != KEY_NONE) tf = key_dist(tk);
     cnt = 0;
     key = key < tk ? key : KEY_NONE;
@@ -1071,6 +1096,7 @@ __device__ __forceinline__ void select_offer(uint64_t key, uint64_t& lk, uint64_
 // > tau. Within the blocks read, a float threshold (the list's C-th distance)
 // rejects values; keys that pass wait in an LDS buffer of 64 and are merged
 // into the sorted list 64 at a time.
+template <int KPL>
 __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
   __shared__ uint64_t s_pend[4][64];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -1082,7 +1108,7 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
   const float* Dr = a.D + row * a.ldd;
   uint64_t* pend = s_pend[w];
   const int C = a.C;
-  uint64_t lk = KEY_NONE;
+  uint64_t lk = KEY_NONE, lk1 = KEY_NONE;
   uint64_t tk = KEY_NONE;
   float tf = INFINITY;
   int cnt = 0;
@@ -1093,11 +1119,12 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
     for (int b0 = 0; b0 < a.nb; b0 += 64) {
       const int b = b0 + l;
       const float v = b < a.nb ? bm[b] : INFINITY;
-      select_offer(v < INFINITY ? dkey(v, static_cast<uint32_t>(b)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
+      select_offer<KPL>(v < INFINITY ? dkey(v, static_cast<uint32_t>(b)) : KEY_NONE, lk, lk1, tk, tf, cnt, pend, C, l);
     }
-    if (cnt > 0) tk = list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
+    if (cnt > 0) tk = select_absorb<KPL>(lk, lk1, l < cnt ? pend[l] : KEY_NONE, C, l);
     if (tk != KEY_NONE) tau = key_dist(tk);
     lk = KEY_NONE;
+    lk1 = KEY_NONE;
     tk = KEY_NONE;
     tf = INFINITY;
     cnt = 0;
@@ -1112,8 +1139,8 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
         const float2 v = *reinterpret_cast<const float2*>(Dr + j);
         const bool ok0 = j < a.n && j != self && v.x <= tf;
         const bool ok1 = j + 1 < a.n && j + 1 != self && v.y <= tf;
-        select_offer(ok0 ? dkey(v.x, static_cast<uint32_t>(j)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
-        select_offer(ok1 ? dkey(v.y, static_cast<uint32_t>(j + 1)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
+        select_offer<KPL>(ok0 ? dkey(v.x, static_cast<uint32_t>(j)) : KEY_NONE, lk, lk1, tk, tf, cnt, pend, C, l);
+        select_offer<KPL>(ok1 ? dkey(v.y, static_cast<uint32_t>(j + 1)) : KEY_NONE, lk, lk1, tk, tf, cnt, pend, C, l);
       }
     }
   } else {
@@ -1129,12 +1156,13 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
       for (int q = 0; q < 4; ++q) {
         const int64_t j = j0 + q;
         const bool ok = j < a.n && j != self && v[q] <= tf;
-        select_offer(ok ? dkey(v[q], static_cast<uint32_t>(j)) : KEY_NONE, lk, tk, tf, cnt, pend, C, l);
+        select_offer<KPL>(ok ? dkey(v[q], static_cast<uint32_t>(j)) : KEY_NONE, lk, lk1, tk, tf, cnt, pend, C, l);
       }
     }
   }
-  if (cnt > 0) list_absorb(lk, l < cnt ? pend[l] : KEY_NONE, C, l);
+  if (cnt > 0) select_absorb<KPL>(lk, lk1, l < cnt ? pend[l] : KEY_NONE, C, l);
   if (l < C) a.cand[row * C + l] = lk;
+  if (KPL == 2 && 64 + l < C) a.cand[row * C + 64 + l] = lk1;
 }
 
 // ----------------------------------------------------------------- rerank --
@@ -1330,6 +1358,147 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   //    quantisation errors). The k-th candidate must lie strictly below.
   bool cert = nvalid < C;
   if (!cert) {
+    const float tau = key_dist(tau_key);
+    const float sqmax = __uint_as_float(*a.sqmax_bits);
+    const float arith = a.err_fac * (a.sqq[i] + sqmax) + 1e-30f;
+    if (!a.eq) {
+      cert = kth_ub < tau - arith;
+    } else {
+      const double t = fmax(static_cast<double>(tau) - static_cast<double>(arith), 0.0);
+      const double r = sqrt(t) - static_cast<double>(a.eq[i]) - static_cast<double>(__uint_as_float(*a.emax_bits));
+      cert = r > 0.0 && static_cast<double>(kth_ub) < r * r * (1.0 - 1e-6);
+    }
+  }
+  if (cert) {
+    if (l < k) {
+      a.out_idx[static_cast<int64_t>(orow) * k + l] = static_cast<int32_t>(out_i);
+      a.out_dist[static_cast<int64_t>(orow) * k + l] = out_d;
+    }
+  } else if (l == 0) {
+    const int p = atomicAdd(a.fb_count, 1);
+    a.fb_rows[p] = orow;
+    atomicAdd(&a.stats[a.fail_stat], 1);
+  }
+}
+
+// Re-rank of up to 128 candidates: the symmetric screen's single list per row
+// (sorted by screen key), for the wider candidate sets a BF16 screen needs
+// before its certificate holds. Same steps as k_rerank with two candidates
+// per lane: FP32 direct differences of every candidate, a 128-key sort by
+// (FP32 distance, index), FP64 only where FP32 cannot separate two of the
+// first k + 1, and the certificate against the C-th screen key. When FP64 is
+// needed it orders the first 64; a row whose 65th candidate is not separated
+// from them goes to the exhaustive scan.
+__global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + w;
+  if (i >= a.nq) return;
+  const int orow = i;
+  const int C = a.C, k = a.k;
+  const uint64_t* cl = a.cand + static_cast<int64_t>(i) * C;
+  const uint64_t c0 = l < C ? cl[l] : KEY_NONE;
+  const uint64_t c1 = 64 + l < C ? cl[64 + l] : KEY_NONE;
+  const int nvalid = __popcll(ballot(c0 != KEY_NONE)) + __popcll(ballot(c1 != KEY_NONE));
+  const uint64_t tau_key = C > 64 ? readlane_u64(c1, C - 65) : readlane_u64(c0, C - 1);
+  const int ci0 = c0 != KEY_NONE ? static_cast<int>(key_idx(c0)) : 0;
+  const int ci1 = c1 != KEY_NONE ? static_cast<int>(key_idx(c1)) : 0;
+
+  // FP32 direct-difference distances, 4 candidates at a time, 16 lanes each:
+  // lane l ends with candidate l (d0) and candidate 64 + l (d1).
+  const float* qrow = a.Q + static_cast<int64_t>(orow) * a.d;
+  const int g = l >> 4, sub = l & 15;
+  float d0 = INFINITY, d1 = INFINITY;
+  for (int r = 0; r * 4 < nvalid; ++r) {
+    const int c = r * 4 + g;
+    const int ja = __shfl(ci0, c & 63), jb = __shfl(ci1, c & 63);
+    const int j = c < 64 ? ja : jb;
+    float acc = 0.f;
+    if (c < nvalid) {
+      const float* xr = a.X + static_cast<int64_t>(j) * a.d;
+      if (a.vec4) {
+        const float4* q4 = reinterpret_cast<const float4*>(qrow);
+        const float4* x4 = reinterpret_cast<const float4*>(xr);
+        for (int64_t t = sub; t < a.d / 4; t += 16) {
+          const float4 qa = q4[t], xb = x4[t];
+          float df = qa.x - xb.x;
+          acc = fmaf(df, df, acc);
+          df = qa.y - xb.y;
+          acc = fmaf(df, df, acc);
+          df = qa.z - xb.z;
+          acc = fmaf(df, df, acc);
+          df = qa.w - xb.w;
+          acc = fmaf(df, df, acc);
+        }
+      } else {
+        for (int64_t t = sub; t < a.d; t += 16) {
+          const float df = qrow[t] - xr[t];
+          acc = fmaf(df, df, acc);
+        }
+      }
+    }
+    acc += __shfl_xor(acc, 8);
+    acc += __shfl_xor(acc, 4);
+    acc += __shfl_xor(acc, 2);
+    acc += __shfl_xor(acc, 1);
+    const float got = __shfl(acc, (l & 3) * 16);
+    if (r < 16) {
+      if ((l >> 2) == r) d0 = got;
+    } else if ((l >> 2) == r - 16) {
+      d1 = got;
+    }
+  }
+
+  // Sort the (up to) 128 keys: each register sorted, the second descending,
+  // then one compare-exchange between them and a half-cleaner sort of each.
+  uint64_t x0 = c0 != KEY_NONE ? dkey(d0, static_cast<uint32_t>(ci0)) : KEY_NONE;
+  uint64_t x1 = c1 != KEY_NONE ? dkey(d1, static_cast<uint32_t>(ci1)) : KEY_NONE;
+  x0 = wave_sort_u64(x0);
+  x1 = wave_sort_u64<true>(x1);
+  {
+    const uint64_t lo = x0 < x1 ? x0 : x1, hi = x0 < x1 ? x1 : x0;
+    x0 = bitonic_clean_u64(lo);
+    x1 = bitonic_clean_u64(hi);
+  }
+  const int64_t terms = a.vec4 ? 4 * ((a.d + 63) / 64) : (a.d + 15) / 16;
+  const float bfac = static_cast<float>(terms + 8) * 5.9604645e-8f * 1.01f;
+  const float ds = key_dist(x0);
+  const float bd = ds * bfac + 1e-35f;
+  const float ds1 = key_dist(x1);
+  const float bd1 = ds1 * bfac + 1e-35f;
+  const float first1 = __shfl(ds1, 0), firstb1 = __shfl(bd1, 0);
+  float dn = __shfl_down(ds, 1), bn = __shfl_down(bd, 1);
+  if (l == 63) {
+    dn = first1;
+    bn = firstb1;
+  }
+  const bool amb = (l < k) && (l + 1 < nvalid) && (dn - bn <= ds + bd);
+  const bool any_amb = ballot(amb) != 0;
+
+  uint32_t out_i = key_idx(x0);
+  float out_d = ds;
+  float kth_ub;
+  bool beyond_ok = true;   // no candidate past the 64th can enter the top k
+  if (any_amb) {
+    double d64 = INFINITY;
+    const bool v = l < nvalid;
+    if (v) d64 = dist64_seq(qrow, a.X + static_cast<int64_t>(out_i) * a.d, a.d);
+    uint64_t dbits = v ? static_cast<uint64_t>(__double_as_longlong(d64)) : KEY_NONE;
+    uint32_t di = v ? out_i : 0xFFFFFFFFu;
+    wave_sort_pair64(dbits, di);
+    const double dv = __longlong_as_double(static_cast<long long>(dbits));
+    out_i = di;
+    out_d = a.euclid ? static_cast<float>(sqrt(dv)) : static_cast<float>(dv);
+    const double kth = __longlong_as_double(static_cast<long long>(shfl_u64(dbits, k - 1)));
+    kth_ub = static_cast<float>(kth * (1.0 + 1e-6));
+    if (nvalid > 64) beyond_ok = first1 - firstb1 > kth_ub;
+    if (l == 0) atomicAdd(&a.stats[1], 1);
+  } else {
+    kth_ub = __shfl(ds + bd, k - 1);
+    out_d = a.euclid ? sqrtf(ds) : ds;
+  }
+
+  bool cert = beyond_ok && nvalid < C;
+  if (beyond_ok && !cert) {
     const float tau = key_dist(tau_key);
     const float sqmax = __uint_as_float(*a.sqmax_bits);
     const float arith = a.err_fac * (a.sqq[i] + sqmax) + 1e-30f;
@@ -1777,14 +1946,17 @@ static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, b
   L.dq_pad = dtype == MV_F32 ? L.d_pad : static_cast<int64_t>(align_up(d, 128 / elem_bytes(dtype)));
   L.rows_per = n > 512 * ROW_CHUNK ? ceil_div(n, 512) : ROW_CHUNK;   // at most 512 column-sum partials
   L.nchunks = ceil_div(n, L.rows_per);
-  int C = rerank * k;
-  if (C < k) C = k;
-  if (C > CMAX) C = CMAX;
-  L.C = C;
   L.dtype = dtype;
   L.same = same;
   L.sym = same && sym_ok && n >= SYM_MIN_N &&
           static_cast<size_t>(L.n_pad) * L.n_pad * 4 <= SYM_MAX_BYTES;
+  // candidates per row: up to 128 from the symmetric screen's selection, 64
+  // from the fused screen's per-lane lists
+  int C = rerank * k;
+  if (C < k) C = k;
+  const int cmax = L.sym ? CMAX_SYM : CMAX;
+  if (C > cmax) C = cmax;
+  L.C = C;
   size_t total = 0;
   carve(nullptr, nq, n, d, L, &total);
   L.bytes = total;
@@ -1948,7 +2120,8 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
 
   const int64_t row_blocks = L.nq_pad / BM;
   const int64_t col_tiles = L.n_pad / BN;
-  const int S = choose_split(row_blocks, col_tiles, L.C, 2 * cu_count_cached());   // two workgroups per CU
+  const int Cf = L.C < CMAX ? L.C : CMAX;   // candidates of a fused screen (also the re-screen)
+  const int S = choose_split(row_blocks, col_tiles, Cf, 2 * cu_count_cached());   // two workgroups per CU
   const int64_t grid = ceil_div(row_blocks, 8 / S) * 8;
   ScreenArgs sa;
   sa.q_self = q_self;
@@ -2019,7 +2192,8 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     xa.bmin = (q_self == nullptr && self_b <= 0) ? B.bmin : nullptr;
     xa.nb = static_cast<int>(col_tiles);
     xa.cand = B.cand;
-    hipLaunchKernelGGL(k_select, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
+    if (L.C > 64) hipLaunchKernelGGL(k_select<2>, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
+    else hipLaunchKernelGGL(k_select<1>, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
     MV_LAUNCHED("k_select");
   } else {
     int rc = dtype == MV_F32      ? launch_screen<DT_F32>(sa, grid, s)
@@ -2066,7 +2240,8 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
   ra.fb_count = fb1_count;
   ra.fb_rows = B.fb1;
   ra.stats = stats;
-  hipLaunchKernelGGL(k_rerank, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, ra);
+  if (L.C > 64) hipLaunchKernelGGL(k_rerank2, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, ra);
+  else hipLaunchKernelGGL(k_rerank, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, ra);
   MV_LAUNCHED("k_rerank");
   if (int rc2 = tm.mark(3, s)) return rc2;
 
@@ -2086,6 +2261,7 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
                        L.d_pad, reinterpret_cast<uint16_t*>(B.Qs2));
     MV_LAUNCHED("k_split_bf16");
     ScreenArgs sb = sa;
+    sb.C = Cf;
     sb.Qq = B.Qs2;
     sb.Xq = B.Xs;
     sb.sqq = B.sqq2;
@@ -2098,6 +2274,7 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     if (int rc2 = launch_screen<DT_BF16X3>(sb, grid, s)) return rc2;
     RerankArgs rb2 = ra;
     rb2.S = S;   // the fused screen's lists (also after a symmetric first pass)
+    rb2.C = Cf;
     rb2.sqq = B.sqq2;
     rb2.sqmax_bits = sqmax32;
     rb2.err_fac = static_cast<float>((3.0 * static_cast<double>(L.d_pad) + 3.0 * 256.0 + 40.0) *

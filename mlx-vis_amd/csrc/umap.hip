@@ -460,7 +460,8 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int64
 }
 
 // In-edge lists of the kept slots: counts per tail, exclusive scan, atomic
-// fill, then k_in_sort puts each node's entries in slot order.
+// fill, then k_in_sort puts the entries of each node with at most 64 of them
+// in slot order.
 __global__ __launch_bounds__(256) void k_in_count(const int32_t* __restrict__ tail, const float* __restrict__ eps,
                                                   int64_t m, uint32_t* __restrict__ cnt) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -539,68 +540,24 @@ __global__ __launch_bounds__(256) void k_scan_add_u32(uint32_t* __restrict__ out
 }
 
 // Orders each node's in-edge segment [off[j], off[j] + cnt[j]) by slot
-// (k_in_fill places them in atomic order), so the list, and the summation
-// order of every tail gradient, is the same on every run. One wavefront per
-// node with at most 1,024 entries: the entries are held in registers (16 per
-// lane), each one's rank is the number of smaller slots (slots are distinct),
-// and it is stored at its rank. Larger segments are left to k_in_sort_big.
+// (k_in_fill places them in atomic order), so the summation order of a tail
+// gradient sum is the same on every run. One wavefront per node: lane t holds
+// entry t, its rank is the number of smaller slots (slots are distinct), and
+// it is stored at its rank. Only segments of at most 64 entries are ordered:
+// a node with more in-edges spans three or more wavefronts of the tail sums,
+// whose partial sums reach it by atomics in arrival order anyway.
 __global__ __launch_bounds__(256) void k_in_sort(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
                                                  int64_t n, int32_t* __restrict__ in_slots) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   if (j >= n) return;
   const int c = static_cast<int>(cnt[j]);
-  if (c < 2 || c > 1024) return;
+  if (c < 2 || c > 64) return;
   int32_t* seg = in_slots + off[j];
-  int32_t v[16];
-  int rank[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    v[r] = r * 64 + l < c ? seg[r * 64 + l] : INT32_MAX;
-    rank[r] = 0;
-  }
-  const int rounds = (c + 63) >> 6;
-  for (int r2 = 0; r2 < rounds; ++r2) {
-    // every lane broadcasts its entry of round r2 in turn
-    int32_t w = INT32_MAX;
-#pragma unroll
-    for (int r = 0; r < 16; ++r)
-      if (r == r2) w = v[r];
-    for (int u = 0; u < 64; ++u) {
-      const int32_t x = __shfl(w, u);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) rank[r] += x < v[r] ? 1 : 0;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r)
-    if (r * 64 + l < c) seg[rank[r]] = v[r];
-}
-
-// Segments of more than 1,024 entries (hubs): one workgroup per node, ranks
-// counted against the segment in memory and staged in the node's in_node
-// entries (all equal to j), then copied back.
-__global__ __launch_bounds__(256) void k_in_sort_big(const uint32_t* __restrict__ cnt,
-                                                     const uint32_t* __restrict__ off, int64_t n,
-                                                     int32_t* __restrict__ in_slots, int32_t* __restrict__ in_node) {
-  for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
-    const int64_t c = cnt[j];
-    if (c <= 1024) continue;
-    int32_t* seg = in_slots + off[j];
-    int32_t* tmp = in_node + off[j];
-    for (int64_t p = threadIdx.x; p < c; p += blockDim.x) {
-      const int32_t x = seg[p];
-      int64_t rank = 0;
-      for (int64_t q = 0; q < c; ++q) rank += seg[q] < x ? 1 : 0;
-      tmp[rank] = x;
-    }
-    __syncthreads();
-    for (int64_t p = threadIdx.x; p < c; p += blockDim.x) {
-      seg[p] = tmp[p];
-      tmp[p] = static_cast<int32_t>(j);
-    }
-    __syncthreads();
-  }
+  const int32_t v = l < c ? seg[l] : INT32_MAX;
+  int rank = 0;
+  for (int u = 0; u < c; ++u) rank += __shfl(v, u) < v ? 1 : 0;
+  if (l < c) seg[rank] = v;
 }
 
 // Y += dY; dY = 0.
@@ -1127,9 +1084,6 @@ int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, in
   MV_LAUNCHED("k_in_fill");
   hipLaunchKernelGGL(k_in_sort, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, cnt, off, n, in_slots);
   MV_LAUNCHED("k_in_sort");
-  hipLaunchKernelGGL(k_in_sort_big, dim3(static_cast<unsigned>(n < 1024 ? n : 1024)), dim3(256), 0, s, cnt, off, n,
-                     in_slots, in_node);
-  MV_LAUNCHED("k_in_sort_big");
   return MV_OK;
 }
 

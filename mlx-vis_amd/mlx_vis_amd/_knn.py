@@ -26,7 +26,7 @@ def knn_brute_device(
     Q: torch.Tensor | None = None,
     *,
     dtype: str = "fp32",
-    rerank: int = 2,
+    rerank: int | None = None,
     q_self: torch.Tensor | None = None,
     exclude_self: bool | None = None,
     euclid: bool = True,
@@ -39,10 +39,16 @@ def knn_brute_device(
     Self exclusion: with ``Q is None`` the query row i never returns i (the
     +1e30 diagonal of _knn.py:64-69); ``q_self`` (nq,) int32 names the row of X
     each query must not return (-1 for none).
+    rerank: the screen nominates rerank * k candidates per row (at most 128
+    with the symmetric self-kNN screen, 64 otherwise). Default: 2, and for
+    the BF16 screen enough for 128, the candidate set its certificate needs on
+    clustered data whose 15th and 30th neighbours are close.
     Returns (idx int32 (nq, k), dist float32 (nq, k)) CUDA tensors.
     """
     if dtype not in _capi.DTYPES:
         raise ValueError(f"unknown kNN dtype {dtype!r}; expected one of {sorted(_capi.DTYPES)}")
+    if rerank is None:
+        rerank = max(2, -(-128 // int(k))) if dtype == "bf16" else 2
     _device.require_device("knn_brute_device", X, Q, stats)
     same = Q is None
     if X.dim() != 2 or (not same and (Q.dim() != 2 or X.shape[1] != Q.shape[1])):

@@ -253,20 +253,27 @@ def _c1_mixture():
     return _C1["X"]
 
 
-@pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp8"])
-def test_c1_full_size_matches_sklearn_on_sampled_rows(dtype):
+@pytest.mark.parametrize("dtype,rerank", [("fp32", None), ("bf16x3", None), ("bf16", None), ("bf16", 2),
+                                          ("fp8", None)])
+def test_c1_full_size_matches_sklearn_on_sampled_rows(dtype, rerank):
     """Self kNN (k = 15) of the full 70,000 x 784 mixture through every screen
     precision; the 2,000 rows of the committed fixture (scikit-learn brute
-    force in float64) must match bit for bit, distances within 1e-5."""
-    print(f"case c1 full dtype={dtype}", flush=True)
+    force in float64) must match bit for bit, distances within 1e-5. BF16 by
+    default nominates 128 candidates, enough for its certificate on this data;
+    with 30 (rerank 2) and with FP8 most rows fail it and go through the
+    split-BF16 re-screen."""
+    print(f"case c1 full dtype={dtype} rerank={rerank}", flush=True)
     X = _c1_mixture()
     g = np.load(Path(__file__).resolve().parent / "golden" / "knn_c1_sample2000.npz")
-    idx, dist, st = _run(X, 15, dtype=dtype)
+    kw = {} if rerank is None else {"rerank": rerank}
+    idx, dist, st = _run(X, 15, dtype=dtype, **kw)
     print("stats [scan, tiecheck, first-pass fail, re-screen fail]", st.tolist(), flush=True)
     rows = g["rows"]
     _check(idx[rows], dist[rows], g["idx"], g["dist2"])
-    if dtype in ("bf16", "fp8"):
+    if dtype == "fp8" or rerank == 2 and dtype == "bf16":
         assert st[2] > 0, "the quantised certificate is expected to fail on this data"
+    if dtype == "bf16" and rerank is None:
+        assert st[2] < 700, "128 BF16 candidates certify almost every row of this data"
 
 
 @pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp8"])
@@ -280,7 +287,7 @@ def test_c1_mixture_8192_rows_all_match_oracle(dtype):
     print(f"case c1 8192 dtype={dtype}", flush=True)
     X = np.ascontiguousarray(_c1_mixture()[:8192])
     oi, od = oknn.exact_knn(X, X, 15, self_base=0)
-    idx, dist, st = _run(X, 15, dtype=dtype)
+    idx, dist, st = _run(X, 15, dtype=dtype, rerank=2)
     print("stats symmetric", st.tolist(), flush=True)
     _check(idx, dist, oi, od)
     if dtype in ("bf16", "fp8"):
@@ -288,11 +295,16 @@ def test_c1_mixture_8192_rows_all_match_oracle(dtype):
     lib = _capi.lib()
     prev = lib.mv_knn_set_symmetric(0)
     try:
-        idx_f, dist_f, st_f = _run(X, 15, dtype=dtype)
+        idx_f, dist_f, st_f = _run(X, 15, dtype=dtype, rerank=2)
     finally:
         lib.mv_knn_set_symmetric(prev)
     print("stats fused", st_f.tolist(), flush=True)
     _check(idx_f, dist_f, oi, od)
+    # the symmetric screen's wider candidate sets (up to 128; rerank 9 -> 135 -> 128)
+    for rr in (5, 9):
+        idx_w, dist_w, st_w = _run(X, 15, dtype=dtype, rerank=rr)
+        print(f"stats symmetric rerank {rr}", st_w.tolist(), flush=True)
+        _check(idx_w, dist_w, oi, od)
 
 
 @pytest.mark.parametrize("case", ["2d_200k", "2d_ties", "1d", "3d_aniso", "queries_outside", "flat_dim"])
