@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--only", choices=["knn", "copy", "learned", "scale"], default=None,
                     help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8", "bf16x3"])
+    ap.add_argument("--rerank", type=int, default=None,
+                    help="candidates per row = rerank * k for --only knn (default: the library's per-dtype default)")
     ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
     ap.add_argument("--no-symmetric", action="store_true",
                     help="self kNN with the fused screen (k_screen) instead of the symmetric distance kernel")
@@ -79,7 +81,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def knn_phase_timing(Xd, reps, dtype="fp32", rerank=2):
+def knn_phase_timing(Xd, reps, dtype="fp32", rerank=None):
     """Times mv_knn_brute's phases with HIP events on its stream."""
     import ctypes
 
@@ -387,7 +389,7 @@ def main():
     torch.cuda.synchronize()
 
     if args.only == "knn":
-        phases, stats, _, scr = knn_phase_timing(Xd, args.knn_reps, args.dtype)
+        phases, stats, _, scr = knn_phase_timing(Xd, args.knn_reps, args.dtype, args.rerank)
         print(json.dumps({"knn_dtype": args.dtype, "ms_phases": phases, "stats": stats, "screen": scr,
                           "tflops_equiv_full_matrix": 2.0 * n * n * d / (phases[1] * 1e-3) / 1e12}), flush=True)
         return
