@@ -252,10 +252,12 @@ __global__ __launch_bounds__(256) void k_center_q(const float* __restrict__ X, i
 // lo = bf16(v - hi) (both subtractions exact in FP32), |r| <= 2^-16 |v|. Row
 // layout per 32-element group g (128 bytes): hi[32g .. 32g+31] then
 // lo[32g .. 32g+31], so one 128-byte chunk of the screen holds one K group.
+// gate (device, nullable): nothing is written while *gate == 0 (the re-screen
+// preparation of a call in which every row was certified).
 __global__ __launch_bounds__(256) void k_split_bf16(const float* __restrict__ Xc, int64_t count, int64_t d_pad,
-                                                    uint16_t* __restrict__ Xs) {
+                                                    uint16_t* __restrict__ Xs, const int32_t* __restrict__ gate) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= count) return;
+  if (t >= count || (gate && *gate == 0)) return;
   const int64_t r = t / d_pad, c = t % d_pad;
   const float v = Xc[t];
   const uint16_t hi = bf16_rne(v);
@@ -1384,11 +1386,40 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
 // Re-rank of up to 128 candidates: the symmetric screen's single list per row
 // (sorted by screen key), for the wider candidate sets a BF16 screen needs
 // before its certificate holds. Same steps as k_rerank with two candidates
-// per lane: FP32 direct differences of every candidate, a 128-key sort by
-// (FP32 distance, index), FP64 only where FP32 cannot separate two of the
-// first k + 1, and the certificate against the C-th screen key. When FP64 is
-// needed it orders the first 64; a row whose 65th candidate is not separated
-// from them goes to the exhaustive scan.
+// per lane: FP32 direct differences of the first 64 candidates; of the next
+// 64 only those whose screen value, less the screen's error bound, is not
+// above the k-th FP32 distance so far (plus its bound): the others cannot
+// enter the top k, they only raise the certificate's threshold to the C-th
+// screen key. Then a sort by (FP32 distance, index), FP64 only where FP32
+// cannot separate two of the first k + 1, and the certificate. When FP64 is
+// needed it orders the first 64; a row whose 65th computed candidate is not
+// separated from them goes to the exhaustive scan.
+__device__ __forceinline__ float rerank_dist32(const RerankArgs& a, const float* qrow, int j, int sub) {
+  float acc = 0.f;
+  const float* xr = a.X + static_cast<int64_t>(j) * a.d;
+  if (a.vec4) {
+    const float4* q4 = reinterpret_cast<const float4*>(qrow);
+    const float4* x4 = reinterpret_cast<const float4*>(xr);
+    for (int64_t t = sub; t < a.d / 4; t += 16) {
+      const float4 qa = q4[t], xb = x4[t];
+      float df = qa.x - xb.x;
+      acc = fmaf(df, df, acc);
+      df = qa.y - xb.y;
+      acc = fmaf(df, df, acc);
+      df = qa.z - xb.z;
+      acc = fmaf(df, df, acc);
+      df = qa.w - xb.w;
+      acc = fmaf(df, df, acc);
+    }
+  } else {
+    for (int64_t t = sub; t < a.d; t += 16) {
+      const float df = qrow[t] - xr[t];
+      acc = fmaf(df, df, acc);
+    }
+  }
+  return acc;
+}
+
 __global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + w;
@@ -1398,69 +1429,74 @@ __global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
   const uint64_t* cl = a.cand + static_cast<int64_t>(i) * C;
   const uint64_t c0 = l < C ? cl[l] : KEY_NONE;
   const uint64_t c1 = 64 + l < C ? cl[64 + l] : KEY_NONE;
-  const int nvalid = __popcll(ballot(c0 != KEY_NONE)) + __popcll(ballot(c1 != KEY_NONE));
+  const int n0 = __popcll(ballot(c0 != KEY_NONE));
+  const int nall = n0 + __popcll(ballot(c1 != KEY_NONE));
   const uint64_t tau_key = C > 64 ? readlane_u64(c1, C - 65) : readlane_u64(c0, C - 1);
   const int ci0 = c0 != KEY_NONE ? static_cast<int>(key_idx(c0)) : 0;
   const int ci1 = c1 != KEY_NONE ? static_cast<int>(key_idx(c1)) : 0;
-
-  // FP32 direct-difference distances, 4 candidates at a time, 16 lanes each:
-  // lane l ends with candidate l (d0) and candidate 64 + l (d1).
   const float* qrow = a.Q + static_cast<int64_t>(orow) * a.d;
   const int g = l >> 4, sub = l & 15;
+  const int64_t terms = a.vec4 ? 4 * ((a.d + 63) / 64) : (a.d + 15) / 16;
+  const float bfac = static_cast<float>(terms + 8) * 5.9604645e-8f * 1.01f;
+  const float sqmax = __uint_as_float(*a.sqmax_bits);
+  const float arith = a.err_fac * (a.sqq[i] + sqmax) + 1e-30f;
+
+  // FP32 direct differences, 4 candidates at a time, 16 lanes each: lane l
+  // ends with candidate l (d0).
   float d0 = INFINITY, d1 = INFINITY;
-  for (int r = 0; r * 4 < nvalid; ++r) {
+  for (int r = 0; r * 4 < n0; ++r) {
     const int c = r * 4 + g;
-    const int ja = __shfl(ci0, c & 63), jb = __shfl(ci1, c & 63);
-    const int j = c < 64 ? ja : jb;
-    float acc = 0.f;
-    if (c < nvalid) {
-      const float* xr = a.X + static_cast<int64_t>(j) * a.d;
-      if (a.vec4) {
-        const float4* q4 = reinterpret_cast<const float4*>(qrow);
-        const float4* x4 = reinterpret_cast<const float4*>(xr);
-        for (int64_t t = sub; t < a.d / 4; t += 16) {
-          const float4 qa = q4[t], xb = x4[t];
-          float df = qa.x - xb.x;
-          acc = fmaf(df, df, acc);
-          df = qa.y - xb.y;
-          acc = fmaf(df, df, acc);
-          df = qa.z - xb.z;
-          acc = fmaf(df, df, acc);
-          df = qa.w - xb.w;
-          acc = fmaf(df, df, acc);
-        }
-      } else {
-        for (int64_t t = sub; t < a.d; t += 16) {
-          const float df = qrow[t] - xr[t];
-          acc = fmaf(df, df, acc);
-        }
-      }
-    }
+    const int j = __shfl(ci0, c & 63);
+    float acc = c < n0 ? rerank_dist32(a, qrow, j, sub) : 0.f;
     acc += __shfl_xor(acc, 8);
     acc += __shfl_xor(acc, 4);
     acc += __shfl_xor(acc, 2);
     acc += __shfl_xor(acc, 1);
     const float got = __shfl(acc, (l & 3) * 16);
-    if (r < 16) {
-      if ((l >> 2) == r) d0 = got;
-    } else if ((l >> 2) == r - 16) {
-      d1 = got;
-    }
+    if ((l >> 2) == r) d0 = got;
   }
-
-  // Sort the (up to) 128 keys: each register sorted, the second descending,
-  // then one compare-exchange between them and a half-cleaner sort of each.
   uint64_t x0 = c0 != KEY_NONE ? dkey(d0, static_cast<uint32_t>(ci0)) : KEY_NONE;
-  uint64_t x1 = c1 != KEY_NONE ? dkey(d1, static_cast<uint32_t>(ci1)) : KEY_NONE;
   x0 = wave_sort_u64(x0);
-  x1 = wave_sort_u64<true>(x1);
-  {
+  // upper bound of the true k-th distance: the k-th FP32 value plus its bound
+  const float kd = key_dist(readlane_u64(x0, k - 1));
+  const float kth32_ub = kd + kd * bfac + 1e-35f;
+  bool need1 = false;
+  if (c1 != KEY_NONE) {
+    const float sv = key_dist(c1);
+    double lb;
+    if (!a.eq) {
+      lb = static_cast<double>(sv) - static_cast<double>(arith);
+    } else {
+      const double t = fmax(static_cast<double>(sv) - static_cast<double>(arith), 0.0);
+      const double r = sqrt(t) - static_cast<double>(a.eq[i]) - static_cast<double>(__uint_as_float(*a.emax_bits));
+      lb = r > 0.0 ? r * r * (1.0 - 1e-6) : 0.0;
+    }
+    need1 = lb <= static_cast<double>(kth32_ub);
+  }
+  // candidates are sorted by screen key, so the ones needed form a prefix
+  const int n1 = __popcll(ballot(need1));
+  for (int r = 0; r * 4 < n1; ++r) {
+    const int c = r * 4 + g;
+    const int j = __shfl(ci1, c & 63);
+    float acc = c < n1 ? rerank_dist32(a, qrow, j, sub) : 0.f;
+    acc += __shfl_xor(acc, 8);
+    acc += __shfl_xor(acc, 4);
+    acc += __shfl_xor(acc, 2);
+    acc += __shfl_xor(acc, 1);
+    const float got = __shfl(acc, (l & 3) * 16);
+    if ((l >> 2) == r) d1 = got;
+  }
+  const int nvalid = n0 + n1;
+
+  // Sort the computed keys: x0 is sorted, x1 sorted descending, then one
+  // compare-exchange between them and a half-cleaner sort of each.
+  uint64_t x1 = l < n1 ? dkey(d1, static_cast<uint32_t>(ci1)) : KEY_NONE;
+  if (n1 > 0) {
+    x1 = wave_sort_u64<true>(x1);
     const uint64_t lo = x0 < x1 ? x0 : x1, hi = x0 < x1 ? x1 : x0;
     x0 = bitonic_clean_u64(lo);
     x1 = bitonic_clean_u64(hi);
   }
-  const int64_t terms = a.vec4 ? 4 * ((a.d + 63) / 64) : (a.d + 15) / 16;
-  const float bfac = static_cast<float>(terms + 8) * 5.9604645e-8f * 1.01f;
   const float ds = key_dist(x0);
   const float bd = ds * bfac + 1e-35f;
   const float ds1 = key_dist(x1);
@@ -1477,7 +1513,7 @@ __global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
   uint32_t out_i = key_idx(x0);
   float out_d = ds;
   float kth_ub;
-  bool beyond_ok = true;   // no candidate past the 64th can enter the top k
+  bool beyond_ok = true;   // no computed candidate past the 64th can enter the top k
   if (any_amb) {
     double d64 = INFINITY;
     const bool v = l < nvalid;
@@ -1497,11 +1533,9 @@ __global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
     out_d = a.euclid ? sqrtf(ds) : ds;
   }
 
-  bool cert = beyond_ok && nvalid < C;
+  bool cert = beyond_ok && nall < C;
   if (beyond_ok && !cert) {
     const float tau = key_dist(tau_key);
-    const float sqmax = __uint_as_float(*a.sqmax_bits);
-    const float arith = a.err_fac * (a.sqq[i] + sqmax) + 1e-30f;
     if (!a.eq) {
       cert = kth_ub < tau - arith;
     } else {
@@ -2097,12 +2131,12 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
   if (dtype == MV_BF16X3) {
     const int64_t cx = L.n_pad * L.d_pad;
     hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cx, 256))), dim3(256), 0, s, B.Xc, cx,
-                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xq));
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xq), static_cast<const int32_t*>(nullptr));
     MV_LAUNCHED("k_split_bf16");
     if (!same) {
       const int64_t cq = L.nq_pad * L.d_pad;
       hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cq, 256))), dim3(256), 0, s, B.Qc, cq,
-                         L.d_pad, reinterpret_cast<uint16_t*>(B.Qq));
+                         L.d_pad, reinterpret_cast<uint16_t*>(B.Qq), static_cast<const int32_t*>(nullptr));
       MV_LAUNCHED("k_split_bf16");
     }
   } else if (dtype != MV_F32) {
@@ -2255,10 +2289,10 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     MV_LAUNCHED("k_gather_rows");
     const int64_t cxs = L.n_pad * L.d_pad, cqs = L.nq_pad * L.d_pad;
     hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cxs, 256))), dim3(256), 0, s, B.Xc, cxs,
-                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xs));
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xs), static_cast<const int32_t*>(fb1_count));
     MV_LAUNCHED("k_split_bf16");
     hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cqs, 256))), dim3(256), 0, s, B.Qc2, cqs,
-                       L.d_pad, reinterpret_cast<uint16_t*>(B.Qs2));
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Qs2), static_cast<const int32_t*>(fb1_count));
     MV_LAUNCHED("k_split_bf16");
     ScreenArgs sb = sa;
     sb.C = Cf;
