@@ -66,7 +66,7 @@ def parse():
                     help="fresh points mapped by the learned form (default 1,000,000)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form line")
     ap.add_argument("--no-scale", action="store_true", help="skip the 1M x 128 NNDescent + UMAP line")
-    ap.add_argument("--only", choices=["knn", "copy", "learned", "scale"], default=None,
+    ap.add_argument("--only", choices=["knn", "copy", "learned", "scale", "layout"], default=None,
                     help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8", "bf16x3"])
     ap.add_argument("--rerank", type=int, default=None,
@@ -392,6 +392,29 @@ def main():
         phases, stats, _, scr = knn_phase_timing(Xd, args.knn_reps, args.dtype, args.rerank)
         print(json.dumps({"knn_dtype": args.dtype, "ms_phases": phases, "stats": stats, "screen": scr,
                           "tflops_equiv_full_matrix": 2.0 * n * n * d / (phases[1] * 1e-3) / 1e12}), flush=True)
+        return
+
+    if args.only == "layout":
+        # the layout epochs alone: the 70,000 x 784 graph built once, then
+        # --knn-reps runs of the epochs from the same random init
+        from mlx_vis_amd import find_ab_params, knn_brute_device
+        from mlx_vis_amd._umap import fuzzy_graph_device, run_epochs
+
+        idx, dist = knn_brute_device(Xd, 15)
+        g = fuzzy_graph_device(idx, dist, args.epochs)
+        a, b = find_ab_params(1.0, 0.1)
+        Y0 = torch.from_numpy(np.random.RandomState(0).uniform(-10, 10, (n, 2)).astype(np.float32)).cuda()
+        times = []
+        for _ in range(args.knn_reps):
+            Y = Y0.clone()
+            g.epn.copy_(g.eps)
+            g.epn[g.eps <= 0] = float("inf")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            run_epochs(Y, None, g, 1, args.epochs, args.epochs, 5, 1.0, a, b, 42)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        print(json.dumps({"layout_epochs_s": times, "epochs": args.epochs - 1}), flush=True)
         return
 
     if args.only == "scale":

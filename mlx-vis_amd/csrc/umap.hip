@@ -208,18 +208,21 @@ __device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 
 
 // One synchronous epoch, phase 1, over the dense slot array. Each head row
 // is served by G lanes, each lane holding S slots (c, c + G, ...; G S >= k),
-// so that a lane keeps several independent gathers in flight. Loads that do
-// not depend on each other are issued first (slot schedules, tail indices,
-// in-edge positions, head coordinates), then the tail gathers; per chunk of
-// up to 8 negatives (Philox-4x32-10 counter (slot, epoch, group of 4)) all
-// slots' draws are computed and gathered before any gradient. The head's
-// updates are reduced over the row's lanes into the next positions Ynext[i]
-// (Y is double buffered); the tail's update (minus the attractive gradient)
-// is stored per slot to gtail. No
-// atomics: a point that is the tail of many edges (a hub of the kNN graph) is
-// summed in phase 2. In in-edge order (pos_in set) only active slots store,
-// since phase 2 clears what it reads; in slot order every slot stores (zero
-// when inactive).
+// so that the slot schedules, tail indices, in-edge positions and tail
+// positions load with several gathers in flight per lane. Only the active
+// slots of the epoch (on average a third at 70,000 x 784) carry the expensive
+// part, the negative draws and gradients: they are compacted per wavefront
+// through LDS (slot, in-edge position, head and tail coordinates, in slot
+// order u-major, lane-minor) and processed one item per lane, usually in a
+// single round instead of S partly idle ones. Per item, the negatives
+// (Philox-4x32-10 counter (slot, epoch, group of 4)) are drawn and gathered
+// before any gradient; the item's tail update (minus the attractive gradient)
+// is stored per slot to gtail, its head update goes back through LDS to the
+// slot's lane. The head's updates are reduced over the row's G lanes into the
+// next positions Ynext[i] (Y is double buffered). No atomics: a point that is
+// the tail of many edges (a hub of the kNN graph) is summed in phase 2. In
+// in-edge order (pos_in set) only active slots store, since phase 2 clears
+// what it reads; in slot order every slot stores (zero when inactive).
 constexpr int NEG_CHUNK = 8;
 
 template <int D, int G, int S>
@@ -230,14 +233,18 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
                                                const float* __restrict__ eps, float* __restrict__ epn, int64_t n,
                                                int k, int epoch, int neg_rate, Law law, uint32_t seed_lo,
                                                uint32_t seed_hi) {
+  constexpr int NI = 64 * S;   // active items of a wavefront, at most
+  __shared__ int32_t s_e[4][NI];
+  __shared__ int32_t s_pos[4][NI];
+  __shared__ float s_yi[4][NI][D];   // head position, then the item's head update
+  __shared__ float s_yj[4][NI][D];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;      // head row
   const int c = static_cast<int>(gid % G);
-  float acc[D];
-#pragma unroll
-  for (int q = 0; q < D; ++q) acc[q] = 0.f;
-  int64_t e[S], j[S], pos[S];
+  int64_t e[S], pos[S];
   float sch[S], nx[S];
+  int32_t j[S];
   bool in_row[S], act[S];
 #pragma unroll
   for (int u = 0; u < S; ++u) {
@@ -266,19 +273,48 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     if (act[u]) {
       epn[e[u]] = nx[u] + sch[u];
 #pragma unroll
-      for (int q = 0; q < D; ++q) yj[u][q] = Y[j[u] * D + q];
+      for (int q = 0; q < D; ++q) yj[u][q] = Y[static_cast<int64_t>(j[u]) * D + q];
     }
   }
-  for (int base = 0; base < neg_rate; base += NEG_CHUNK) {
-    float yt[S][NEG_CHUNK][D];
+  // compaction of the active slots: item rank[u] of this wavefront
+  int rank[S];
+  int A = 0;
 #pragma unroll
-    for (int u = 0; u < S; ++u) {
-      if (act[u]) {
-        const uint4 r0 = Philox::gen(make_uint4(static_cast<uint32_t>(e[u]), static_cast<uint32_t>(e[u] >> 32),
-                                                static_cast<uint32_t>(epoch), static_cast<uint32_t>(base >> 2)),
+  for (int u = 0; u < S; ++u) {
+    const uint64_t m = __ballot(act[u]);
+    rank[u] = A + static_cast<int>(__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
+    A += __popcll(m);
+    if (act[u]) {
+      s_e[w][rank[u]] = static_cast<int32_t>(e[u]);
+      s_pos[w][rank[u]] = static_cast<int32_t>(pos[u]);
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        s_yi[w][rank[u]][q] = yi[q];
+        s_yj[w][rank[u]][q] = yj[u][q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r * 64 < A; ++r) {
+    const int pi = r * 64 + l;
+    if (pi < A) {
+      const int32_t ei = s_e[w][pi];
+      const int64_t pe = s_pos[w][pi];
+      float hi_[D], hj[D], h[D];
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        hi_[q] = s_yi[w][pi][q];
+        hj[q] = s_yj[w][pi][q];
+        h[q] = 0.f;
+      }
+      for (int base = 0; base < neg_rate; base += NEG_CHUNK) {
+        float yt[NEG_CHUNK][D];
+        const uint4 r0 = Philox::gen(make_uint4(static_cast<uint32_t>(ei), 0u, static_cast<uint32_t>(epoch),
+                                                static_cast<uint32_t>(base >> 2)),
                                      seed_lo, seed_hi);
-        const uint4 r1 = Philox::gen(make_uint4(static_cast<uint32_t>(e[u]), static_cast<uint32_t>(e[u] >> 32),
-                                                static_cast<uint32_t>(epoch), static_cast<uint32_t>((base >> 2) + 1)),
+        const uint4 r1 = Philox::gen(make_uint4(static_cast<uint32_t>(ei), 0u, static_cast<uint32_t>(epoch),
+                                                static_cast<uint32_t>((base >> 2) + 1)),
                                      seed_lo, seed_hi);
         const uint32_t words[NEG_CHUNK] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 #pragma unroll
@@ -286,14 +322,9 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
           if (base + s2 < neg_rate) {
             const int64_t t = uniform_below(words[s2], static_cast<uint32_t>(n));
 #pragma unroll
-            for (int q = 0; q < D; ++q) yt[u][s2][q] = Y[t * D + q];
+            for (int q = 0; q < D; ++q) yt[s2][q] = Y[t * D + q];
           }
         }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < S; ++u) {
-      if (act[u]) {
 #pragma unroll
         for (int s2 = 0; s2 < NEG_CHUNK; ++s2) {
           if (base + s2 < neg_rate) {
@@ -301,42 +332,45 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
             float n2 = 0.f;
 #pragma unroll
             for (int q = 0; q < D; ++q) {
-              nd[q] = yi[q] - yt[u][s2][q];
+              nd[q] = hi_[q] - yt[s2][q];
               n2 += nd[q] * nd[q];
             }
             const float cr = law.repulse(n2);
 #pragma unroll
-            for (int q = 0; q < D; ++q) acc[q] += clip4(cr * nd[q]) * law.alpha;
+            for (int q = 0; q < D; ++q) h[q] += clip4(cr * nd[q]) * law.alpha;
           }
         }
       }
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < S; ++u) {
-    float gt[D];
-#pragma unroll
-    for (int q = 0; q < D; ++q) gt[q] = 0.f;
-    if (act[u]) {
       float diff[D];
       float d2 = 0.f;
 #pragma unroll
       for (int q = 0; q < D; ++q) {
-        diff[q] = yi[q] - yj[u][q];
+        diff[q] = hi_[q] - hj[q];
         d2 += diff[q] * diff[q];
       }
       const float ca = law.attract(d2);
 #pragma unroll
       for (int q = 0; q < D; ++q) {
         const float g = clip4(ca * diff[q]) * law.alpha;
-        acc[q] += g;
-        gt[q] = -g;
+        h[q] += g;
+        gtail[pe * D + q] = -g;
+        s_yi[w][pi][q] = h[q];
       }
     }
-    // pos < 0: a pruned slot (in-edge order)
-    if (pos[u] >= 0 && (act[u] || (in_row[u] && !pos_in))) {
+  }
+  __syncthreads();
+  float acc[D];
 #pragma unroll
-      for (int q = 0; q < D; ++q) gtail[pos[u] * D + q] = gt[q];
+  for (int q = 0; q < D; ++q) acc[q] = 0.f;
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    if (act[u]) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) acc[q] += s_yi[w][rank[u]][q];
+    } else if (in_row[u] && !pos_in) {
+      // slot order: phase 2 reads every slot
+#pragma unroll
+      for (int q = 0; q < D; ++q) gtail[e[u] * D + q] = 0.f;
     }
   }
   // Reduce the head's updates over its G lanes (contiguous in the wavefront).
@@ -1103,7 +1137,7 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
   MV_REQUIRE(n >= 1 && k >= 1 && k <= 64, "mv_umap_epochs: bad n=%lld k=%d", (long long)n, k);
   MV_REQUIRE(n_in >= 0 && n_in <= n * k, "mv_umap_epochs: bad in-edge count %lld", (long long)n_in);
   MV_REQUIRE(dim >= 1 && dim <= 4, "mv_umap_epochs: dim=%d outside [1, 4]", dim);
-  MV_REQUIRE(n < (1ll << 31), "mv_umap_epochs: n too large");
+  MV_REQUIRE(n * k < (1ll << 31), "mv_umap_epochs: n * k = %lld slots, at most 2^31 - 1", (long long)(n * k));
   MV_REQUIRE(0 <= e0 && e0 <= e1 && e1 <= n_epochs, "mv_umap_epochs: bad epoch range [%d, %d) of %d", e0, e1,
              n_epochs);
   MV_REQUIRE(neg_rate >= 0, "mv_umap_epochs: negative_sample_rate < 0");
