@@ -298,63 +298,108 @@ def scale_form(n=1_000_000, sample=10_000):
     return out
 
 
-def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs=8, spec_iters=5):
-    """The oracle restatement of fit_transform on a bounded sample, scaled."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs=8, spec_iters=5, c0_runs=3):
+    """The oracle restatement (NumPy with BLAS threads, torch-CPU index_add_
+    scatters), not MLX: a full run of the small plain-method workload
+    (10,000 x 128, 200 epochs; one warm-up on 2,000 rows, then the median of
+    c0_runs full runs), and the 70,000 x 784 fit_transform extrapolated from
+    a bounded sample (kNN rows, spectral iterations and layout epochs scaled
+    up)."""
     import numpy as np
+    import torch
 
     from oracle import knn as oknn
     from oracle import umap as ou
 
+    cores = min(16, len(os.sched_getaffinity(0)))
     try:
-        from threadpoolctl import threadpool_info
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # noqa: BLE001
+        threadpool_limits = None
+    torch_threads = torch.get_num_threads()
+    # the layout's small per-epoch tensors run fastest on one thread (measured:
+    # 17 ms per 56 K-edge step on 1 thread, 290 ms on 8)
+    torch.set_num_threads(1)
+    ctx = threadpool_limits(limits=cores) if threadpool_limits else None
+    try:
+        if ctx is not None:
+            ctx.__enter__()
+        # full runs of the small workload
+        Xc0 = np.random.default_rng(0).standard_normal((10000, 128), dtype=np.float32)
+        ou.fit_transform_random_init(Xc0[:2000], n_epochs=20)
+        c0 = []
+        for _ in range(c0_runs):
+            t0 = time.perf_counter()
+            ou.fit_transform_random_init(Xc0, n_epochs=200)
+            c0.append(time.perf_counter() - t0)
 
-        cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:  # noqa: BLE001
-        cores = 1
-    n = X.shape[0]
-    rng = np.random.default_rng(0)
-    rows = np.sort(rng.choice(n, budget_rows, replace=False))
-    t0 = time.perf_counter()
-    oknn.reference_knn_rows(X, rows, 15)
-    t_knn = (time.perf_counter() - t0) * n / budget_rows
+        # the 70,000 x 784 workload on a bounded sample, scaled
+        n = X.shape[0]
+        rng = np.random.default_rng(0)
+        rows = np.sort(rng.choice(n, budget_rows, replace=False))
+        t0 = time.perf_counter()
+        oknn.reference_knn_rows(X, rows, 15)
+        t_knn = (time.perf_counter() - t0) * n / budget_rows
 
-    t0 = time.perf_counter()
-    r, c, v = ou.fuzzy_simplicial_set(knn_idx, knn_dist, n_epochs)
-    ou.find_ab_params(1.0, 0.1)
-    t_graph = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r, c, v = ou.fuzzy_simplicial_set(knn_idx, knn_dist, n_epochs)
+        ou.find_ab_params(1.0, 0.1)
+        t_graph = time.perf_counter() - t0
 
-    t0 = time.perf_counter()
-    ou.spectral_power(r, c, v.astype(np.float64), n, rng.standard_normal((n, 3)), iters=spec_iters)
-    t_spec = (time.perf_counter() - t0) * 100 / spec_iters
+        t0 = time.perf_counter()
+        ou.spectral_power(r, c, v.astype(np.float64), n, rng.standard_normal((n, 3)), iters=spec_iters)
+        t_spec = (time.perf_counter() - t0) * 100 / spec_iters
 
-    sets = ou.schedule(v, n_epochs)
-    total = sum(s.size for s in sets)
-    ep = [e for e in range(n_epochs) if sets[e].size][:budget_epochs]
-    Y = rng.uniform(-10, 10, size=(n, 2)).astype(np.float32)
-    done = 0
-    t0 = time.perf_counter()
-    for e in ep:
-        act = sets[e]
-        A = act.size
-        ef, et = r[act], c[act]
-        nf = ef[np.arange(5 * A) % A]
-        nt = rng.integers(0, n, size=5 * A)
-        Y = ou.sgd_step(Y, ef, et, nf, nt, np.float32(1.0 - e / n_epochs), np.float32(1.577),
-                        np.float32(0.895), dtype=np.float32)
-        done += A
-    t_layout = (time.perf_counter() - t0) * total / max(done, 1)
+        sets = ou.schedule(v, n_epochs)
+        total = sum(s_.size for s_ in sets)
+        ep = [e for e in range(n_epochs) if sets[e].size][:budget_epochs]
+        Y = torch.from_numpy(rng.uniform(-10, 10, size=(n, 2)).astype(np.float32))
+        rt, ct = torch.from_numpy(r.astype(np.int64)), torch.from_numpy(c.astype(np.int64))
+        done = 0
+        t0 = time.perf_counter()
+        for e in ep:
+            act = torch.from_numpy(sets[e].astype(np.int64))
+            A = act.numel()
+            ef, et = rt[act], ct[act]
+            nf = ef[torch.arange(5 * A) % A]
+            nt = torch.from_numpy(rng.integers(0, n, size=5 * A))
+            Y = ou.sgd_step_index_add(Y, ef, et, nf, nt, float(1.0 - e / n_epochs), 1.577, 0.895)
+            done += A
+        t_layout = (time.perf_counter() - t0) * total / max(done, 1)
+    finally:
+        if ctx is not None:
+            ctx.__exit__(None, None, None)
+        torch.set_num_threads(torch_threads)
     total_s = t_knn + t_graph + t_spec + t_layout
     return {
         "value": round(total_s, 2),
         "unit": "s",
         "cores": int(cores),
         "kind": "port",
-        "sample": (f"NumPy restatement of the reference (oracle), not MLX: kNN on {budget_rows} of {n} query rows "
-                   f"(x{n / budget_rows:.0f}); fuzzy graph + a/b in full; spectral init {spec_iters} of 100 "
-                   f"iterations (x{100 / spec_iters:.0f}); layout epochs {ep[0]}-{ep[-1]} ({len(ep)} epochs, "
-                   f"scaled by edge samples {total}/{done})"),
+        "sample": (f"NumPy restatement of the reference (oracle), not MLX, extrapolated: kNN (FP32 expansion + "
+                   f"full-row argsort) on {budget_rows} of {n} query rows (x{n / budget_rows:.0f}); fuzzy graph + "
+                   f"a/b in full; spectral init {spec_iters} of 100 iterations (x{100 / spec_iters:.0f}); layout "
+                   f"epochs {ep[0]}-{ep[-1]} ({len(ep)} epochs, torch-CPU index_add_, scaled by edge samples "
+                   f"{total}/{done})"),
         "parts_s": {"knn": round(t_knn, 2), "graph": round(t_graph, 2), "spectral": round(t_spec, 2),
                     "layout": round(t_layout, 2)},
+        "cpu_model": cpu_model(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "threads": {"blas": int(cores), "torch_layout": 1},
+        "small_workload_full_run": {
+            "workload": "10,000 x 128 N(0,1) seed 0, FP32 brute kNN k=15 (full-row argsort), 200 epochs, "
+                        "5 negatives, random init",
+            "runs_s": [round(t, 2) for t in c0], "median_s": round(float(np.median(c0)), 2)},
     }
 
 
@@ -445,6 +490,20 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / max(args.steps, 1)
     log(f"fit_transform {dt * 1000:.1f} ms")
+    # the same call with the NumPy array as input: the host -> device copy of
+    # X (220 MB) and the device -> host copy of the embedding inside the call
+    t = time.perf_counter()
+    for _ in range(3):
+        UMAP(n_neighbors=15, min_dist=0.1, n_epochs=args.epochs, negative_sample_rate=5, knn_method="brute",
+             random_state=42).fit_transform(X)
+    torch.cuda.synchronize()
+    dt_numpy = (time.perf_counter() - t) / 3
+    # peak device memory of one call (workspaces are torch allocations)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    step()
+    torch.cuda.synchronize()
+    peak_bytes = torch.cuda.max_memory_allocated()
 
     # phase breakdown of one more run (synchronised between phases; not timed above)
     from mlx_vis_amd import knn_brute_device
@@ -529,6 +588,8 @@ def main():
                 # screen multiplies only half of the pairs), not a hardware rate
                 "tflops_equiv_whole_knn": round(flops / (knn_total_ms * 1e-3) / 1e12, 2)},
         "phases_s": {k: round(v, 4) for k, v in ph.items()},
+        "numpy_input_fit_transform_s": round(dt_numpy, 4),
+        "peak_device_memory_gb": round(peak_bytes / 1e9, 2),
         "layout": layout,
         "quality": {"knn15_preservation": round(quality, 4), "trustworthiness_15": round(trust, 5),
                     "trustworthiness_s": round(t_trust, 4), "n_edges": g.n_edges},

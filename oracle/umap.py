@@ -197,3 +197,59 @@ def spectral_scale(V, noise):
     e = e + (np.asarray(noise, dtype=F32) * F32(1e-4)).astype(F32)
     lo, hi = e.min(axis=0), e.max(axis=0)
     return (F32(10.0) * (e - lo) / (hi - lo + F32(1e-10))).astype(F32)
+
+
+def sgd_step_index_add(Y, ef, et, nf, nt, alpha, a, b):
+    """_sgd_step (umap.py:361-387) in FP32 with torch-CPU index_add_ for the
+    three scatter-adds (the reference's `.at[].add`), for the CPU baseline:
+    same arithmetic as ``sgd_step(dtype=np.float32)``, faster scatters than
+    np.add.at. Y: (n, dim) float32 torch tensor (CPU); index arrays int64
+    tensors; returns the new Y."""
+    import torch
+
+    a = torch.tensor(a, dtype=torch.float32)
+    b = torch.tensor(b, dtype=torch.float32)
+    diff = Y[ef] - Y[et]
+    d2 = torch.clamp((diff * diff).sum(dim=1, keepdim=True), min=1e-6)
+    coeff = -2.0 * a * b * torch.pow(d2, b - 1.0) / (1.0 + a * torch.pow(d2, b))
+    pos = torch.clamp(coeff * diff, -4.0, 4.0) * alpha
+    nd = Y[nf] - Y[nt]
+    n2 = torch.clamp((nd * nd).sum(dim=1, keepdim=True), min=1e-6)
+    ncoeff = 2.0 * b / ((0.001 + n2) * (1.0 + a * torch.pow(n2, b)))
+    neg = torch.clamp(ncoeff * nd, -4.0, 4.0) * alpha
+    out = Y.clone()
+    out.index_add_(0, ef, pos)
+    out.index_add_(0, et, -pos)
+    out.index_add_(0, nf, neg)
+    return out
+
+
+def fit_transform_random_init(X, n_neighbors=15, n_epochs=200, min_dist=0.1, neg_rate=5, seed=0):
+    """The reference's fit_transform (umap.py:75-144) with a random-uniform
+    [-10, 10] init, restated for the CPU baseline: the FP32 brute kNN with
+    full-row argsort (knn.reference_knn), the fuzzy set, a/b, the schedule and
+    the epoch loop with torch-CPU scatters. Returns the (n, 2) embedding."""
+    import torch
+
+    from .knn import reference_knn
+
+    X = np.asarray(X, dtype=F32)
+    n = X.shape[0]
+    idx, dist = reference_knn(X, n_neighbors)
+    rows, cols, vals = fuzzy_simplicial_set(idx, dist, n_epochs)
+    a, b = find_ab_params(1.0, min_dist)
+    rng = np.random.default_rng(seed)
+    Y = torch.from_numpy(rng.uniform(-10.0, 10.0, size=(n, 2)).astype(F32))
+    sets = schedule(vals, n_epochs)
+    r_t, c_t = torch.from_numpy(rows.astype(np.int64)), torch.from_numpy(cols.astype(np.int64))
+    for epoch in range(n_epochs):
+        act = sets[epoch]
+        if act.size == 0:
+            continue
+        A = act.size
+        at = torch.from_numpy(act.astype(np.int64))
+        ef, et = r_t[at], c_t[at]
+        nf = ef[torch.arange(neg_rate * A) % A]
+        nt = torch.from_numpy(rng.integers(0, n, size=neg_rate * A))
+        Y = sgd_step_index_add(Y, ef, et, nf, nt, float(1.0 - epoch / n_epochs), a, b)
+    return Y.numpy()
