@@ -147,9 +147,9 @@ int mv_row_sqnorms(const float* X, int64_t n, int64_t d, float* out, void* strea
  * init_idx (n, k) (nndescent.py:54-56), P1 (d, p1) and P2 (d, p2) (NULL with
  * p = 0 when the projection is not smaller than d). out_changed (HOST, n_iters
  * entries, nullable): changed slots per iteration; out_iters (HOST, nullable):
- * iterations run. out_dist: Euclidean. Limits: k + min(max_candidates, k) * k
- * + k <= 1024 candidate slots per row (max_candidates 0 means k; k <= 31
- * then), d <= 4096.
+ * iterations run. out_dist: Euclidean. Limits: k <= 64 (any max_candidates;
+ * rows with more than 1,024 candidate slots are merged 1,024 at a time),
+ * d <= 4096.
  * One device-to-host read of a counter per iteration synchronises `stream`.
  */
 int mv_nnd_ws_bytes(int64_t n, int64_t d, int k, int64_t p1, int64_t p2, size_t* out_bytes);

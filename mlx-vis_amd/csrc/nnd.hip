@@ -365,6 +365,141 @@ __global__ __launch_bounds__(256) void k_nnd_update(UpdateArgs a) {
   if (l == 0 && fm) atomicAdd(a.changed, static_cast<uint32_t>(__popcll(fm)));
 }
 
+// Rows with more candidate slots than one pass holds (k + mc k + k > 1024,
+// from k = 32 at the default max_candidates): the slots are taken in chunks
+// of 1024 (NQ = 16 per lane) and each chunk is merged into the row's running
+// top-k list, smallest key first, one copy of each key. Copies of an id carry
+// the same key, since a pair's distance does not depend on the slot that
+// computes it, except a current neighbour kept with its stored distance
+// (recompute_cur == 0): a new candidate equal to a current neighbour is then
+// skipped, which is the reference's rule that the earliest slot of an id keeps
+// its distance and later copies drop out (nndescent.py:441-449). The self
+// slot drops out as well; a row with fewer than k distinct candidates (only
+// possible for tiny n) keeps its remaining current entries.
+__global__ __launch_bounds__(256) void k_nnd_update_chunked(UpdateArgs a) {
+  constexpr int NQ = NND_NQ_MAX;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (i >= a.n) return;
+  const int k = a.k;
+  const int64_t bytes_per_wave = a.src.xh ? a.src.ld * 2 : a.src.ld * 4;
+  float* lds_f = reinterpret_cast<float*>(smem + w * bytes_per_wave);
+  __half* lds_h = reinterpret_cast<__half*>(smem + w * bytes_per_wave);
+
+  const int32_t cur_id = l < k ? a.idx[i * k + l] : -1;
+  const float cur_d = l < k ? a.dist[i * k + l] : 0.f;
+  const bool cur_new = l < k ? a.flags[i * k + l] != 0 : false;
+  if (a.converging && ballot(cur_new) == 0) {
+    if (l < k) {
+      a.out_idx[i * k + l] = cur_id;
+      a.out_dist[i * k + l] = cur_d;
+      a.out_flags[i * k + l] = 0;
+    }
+    return;
+  }
+  int32_t first = cur_id;
+  if (a.converging) {
+    const uint64_t nm = ballot(cur_new && l < k);
+    const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
+    const int n_new = __popcll(nm);
+    int pos;
+    if (l >= k) pos = l;
+    else if (cur_new) pos = __popcll(nm & below);
+    else pos = n_new + (l - __popcll(nm & below));
+    first = __builtin_amdgcn_ds_permute(pos * 4, cur_id);
+  }
+  const int n_nn = a.mc_new * a.j_nn;
+  const int n_rev = a.rev ? k : 0;
+  const int total = k + n_nn + n_rev;
+
+  stage_row(a.src, i, lds_f, lds_h, l);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const float sqi = a.src.sq[i];
+  const int g = l >> 4, sub = l & 15;
+
+  uint64_t lk = KEY_NONE;   // running list: lane l holds the l-th smallest key (l < k)
+  for (int base = 0; base < total; base += 64 * NQ) {
+    int32_t sid[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int s = base + 64 * q + l;
+      const int rel = s - k;
+      const int m = (rel >= 0 && rel < n_nn) ? rel / a.j_nn : 0;
+      const int t = (rel >= 0 && rel < n_nn) ? rel % a.j_nn : 0;
+      const int32_t srow = __shfl(first, m);
+      int32_t id = -1;
+      if (s < k) id = cur_id;
+      else if (s < k + n_nn) id = a.idx[static_cast<int64_t>(srow) * k + t];
+      else if (s < total) id = a.rev[i * k + (s - k - n_nn)];
+      sid[q] = id;
+    }
+    // a new candidate equal to a current neighbour kept with its stored distance
+    if (!a.recompute_cur) {
+      for (int u = 0; u < k; ++u) {
+        const int32_t cu = __shfl(cur_id, u);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (base + 64 * q + l >= k && sid[q] == cu) sid[q] = -1;
+      }
+    }
+    float sd[NQ];
+    const int chunk = total - base < 64 * NQ ? total - base : 64 * NQ;
+    const int rounds = (chunk + 3) / 4;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      sd[q] = 0.f;
+      for (int rr = 0; rr < 16 && 16 * q + rr < rounds; ++rr) {
+        const int r = 16 * q + rr;
+        const int sl = 4 * r + g;   // slot within the chunk
+        const int32_t id = __shfl(sid[q], sl & 63);
+        const bool stored = base + sl < k && !a.recompute_cur;
+        const bool live = sl < chunk && id >= 0 && id != i && !stored;
+        const float dot = group_dot(a.src, id, lds_f, lds_h, sub, live);
+        float v = 0.f;
+        if (live) v = pair_dist(a.src, sqi, id, dot);
+        const float got = __shfl(v, (l & 3) * 16);
+        if ((l >> 2) == rr) sd[q] = got;
+      }
+    }
+    uint64_t key[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int s = base + 64 * q + l;
+      float dv = sd[q];
+      if (s < k && !a.recompute_cur) dv = cur_d;
+      const bool ok = s < total && sid[q] >= 0 && sid[q] != static_cast<int32_t>(i);
+      key[q] = ok ? dkey(dv, static_cast<uint32_t>(sid[q])) : KEY_NONE;
+    }
+    // merge: the k smallest distinct keys of the list and the chunk
+    uint64_t outk = KEY_NONE;
+    for (int produced = 0; produced < k; ++produced) {
+      uint64_t lm = lk;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) lm = key[q] < lm ? key[q] : lm;
+      const uint64_t m = wave_min_u64(lm);
+      if (m == KEY_NONE) break;
+      lk = lk == m ? KEY_NONE : lk;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) key[q] = key[q] == m ? KEY_NONE : key[q];
+      if (l == produced) outk = m;
+    }
+    lk = outk;
+  }
+  const bool have = l < k && lk != KEY_NONE;
+  const int32_t nid = have ? static_cast<int32_t>(key_idx(lk)) : cur_id;
+  const float nd = have ? key_dist(lk) : cur_d;
+  const bool flag = l < k && nid != cur_id;
+  if (l < k) {
+    a.out_idx[i * k + l] = nid;
+    a.out_dist[i * k + l] = nd;
+    a.out_flags[i * k + l] = flag ? 1 : 0;
+  }
+  const uint64_t fm = ballot(flag);
+  if (l == 0 && fm) atomicAdd(a.changed, static_cast<uint32_t>(__popcll(fm)));
+}
+
 // Distances of (row, idx[row, c]) for all c with src; FP32 (or FP16) like
 // _gather_dists(_half). Optionally sorts each row by (distance, position)
 // (the init sort, nndescent.py:62-65).
@@ -634,10 +769,7 @@ static int launch_update(const Plan& P, const Src& src, const int32_t* idx, cons
   if (total <= 64 * 4) hipLaunchKernelGGL(k_nnd_update<4>, grid, dim3(256), lds, s, a);
   else if (total <= 64 * 8) hipLaunchKernelGGL(k_nnd_update<8>, grid, dim3(256), lds, s, a);
   else if (total <= 64 * NND_NQ_MAX) hipLaunchKernelGGL(k_nnd_update<NND_NQ_MAX>, grid, dim3(256), lds, s, a);
-  else {
-    mv::set_error("k_nnd_update: %d candidate slots exceed %d", total, 64 * NND_NQ_MAX);
-    return MV_EINVAL;
-  }
+  else hipLaunchKernelGGL(k_nnd_update_chunked, grid, dim3(256), lds, s, a);
   MV_LAUNCHED("k_nnd_update");
   return MV_OK;
 }
@@ -737,11 +869,6 @@ int mv_nnd_build(const float* X, int64_t n, int64_t d, int k, int n_iters, int m
   if (align_up(d, 8) > static_cast<size_t>(MAX_LD)) {
     mv::set_error("mv_nnd_build: d=%lld exceeds the %d-element LDS row", (long long)d, MAX_LD);
     return MV_EUNSUPPORTED;
-  }
-  {
-    const int mcc = std::min(mc, k);
-    MV_REQUIRE(k + mcc * k + k <= 64 * NND_NQ_MAX,
-               "mv_nnd_build: k=%d (max_candidates %d) needs more than %d candidate slots", k, mc, 64 * NND_NQ_MAX);
   }
   Plan P = make_plan(n, d, k, p1, p2);
   if (!ws || ws_bytes < P.bytes) {

@@ -22,16 +22,15 @@ import torch
 from . import _capi, _device
 
 
-# Candidate slots per row of the device update (one wavefront, sixteen keys per
-# lane at most): the current k, mc * k neighbours of neighbours and k reverse ones.
-NND_SLOTS = 1024
+# Largest k of the device NNDescent: one wavefront per row holds the row's k
+# neighbours, one per lane; its candidates (the current k, mc * k neighbours of
+# neighbours and k reverse ones) are taken 1,024 at a time.
+NND_K_MAX = 64
 
 
 def nndescent_fits(k: int, max_candidates: int | None = None) -> bool:
-    """Whether the device NNDescent holds a row's candidates at this k (the
-    reference's nn-of-nn expansion, nndescent.py:100-101, 221-280)."""
-    mc = min(max_candidates or k, k)
-    return k + mc * k + k <= NND_SLOTS
+    """Whether the device NNDescent handles this k (any max_candidates)."""
+    return 1 <= k <= NND_K_MAX
 
 
 class NNDescent:

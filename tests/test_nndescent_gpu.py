@@ -68,10 +68,12 @@ def test_gather_dists(half):
 
 
 @pytest.mark.parametrize("n,d,seed,k", [(4000, 64, 3, 15), (3000, 784, 4, 15), (6000, 128, 5, 15),
-                                        (3000, 64, 7, 20), (2500, 48, 8, 30)])
+                                        (3000, 64, 7, 20), (2500, 48, 8, 30), (2500, 40, 11, 32),
+                                        (1500, 24, 12, 48), (900, 16, 13, 64)])
 def test_build_recall_not_below_reference(n, d, seed, k):
     """k = 15 uses 256 candidate slots per row, k = 20 and 30 (max_candidates
-    = k: k + k^2 + k slots) the 512- and 1024-slot variants."""
+    = k: k + k^2 + k slots) the 512- and 1024-slot variants, k = 32, 48 and 64
+    (1,088 to 4,224 slots) the chunked merge."""
     from mlx_vis_amd import NNDescent
 
     print(f"case n={n} d={d} k={k}", flush=True)
@@ -111,12 +113,12 @@ def test_knn_method_nndescent_via_compute_knn():
     np.testing.assert_allclose(d1 ** 2, d2, rtol=1e-6)
 
 
-def test_build_rejects_k_beyond_the_candidate_slots():
+def test_build_rejects_k_beyond_64():
     from mlx_vis_amd import NNDescent
 
     X = _data(500, 8, 9)
     with pytest.raises(ValueError):
-        NNDescent(k=32, random_state=0).build(X)   # 32 + 32 * 32 + 32 > 1024 slots
+        NNDescent(k=65, random_state=0).build(X)
 
 
 def _fixture(name):
