@@ -205,7 +205,7 @@ int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, in
  * against nodes drawn uniformly from [0, n) by Philox-4x32-10 keyed by seed;
  * the step size is lr * (1 - e / n_epochs). tail = the kNN indices (n, k);
  * (in_slots, in_node, n_in) from mv_umap_in_edges. ws sized by
- * mv_umap_epochs_ws_bytes. dim <= 4, k <= 64. */
+ * mv_umap_epochs_ws_bytes. dim <= 8, k <= 64. */
 int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes);
 int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
                    const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1, int n_epochs,
@@ -232,7 +232,7 @@ int mv_umap_sgd_step(float* Y_out, const float* Y_in, int64_t n, int dim, const 
  * (umap.py:298-340): `iters` products of D^-1/2 W D^-1/2 (degrees from the
  * heads of the kept slots, umap.py:307-314) with re-orthonormalisation of the
  * nc columns after each product. V (n, nc) holds the start vectors on entry
- * and the orthonormal result on return. 2 <= nc <= 5. */
+ * and the orthonormal result on return. 2 <= nc <= 9. */
 int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes);
 int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, int64_t n, int k, int nc,
                      int iters, float* V, void* ws, size_t ws_bytes, void* stream);
@@ -354,7 +354,7 @@ int mv_random_normal(float* out, int64_t count, uint64_t seed, void* stream);
  * Philox-4x32-10 keyed by seed (the reference draws it from NumPy's legacy
  * generator). out_nonfinite (device int32, nullable) receives 1 if V holds a
  * non-finite value (the reference then falls back to a random init).
- * ws: >= 64 bytes of device memory. */
+ * 2 <= nc <= 9. ws: >= 128 bytes of device memory. */
 int mv_umap_spectral_embed(const float* V, int64_t n, int nc, uint64_t seed, float* Y, int32_t* out_nonfinite,
                            void* ws, size_t ws_bytes, void* stream);
 

@@ -40,6 +40,7 @@ namespace mv {
 namespace umap {
 
 constexpr int KMAX = 64;
+constexpr int DIM_MAX = 8;   // largest embedding dimension of the layout kernels
 
 __global__ void k_zero_u32(uint32_t* p, int count) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -805,6 +806,9 @@ __global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U,
   }
 }
 
+constexpr int NC_MAX = DIM_MAX + 1;   // spectral vectors (the first is dropped)
+constexpr int GRAM_STRIDE = 48;       // >= NC_MAX (NC_MAX + 1) / 2 packed entries
+
 template <int NC>
 static int spectral_run(const float* wsym, const float* eps, const int32_t* tail, int64_t n, int k, int iters,
                         float* V, char* ws, hipStream_t s) {
@@ -813,15 +817,15 @@ static int spectral_run(const float* wsym, const float* eps, const int32_t* tail
   float* wn = cv.take<float>(static_cast<size_t>(n) * k);
   float* U0 = cv.take<float>(static_cast<size_t>(n) * NC);
   float* U1 = cv.take<float>(static_cast<size_t>(n) * NC);
-  double* gram = cv.take<double>(3 * 16);   // three rotating packed Gram buffers
+  double* gram = cv.take<double>(3 * GRAM_STRIDE);   // three rotating packed Gram buffers
   const unsigned rb = static_cast<unsigned>(ceil_div(n, 256));
   hipLaunchKernelGGL(k_spec_prep, dim3(rb), dim3(256), 0, s, wsym, eps, n, k, dis);
   MV_LAUNCHED("k_spec_prep");
   hipLaunchKernelGGL(k_spec_wnorm, dim3(static_cast<unsigned>(ceil_div(n * k, 256))), dim3(256), 0, s, wsym, eps,
                      tail, dis, n, k, wn);
   MV_LAUNCHED("k_spec_wnorm");
-  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(128), 0, s, reinterpret_cast<float*>(gram),
-                     static_cast<int64_t>(2 * 3 * 16));
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, reinterpret_cast<float*>(gram),
+                     static_cast<int64_t>(2 * 3 * GRAM_STRIDE));
   MV_LAUNCHED("k_zero_f32");
   hipLaunchKernelGGL(k_copy_f32, dim3(static_cast<unsigned>(ceil_div(n * NC, 256))), dim3(256), 0, s, U0, V,
                      n * NC);
@@ -830,22 +834,23 @@ static int spectral_run(const float* wsym, const float* eps, const int32_t* tail
   float* cur = U0;
   float* nxt = U1;
   for (int it = 0; it < iters; ++it) {
-    const double* prev = it == 0 ? nullptr : gram + 16 * ((it + 2) % 3);
+    const double* prev = it == 0 ? nullptr : gram + GRAM_STRIDE * ((it + 2) % 3);
     hipLaunchKernelGGL((k_spec_iter<NC>), dim3(rb), dim3(256), 0, s, cur, prev, wn, tail, n, k, nxt,
-                       gram + 16 * (it % 3), gram + 16 * ((it + 1) % 3));
+                       gram + GRAM_STRIDE * (it % 3), gram + GRAM_STRIDE * ((it + 1) % 3));
     MV_LAUNCHED("k_spec_iter");
     float* t = cur;
     cur = nxt;
     nxt = t;
   }
-  hipLaunchKernelGGL((k_spec_apply<NC>), dim3(rb), dim3(256), 0, s, cur, gram + 16 * ((iters - 1) % 3), n, V);
+  hipLaunchKernelGGL((k_spec_apply<NC>), dim3(rb), dim3(256), 0, s, cur, gram + GRAM_STRIDE * ((iters - 1) % 3), n,
+                     V);
   MV_LAUNCHED("k_spec_apply");
   return MV_OK;
 }
 
 static size_t spectral_ws(int64_t n, int k, int nc) {
   return align_up(n * 4, 256) + align_up(static_cast<size_t>(n) * k * 4, 256) +
-         2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + align_up(3 * 16 * 8, 256) + 1024;
+         2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + align_up(3 * GRAM_STRIDE * 8, 256) + 1024;
 }
 
 struct EpochBufs {
@@ -945,15 +950,15 @@ __device__ __forceinline__ float ordered_value(uint32_t k) {
 }
 struct EmbedScal {
   uint32_t absmax;
-  uint32_t lo[4];   // min keys
-  uint32_t hi[4];   // max keys
+  uint32_t lo[DIM_MAX];   // min keys
+  uint32_t hi[DIM_MAX];   // max keys
   int32_t nonfinite;
 };
 
 __global__ void k_embed_init(EmbedScal* sc) {
   if (threadIdx.x == 0) {
     sc->absmax = 0u;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < DIM_MAX; ++q) {
       sc->lo[q] = 0xFFFFFFFFu;
       sc->hi[q] = 0u;
     }
@@ -1000,7 +1005,12 @@ __global__ __launch_bounds__(256) void k_embed_scale(const float* __restrict__ V
   const int dim = nc - 1;
   const float amax = __uint_as_float(sc->absmax);
   const float expansion = 10.0f / amax;
-  uint32_t lo[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[4] = {0u, 0u, 0u, 0u};
+  uint32_t lo[DIM_MAX], hi[DIM_MAX];
+#pragma unroll
+  for (int c = 0; c < DIM_MAX; ++c) {
+    lo[c] = 0xFFFFFFFFu;
+    hi[c] = 0u;
+  }
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n * dim;
        t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int q = static_cast<int>(t % dim);
@@ -1008,14 +1018,14 @@ __global__ __launch_bounds__(256) void k_embed_scale(const float* __restrict__ V
     Y[t] = e;
     const uint32_t key = ordered_key(e);
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < DIM_MAX; ++c)
       if (c == q) {
         lo[c] = min(lo[c], key);
         hi[c] = max(hi[c], key);
       }
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < DIM_MAX; ++c) {
     if (c >= dim) break;
     const uint32_t l = ~block_reduce_max_u32(~lo[c], red);
     const uint32_t h = block_reduce_max_u32(hi[c], red);
@@ -1123,7 +1133,7 @@ int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, in
 
 int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes) {
   MV_REQUIRE(out_bytes, "mv_umap_epochs_ws_bytes: out_bytes is NULL");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= 64 && dim >= 1 && dim <= 4, "mv_umap_epochs_ws_bytes: bad sizes");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= 64 && dim >= 1 && dim <= DIM_MAX, "mv_umap_epochs_ws_bytes: bad sizes");
   *out_bytes = epochs_ws(n, k, dim);
   return MV_OK;
 }
@@ -1136,7 +1146,7 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
   MV_REQUIRE(n_in == 0 || (in_slots && in_node), "mv_umap_epochs: null in-edge list");
   MV_REQUIRE(n >= 1 && k >= 1 && k <= 64, "mv_umap_epochs: bad n=%lld k=%d", (long long)n, k);
   MV_REQUIRE(n_in >= 0 && n_in <= n * k, "mv_umap_epochs: bad in-edge count %lld", (long long)n_in);
-  MV_REQUIRE(dim >= 1 && dim <= 4, "mv_umap_epochs: dim=%d outside [1, 4]", dim);
+  MV_REQUIRE(dim >= 1 && dim <= DIM_MAX, "mv_umap_epochs: dim=%d outside [1, %d]", dim, DIM_MAX);
   MV_REQUIRE(n * k < (1ll << 31), "mv_umap_epochs: n * k = %lld slots, at most 2^31 - 1", (long long)(n * k));
   MV_REQUIRE(0 <= e0 && e0 <= e1 && e1 <= n_epochs, "mv_umap_epochs: bad epoch range [%d, %d) of %d", e0, e1,
              n_epochs);
@@ -1182,7 +1192,11 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
       case 1: rc = run_epoch<1>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
       case 2: rc = run_epoch<2>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
       case 3: rc = run_epoch<3>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      default: rc = run_epoch<4>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 4: rc = run_epoch<4>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 5: rc = run_epoch<5>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 6: rc = run_epoch<6>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 7: rc = run_epoch<7>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      default: rc = run_epoch<8>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
     }
     if (rc != MV_OK) return rc;
     float* t = cur;
@@ -1207,7 +1221,7 @@ int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, con
 
 int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes) {
   MV_REQUIRE(out_bytes, "mv_umap_spectral_ws_bytes: out_bytes is NULL");
-  MV_REQUIRE(n >= 1 && k >= 1 && nc >= 2 && nc <= 5, "mv_umap_spectral_ws_bytes: bad n=%lld k=%d nc=%d",
+  MV_REQUIRE(n >= 1 && k >= 1 && nc >= 2 && nc <= NC_MAX, "mv_umap_spectral_ws_bytes: bad n=%lld k=%d nc=%d",
              (long long)n, k, nc);
   *out_bytes = spectral_ws(n, k, nc);
   return MV_OK;
@@ -1216,7 +1230,7 @@ int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes) {
 int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, int64_t n, int k, int nc,
                      int iters, float* V, void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(w_sym && eps && tail && V, "mv_umap_spectral: null pointer");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX && nc >= 2 && nc <= 5 && iters >= 0,
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX && nc >= 2 && nc <= NC_MAX && iters >= 0,
              "mv_umap_spectral: bad n=%lld k=%d nc=%d iters=%d", (long long)n, k, nc, iters);
   if (!ws || ws_bytes < spectral_ws(n, k, nc)) {
     mv::set_error("mv_umap_spectral: workspace of %zu bytes < required %zu", ws_bytes, spectral_ws(n, k, nc));
@@ -1229,14 +1243,18 @@ int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, 
     case 2: return spectral_run<2>(w_sym, eps, tail, n, k, iters, V, w, s);
     case 3: return spectral_run<3>(w_sym, eps, tail, n, k, iters, V, w, s);
     case 4: return spectral_run<4>(w_sym, eps, tail, n, k, iters, V, w, s);
-    default: return spectral_run<5>(w_sym, eps, tail, n, k, iters, V, w, s);
+    case 5: return spectral_run<5>(w_sym, eps, tail, n, k, iters, V, w, s);
+    case 6: return spectral_run<6>(w_sym, eps, tail, n, k, iters, V, w, s);
+    case 7: return spectral_run<7>(w_sym, eps, tail, n, k, iters, V, w, s);
+    case 8: return spectral_run<8>(w_sym, eps, tail, n, k, iters, V, w, s);
+    default: return spectral_run<9>(w_sym, eps, tail, n, k, iters, V, w, s);
   }
 }
 
 int mv_umap_spectral_embed(const float* V, int64_t n, int nc, uint64_t seed, float* Y, int32_t* out_nonfinite,
                            void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(V && Y, "mv_umap_spectral_embed: null pointer");
-  MV_REQUIRE(n >= 1 && nc >= 2 && nc <= 5, "mv_umap_spectral_embed: bad n=%lld nc=%d", (long long)n, nc);
+  MV_REQUIRE(n >= 1 && nc >= 2 && nc <= NC_MAX, "mv_umap_spectral_embed: bad n=%lld nc=%d", (long long)n, nc);
   if (!ws || ws_bytes < sizeof(EmbedScal)) {
     mv::set_error("mv_umap_spectral_embed: workspace of %zu bytes < required %zu", ws_bytes, sizeof(EmbedScal));
     return MV_EWORKSPACE;

@@ -271,8 +271,8 @@ class UMAP:
                 X = self._pca(X)
             Xd = _device.as_device_f32(X)
         n = Xd.shape[0]
-        if not 1 <= self.n_components <= 4:
-            raise ValueError("n_components must be in [1, 4]")
+        if not 1 <= self.n_components <= 8:
+            raise ValueError("n_components must be in [1, 8]")
 
         if self.verbose:
             print("Computing nearest neighbors...")
