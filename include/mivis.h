@@ -358,6 +358,21 @@ int mv_random_normal(float* out, int64_t count, uint64_t seed, void* stream);
 int mv_umap_spectral_embed(const float* V, int64_t n, int nc, uint64_t seed, float* Y, int32_t* out_nonfinite,
                            void* ws, size_t ws_bytes, void* stream);
 
+/* ----------------------------------------------------------------- PCA --
+ * The array-sized part of UMAP's PCA preprocessing (mlx_vis/_umap/umap.py:
+ * 92-113): mean_out (d, FP64 device) = column means of X (n x d FP32),
+ * cov_out (d x d, FP64 device) = (X - mean)^T (X - mean) / (n - 1) (FP32
+ * products, FP64 sums over 1,024-row chunks) — replaces umap.py:95-98; the
+ * host then takes the eigenvectors (umap.py:100-103, mx.linalg.eigh on the
+ * CPU stream in the reference). d <= 8192. */
+int mv_pca_ws_bytes(int64_t n, int64_t d, size_t* out_bytes);
+int mv_pca_cov(const float* X, int64_t n, int64_t d, double* mean_out, double* cov_out, void* ws, size_t ws_bytes,
+               void* stream);
+/* out (n x p FP32) = (X - mean) V with mean (d, FP32) and V (d x p FP32
+ * row-major), the projection of umap.py:104-105. 1 <= p <= d. */
+int mv_pca_project(const float* X, int64_t n, int64_t d, const float* mean, const float* V, int64_t p, float* out,
+                   void* stream);
+
 /* p[0:count] = 0 on the stream (a fill kernel, safe under graph capture). */
 int mv_zero_f32(float* p, int64_t count, void* stream);
 

@@ -90,6 +90,9 @@ _SIGNATURES: dict[str, list] = {
     "mv_cne_optimize": [_p, _i64, _i32, _p, _p, _i64, _i32, _i32, _i32, _i32, _f32, _i64, _u64, _p, _p, _p, _p],
     "mv_pair_grad": [_p, _i64, _i32, _p, _p, _i64, _p, _f32, _f32, _i32, _p, _p],
     "mv_adam_update": [_p, _p, _p, _p, _i64, _f32, _f32, _f32, _f32, _p],
+    "mv_pca_ws_bytes": [_i64, _i64, C.POINTER(_sz)],
+    "mv_pca_cov": [_p, _i64, _i64, _p, _p, _p, _sz, _p],
+    "mv_pca_project": [_p, _i64, _i64, _p, _p, _i64, _p, _p],
 }
 _RESTYPES = {"mv_last_error": C.c_char_p}
 

@@ -203,15 +203,11 @@ class UMAP:
         self.knn_ = None
 
     # -- pieces ------------------------------------------------------------
-    def _pca(self, X: np.ndarray) -> np.ndarray:
-        # umap.py:92-113: covariance, eigh, top pca_dim components (host).
-        Xc = X - X.mean(axis=0)
-        cov = (Xc.T @ Xc) / (X.shape[0] - 1)
-        vals, vecs = np.linalg.eigh(cov.astype(np.float64))
-        proj = vecs[:, -self.pca_dim:][:, ::-1].astype(np.float32)
-        if self.verbose:
-            print(f"Variance retained: {vals[-self.pca_dim:].sum() / vals.sum() * 100:.1f}%")
-        return np.ascontiguousarray(Xc @ proj, dtype=np.float32)
+    def _pca(self, Xd: torch.Tensor) -> torch.Tensor:
+        # umap.py:92-113: covariance and projection on the device, eigh on the host
+        from ._pca import pca_device
+
+        return pca_device(Xd, int(self.pca_dim), self.verbose)
 
     def _knn(self, Xd: torch.Tensor):
         n = Xd.shape[0]
@@ -258,18 +254,18 @@ class UMAP:
             X: input data (n_samples, n_features): numpy, torch (CPU or GPU).
             epoch_callback: optional callable(epoch, Y_numpy) for snapshots.
         """
-        if isinstance(X, torch.Tensor) and X.is_cuda and not self.normalize and self.pca_dim is None:
-            Xd = X.detach().float().contiguous()
+        if isinstance(X, torch.Tensor) and X.is_cuda and not self.normalize:
+            Xd = _device.f32_rows(X.detach())
         else:
             if isinstance(X, torch.Tensor):
                 X = X.detach().cpu().numpy()
             X = np.asarray(X, dtype=np.float32)
             X = normalize_input(X, self.normalize)
-            if self.pca_dim is not None and X.shape[1] > self.pca_dim:
-                if self.verbose:
-                    print(f"Applying PCA: {X.shape[1]} -> {self.pca_dim} dims...")
-                X = self._pca(X)
             Xd = _device.as_device_f32(X)
+        if self.pca_dim is not None and Xd.shape[1] > self.pca_dim:
+            if self.verbose:
+                print(f"Applying PCA: {Xd.shape[1]} -> {self.pca_dim} dims...")
+            Xd = self._pca(Xd)
         n = Xd.shape[0]
         if not 1 <= self.n_components <= 8:
             raise ValueError("n_components must be in [1, 8]")
