@@ -59,10 +59,45 @@ __device__ __forceinline__ NormCdf norm_cdf(float x) {
   const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
                               0.254829592f);
   const float e = __expf(-z * z);
-  const float half_erfc = 0.5f * poly * e;
-  return NormCdf{x >= 0.f ? 1.f - half_erfc : half_erfc, e};
+  // 1 - half_erfc is written as one fused operation, so that every inlined
+  // copy rounds alike (the lookup table below is built from this function).
+  const float hp = 0.5f * poly;
+  return NormCdf{x >= 0.f ? fmaf(-hp, e, 1.f) : hp * e, e};
 }
 __device__ __forceinline__ float gelu(float x) { return x * norm_cdf(x).cdf; }
+
+// GELU of a BF16 input rounded to BF16, by table for |x| in [2^-12, 16): the
+// 2 x 16 x 128 BF16 values there (sign, exponent field 115..130, mantissa)
+// index a 4,096-entry table that holds f2bf(gelu(x)) computed by the function
+// above, so the lookup returns the same bits as the formula. Other inputs
+// (zero, tiny, |x| >= 16, Inf/NaN) take the formula.
+constexpr int GELU_LUT = 4096;
+constexpr uint32_t GELU_LUT_EXP0 = 115;
+__device__ __forceinline__ uint32_t gelu_lut_bits(int i) {
+  return (static_cast<uint32_t>(i >> 11) << 15) | (static_cast<uint32_t>(i & 2047) + (GELU_LUT_EXP0 << 7));
+}
+__device__ __forceinline__ uint32_t gelu_lut_off(uint32_t pb) { return (pb & 0x7FFFu) - (GELU_LUT_EXP0 << 7); }
+__device__ __forceinline__ bf16_t gelu_bf16(const bf16_t* lut, bf16_t pb) {
+  const uint32_t off = gelu_lut_off(pb);
+  if (off < 2048u) return lut[((static_cast<uint32_t>(pb) >> 15) << 11) + off];
+  return f2bf(gelu(bf2f(pb)));
+}
+// N inputs at once: when every lane's inputs lie in the table's range (a
+// wave-uniform test) the N table reads are issued unconditionally, so that
+// they are in flight together instead of one read per branch.
+template <int N>
+__device__ __forceinline__ void gelu_bf16_n(const bf16_t* lut, const uint32_t (&pb)[N], uint32_t (&out)[N]) {
+  bool miss = false;
+#pragma unroll
+  for (int k = 0; k < N; ++k) miss |= gelu_lut_off(pb[k]) >= 2048u;
+  if (__builtin_amdgcn_ballot_w64(miss) == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = lut[((pb[k] >> 15) << 11) + gelu_lut_off(pb[k])];
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = gelu_bf16(lut, static_cast<bf16_t>(pb[k]));
+  }
+}
 __device__ __forceinline__ float gelu_grad(float x) {
   const NormCdf n = norm_cdf(x);
   return n.cdf + x * 0.39894228040143268f * n.e;
@@ -120,10 +155,16 @@ __device__ __forceinline__ float4 pack_bf16x8(float4 a, float4 b) {
 //   EPI_BIAS_GELU        pre = bf16(acc + bias), C = bf16(gelu(pre));
 //   EPI_GELU_GRAD        C = bf16(acc * gelu'(pre_in)), colsum += C;
 //   EPI_HEAD             h = bf16(gelu(bf16(acc + bias))), Z[row, o] += sum_col h W3[o, col].
-template <int EPI, bool AF32>
+template <int EPI, bool AF32, int DO = 4>
 __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) float s_stage[2 * 2 * BM * LDSA];
+  constexpr bool LUT = EPI == EPI_BIAS_GELU || EPI == EPI_HEAD;
+  __shared__ bf16_t s_gelu[LUT ? GELU_LUT : 1];
   const int tid = threadIdx.x;
+  if constexpr (LUT) {
+    for (int i = tid; i < GELU_LUT; i += NT) s_gelu[i] = f2bf(gelu(bf2f(static_cast<bf16_t>(gelu_lut_bits(i)))));
+    __syncthreads();
+  }
   const int w = tid >> 6, l = tid & 63;
   const int wr = w >> 1, wc = w & 1;
   const int lr = tid >> 3, lq = tid & 7;   // staging row, 16-byte slot in the 128-byte BF16 chunk row
@@ -193,7 +234,12 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
 #define MV_LSTORE1(SET, I, KQ, AS, BS)                                                              \
   do {                                                                                            \
     float4 av_ = ra##SET##_##I;                                                                      \
-    if (AF32) av_ = (KQ) * 64 + lq * 8 >= g.kvalid ? make_float4(0.f, 0.f, 0.f, 0.f) : pack_bf16x8(ra##SET##_##I, rx##SET##_##I); \
+    if (AF32) {                                                                                   \
+      /* masked, not branched: a branch here makes the compiler wait for every load in flight */  \
+      const uint32_t m_ = (KQ) * 64 + lq * 8 < g.kvalid ? 0xFFFFFFFFu : 0u;                       \
+      const uint4 p_ = __builtin_bit_cast(uint4, pack_bf16x8(ra##SET##_##I, rx##SET##_##I));         \
+      av_ = __builtin_bit_cast(float4, make_uint4(p_.x & m_, p_.y & m_, p_.z & m_, p_.w & m_));   \
+    }                                                                                             \
     *reinterpret_cast<float4*>((AS) + (lr + 32 * (I)) * LDSA + lq * 4) = av_;                     \
     *reinterpret_cast<float4*>((BS) + (lr + 32 * (I)) * LDSA + lq * 4) = rv##SET##_##I;              \
   } while (0)
@@ -207,8 +253,12 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
     MV_LSTORE1(SET, 2, kq_, As2_, Bs2_);                                                          \
     MV_LSTORE1(SET, 3, kq_, As2_, Bs2_);                                                          \
   } while (0)
+  // The scheduling barriers keep the loads of chunk 1 ahead of those of
+  // chunk NPF, the order the loop's wait counts assume.
   MV_GLOAD(0, 0);
+  __builtin_amdgcn_sched_barrier(0);
   if constexpr (NPF > 1) MV_GLOAD(1, 1);
+  __builtin_amdgcn_sched_barrier(0);
   MV_LSTORE(0, 0, 0);
   MV_GLOAD(0, NPF);
   __syncthreads();
@@ -230,12 +280,13 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
           _Pragma("unroll") for (int ni = 0; ni < 4; ++ni)                                        \
             acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0); \
       }                                                                                           \
-      if (t_ + 1 < nch) {                                                                         \
-        MV_LSTORE(NS, t_ + 1, (t_ + 1) & 1);                                                      \
-        MV_GLOAD(NS, t_ + 1 + NPF);                                                               \
-      }                                                                                           \
-      __syncthreads();                                                                            \
     }                                                                                             \
+    /* The staging of the next chunk is unconditional (past the end: the     */                   \
+    /* clamped last chunk into the buffer no step reads): loads and stores   */                   \
+    /* under a branch make the compiler wait for every load in flight.       */                   \
+    MV_LSTORE(NS, t_ + 1, (t_ + 1) & 1);                                                          \
+    MV_GLOAD(NS, t_ + 1 + NPF);                                                                   \
+    __syncthreads();                                                                              \
   } while (0)
   for (int base = 0; base < nch; base += NPF) {
     if constexpr (NPF == 2) {
@@ -262,38 +313,42 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
     bv[ni] = (EPI == EPI_BIAS_GELU || EPI == EPI_HEAD) ? g.bias[col] : 0.f;
   }
   if constexpr (EPI == EPI_HEAD) {
-    float w3[4][4];   // [o][ni]
+    // DO outputs (= g.dout); columns past N carry zero weights, so that
+    // every lane runs the same instructions
+    float w3[DO][4];   // [o][ni]
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) {
-      const int col = min(bn + wc * 64 + ni * 16 + (l & 15), g.N - 1);
+      const int col = bn + wc * 64 + ni * 16 + (l & 15);
 #pragma unroll
-      for (int o = 0; o < 4; ++o) w3[o][ni] = o < g.dout ? g.W3[o * g.N + col] : 0.f;
+      for (int o = 0; o < DO; ++o) w3[o][ni] = col < g.N ? g.W3[o * g.N + col] : 0.f;
     }
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < 4; ++mi) {
+      uint32_t pb[16], hb[16];   // [r][ni]
+#pragma unroll
+      for (int k = 0; k < 16; ++k) pb[k] = f2bf(acc[mi][k & 3][k >> 2] + bv[k & 3]);
+      gelu_bf16_n<16>(s_gelu, pb, hb);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = bm + wr * 64 + mi * 16 + (l >> 4) * 4 + r;
-        float part[4] = {0.f, 0.f, 0.f, 0.f};
+        float part[DO];
+#pragma unroll
+        for (int o = 0; o < DO; ++o) part[o] = 0.f;
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
-          const int col = bn + wc * 64 + ni * 16 + (l & 15);
-          if (col < g.N) {
-            const float h = bf2f(f2bf(gelu(bf2f(f2bf(acc[mi][ni][r] + bv[ni])))));
+          const float h = bf2f(static_cast<bf16_t>(hb[r * 4 + ni]));
 #pragma unroll
-            for (int o = 0; o < 4; ++o) part[o] = fmaf(h, w3[o][ni], part[o]);
-          }
+          for (int o = 0; o < DO; ++o) part[o] = fmaf(h, w3[o][ni], part[o]);
         }
         // the 16 lanes of a row group (one DPP row) hold its 64 columns
-        float* z = static_cast<float*>(g.C) + static_cast<int64_t>(row) * g.dout;
+        float* z = static_cast<float*>(g.C) + static_cast<int64_t>(row) * DO;
 #pragma unroll
-        for (int o = 0; o < 4; ++o) {
-          if (o < g.dout) {
-            const float v = row16_sum(part[o]);
-            if ((l & 15) == 0 && row < g.M) atomicAdd(&z[o], v);
-          }
+        for (int o = 0; o < DO; ++o) {
+          const float v = row16_sum(part[o]);
+          if ((l & 15) == 0 && row < g.M) atomicAdd(&z[o], v);
         }
       }
+    }
     continue;
   }
 
@@ -310,6 +365,12 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
 #define MV_EPI_LOOP(VEC)                                                                            \
   _Pragma("unroll") for (int mi = 0; mi < 4; ++mi) {                                               \
     bf16_t pre_v[4][4];   /* EPI_GELU_GRAD: [r][ni], loaded before the row group's outputs */     \
+    uint32_t gv_[16];     /* EPI_BIAS_GELU: [r][ni], the row group's activations */                \
+    if constexpr (EPI == EPI_BIAS_GELU) {                                                         \
+      uint32_t pbv_[16];                                                                          \
+      _Pragma("unroll") for (int k = 0; k < 16; ++k) pbv_[k] = f2bf(acc[mi][k & 3][k >> 2] + bv[k & 3]); \
+      gelu_bf16_n<16>(s_gelu, pbv_, gv_);                                                         \
+    }                                                                                             \
     if constexpr (EPI == EPI_GELU_GRAD) {                                                         \
       _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                             \
         const int row = min(bm + wr * 64 + mi * 16 + (l >> 4) * 4 + r, g.M - 1);                  \
@@ -337,7 +398,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(GemmArgs g) {
             o = f2bf(v);                                                                          \
           } else if constexpr (EPI == EPI_BIAS_GELU) {                                            \
             pb = f2bf(v + bv[ni]);                                                                \
-            o = f2bf(gelu(bf2f(pb)));                                                             \
+            o = static_cast<bf16_t>(gv_[r * 4 + ni]);                                             \
           } else {                                                                                \
             o = f2bf(v * gelu_grad(bf2f(pre_v[r][ni])));                                          \
             if (ok) csum[ni] += bf2f(o);                                                          \
@@ -985,7 +1046,7 @@ static int cu_count() {
 // Launches k_gemm_nt over ceil(N/128) x ceil(M/128) tiles and `splits` K
 // ranges (splits > 1 with EPI_F32_ADD, or EPI_F32 into split_stride-spaced
 // partial slices) on a persistent grid.
-template <int EPI, bool AF32 = false>
+template <int EPI, bool AF32 = false, int DO = 4>
 static int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
   const int nk = g.K / 64;
   splits = split_count(g.K, splits);
@@ -996,7 +1057,7 @@ static int gemm(GemmArgs g, hipStream_t s, int splits = 1) {
   const int64_t slots = 8 * ceil_div(g.m_tiles, 8) * static_cast<int64_t>(g.n_tiles) * splits;
   const int64_t resident = 2 * static_cast<int64_t>(cu_count());   // two workgroups per CU (LDS)
   const int64_t grid = std::max<int64_t>(8, (std::min(slots, resident) + 7) / 8 * 8);
-  hipLaunchKernelGGL((k_gemm_nt<EPI, AF32>), dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, g);
+  hipLaunchKernelGGL((k_gemm_nt<EPI, AF32, DO>), dim3(static_cast<unsigned>(grid)), dim3(NT), 0, s, g);
   MV_LAUNCHED("k_gemm_nt");
   return MV_OK;
 }
@@ -1106,7 +1167,12 @@ static int forward_infer(const Params& W, const Dims& D, const float* X, int row
   g.bias = W.b2;
   g.W3 = W.W3;
   g.dout = D.dout;
-  return gemm<EPI_HEAD>(g, s);
+  switch (D.dout) {
+    case 1: return gemm<EPI_HEAD, false, 1>(g, s);
+    case 2: return gemm<EPI_HEAD, false, 2>(g, s);
+    case 3: return gemm<EPI_HEAD, false, 3>(g, s);
+    default: return gemm<EPI_HEAD, false, 4>(g, s);
+  }
 }
 
 // Backward from dZ into the gradient buffer (zeroed here).

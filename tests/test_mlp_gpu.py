@@ -70,6 +70,40 @@ def test_forward_matches_oracle(dims, m):
     assert _rel(z, z64) < 3e-2, _rel(z, z64)
 
 
+def _forward(blob, dims, Xd, m, chunk):
+    capi = _lib()
+    nb = ctypes.c_size_t(0)
+    capi.check(capi.lib().mv_mlp_batch_bytes(*dims, chunk, ctypes.byref(nb)))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    Z = torch.empty((m, dims[3]), dtype=torch.float32, device="cuda")
+    capi.check(capi.lib().mv_mlp_forward(blob.data_ptr(), *dims, Xd.data_ptr(), m, Z.data_ptr(), chunk,
+                                         ws.data_ptr(), ws.numel(), None))
+    return Z.cpu().numpy()
+
+
+@pytest.mark.parametrize("dims,m", [((784, 512, 512, 2), 70001), ((100, 192, 320, 1), 40000)])
+def test_large_chunk_forward_matches_small_chunks_and_oracle(dims, m):
+    """One chunk of every row (a persistent grid over many tiles per
+    workgroup; FP32 input staged as BF16 for din % 8 == 0, a BF16 copy
+    otherwise; one output or two) against chunks of 16,384 rows on every row,
+    and against the oracle on 2,000 sampled rows (rows past a multiple of the
+    tile, hidden widths that are not multiples of it)."""
+    print(f"large-chunk forward dims={dims} m={m}", flush=True)
+    p = _params(dims, 3)
+    blob, _ = _blob(dims, p)
+    X = np.random.default_rng(4).uniform(0, 1, (m, dims[0])).astype(np.float32)
+    Xd = torch.from_numpy(X).cuda()
+    z_big = _forward(blob, dims, Xd, m, m)
+    z_small = _forward(blob, dims, Xd, m, 16384)
+    assert np.isfinite(z_big).all()
+    # only the order of the output layer's atomic sums differs
+    assert _rel(z_big, z_small) < 1e-5, _rel(z_big, z_small)
+    rows = np.sort(np.random.default_rng(5).choice(m, 2000, replace=False))
+    rows[-1] = m - 1
+    zo, _ = om.forward(p, X[rows], bf16=True)
+    assert _rel(z_big[rows], zo) < 2e-3, _rel(z_big[rows], zo)
+
+
 @pytest.mark.parametrize("dims,B,neg", [((100, 128, 192, 2), 128, 5), ((784, 512, 512, 2), 4096, 5),
                                         ((33, 64, 128, 2), 64, 0)])
 def test_batch_gradient_matches_oracle(dims, B, neg):
