@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--rerank", type=int, default=None,
                     help="candidates per row = rerank * k for --only knn (default: the library's per-dtype default)")
     ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
+    ap.add_argument("--neg-rate", type=int, default=5,
+                    help="negative samples per edge for --only layout (default 5, the workload's)")
     ap.add_argument("--no-symmetric", action="store_true",
                     help="self kNN with the fused screen (k_screen) instead of the symmetric distance kernel")
     return ap.parse_args()
@@ -456,10 +458,10 @@ def main():
             g.epn[g.eps <= 0] = float("inf")
             torch.cuda.synchronize()
             t = time.perf_counter()
-            run_epochs(Y, None, g, 1, args.epochs, args.epochs, 5, 1.0, a, b, 42)
+            run_epochs(Y, None, g, 1, args.epochs, args.epochs, args.neg_rate, 1.0, a, b, 42)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t)
-        print(json.dumps({"layout_epochs_s": times, "epochs": args.epochs - 1}), flush=True)
+        print(json.dumps({"layout_epochs_s": times, "epochs": args.epochs - 1, "neg_rate": args.neg_rate}), flush=True)
         return
 
     if args.only == "scale":
