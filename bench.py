@@ -645,6 +645,43 @@ def main():
 
         prec = {"fp32_dist_rel_err_vs_fp64": dist_err("fp32", 2),
                 "screen_sq_dist_rel_err_vs_fp64": {dt: screen_err(dt) for dt in ("bf16x3", "bf16", "fp8")}}
+
+        # What the quantised screens' certificate can prove, per candidate
+        # count C: for the sampled rows, the exact squared distances to every
+        # point (FP64 on the device, centred rows, self dropped); tau = the
+        # C-th smallest (the screen's C-th value up to its error); the bound
+        # for every point past the candidates, (sqrt(tau - arith) - E_i -
+        # E_max)^2, with E the norms of the rows' quantisation errors and
+        # arith the screen's FP32 accumulation bound; a row certifies when its
+        # 15th distance is below it. Reported: the median gap tau - d15, the
+        # median margin the certificate needs (tau - bound), the fraction of
+        # rows that can certify.
+        def cert_gap():
+            rows_t = torch.as_tensor(err_rows, device=Xd.device)
+            Xc64 = torch.from_numpy(Xall64 - mu64).to(Xd.device)
+            d2 = torch.cdist(Xc64.index_select(0, rows_t), Xc64).pow_(2)
+            d2.scatter_(1, rows_t[:, None], float("inf"))
+            top = torch.topk(d2, 256, dim=1, largest=False).values
+            del d2
+            sq = (Xc64 * Xc64).sum(dim=1)
+            sqq, sqmax = sq.index_select(0, rows_t), sq.max()
+            kth = top[:, 14]
+            out = {}
+            for dt, row_elems in (("bf16", 832), ("fp8", 896)):
+                E = (Xc32.double() - quantise(dt)).norm(dim=1).to(Xd.device)
+                Ei, Emax = E.index_select(0, rows_t), E.max()
+                arith = (2 * row_elems + 10) * 2.0 ** -24 * 1.01 * (sqq + sqmax)
+                res = {"E_median": round(float(E.median()), 5), "E_max": round(float(Emax), 5)}
+                for C in (30, 60, 128, 256):
+                    tau = top[:, C - 1]
+                    lb = (torch.sqrt((tau - arith).clamp_min(0.0)) - Ei - Emax).clamp_min(0.0) ** 2
+                    res[f"c{C}"] = {"gap_median": round(float((tau - kth).median()), 4),
+                                    "margin_needed_median": round(float((tau - lb).median()), 4),
+                                    "certifiable": round(float((kth < lb).double().mean()), 4)}
+                out[dt] = res
+            return out
+
+        prec["certificate_gap"] = cert_gap()
         for dt, rr in (("bf16x3", 2), ("bf16", 2), ("bf16", 4), ("bf16", 9), ("fp8", 2), ("fp8", 9)):
             C = min(rr * 15, 128)
             log(f"kNN {dt} screen, {C} candidates")

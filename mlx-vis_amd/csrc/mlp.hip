@@ -69,17 +69,24 @@ __device__ __forceinline__ float gelu(float x) { return x * norm_cdf(x).cdf; }
 // GELU of a BF16 input rounded to BF16, by table for |x| in [2^-12, 16): the
 // 2 x 16 x 128 BF16 values there (sign, exponent field 115..130, mantissa)
 // index a 4,096-entry table that holds f2bf(gelu(x)) computed by the function
-// above, so the lookup returns the same bits as the formula. Other inputs
-// (zero, tiny, |x| >= 16, Inf/NaN) take the formula.
+// above, so the lookup returns the same bits as the formula. For finite
+// |x| >= 16 the formula's exp(-x^2 / 2) underflows to 0, so it returns x
+// (x >= 16) or -0 (x <= -16) exactly, and so does the rule here. Other
+// inputs (zero, |x| < 2^-12, Inf/NaN) take the formula.
 constexpr int GELU_LUT = 4096;
 constexpr uint32_t GELU_LUT_EXP0 = 115;
 __device__ __forceinline__ uint32_t gelu_lut_bits(int i) {
   return (static_cast<uint32_t>(i >> 11) << 15) | (static_cast<uint32_t>(i & 2047) + (GELU_LUT_EXP0 << 7));
 }
+// off = (exponent field - 115) * 128 + mantissa: < 2,048 in the table,
+// [2,048, 17,920) for 16 <= |x| < Inf, larger (or wrapped) otherwise
+constexpr uint32_t GELU_OFF_BIG_END = (255u - GELU_LUT_EXP0) << 7;
 __device__ __forceinline__ uint32_t gelu_lut_off(uint32_t pb) { return (pb & 0x7FFFu) - (GELU_LUT_EXP0 << 7); }
+__device__ __forceinline__ uint32_t gelu_big(uint32_t pb) { return (pb & 0x8000u) ? 0x8000u : pb; }
 __device__ __forceinline__ bf16_t gelu_bf16(const bf16_t* lut, bf16_t pb) {
   const uint32_t off = gelu_lut_off(pb);
   if (off < 2048u) return lut[((static_cast<uint32_t>(pb) >> 15) << 11) + off];
+  if (off < GELU_OFF_BIG_END) return static_cast<bf16_t>(gelu_big(pb));
   return f2bf(gelu(bf2f(pb)));
 }
 // N inputs at once: when every lane's inputs lie in the table's range (a
@@ -89,10 +96,14 @@ template <int N>
 __device__ __forceinline__ void gelu_bf16_n(const bf16_t* lut, const uint32_t (&pb)[N], uint32_t (&out)[N]) {
   bool miss = false;
 #pragma unroll
-  for (int k = 0; k < N; ++k) miss |= gelu_lut_off(pb[k]) >= 2048u;
+  for (int k = 0; k < N; ++k) miss |= gelu_lut_off(pb[k]) >= GELU_OFF_BIG_END;
   if (__builtin_amdgcn_ballot_w64(miss) == 0) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) out[k] = lut[((pb[k] >> 15) << 11) + gelu_lut_off(pb[k])];
+    for (int k = 0; k < N; ++k) {
+      const uint32_t off = gelu_lut_off(pb[k]);
+      const uint32_t t = lut[((pb[k] >> 15) << 11) + min(off, 2047u)];
+      out[k] = off < 2048u ? t : gelu_big(pb[k]);
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < N; ++k) out[k] = gelu_bf16(lut, static_cast<bf16_t>(pb[k]));

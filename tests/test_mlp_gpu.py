@@ -70,6 +70,25 @@ def test_forward_matches_oracle(dims, m):
     assert _rel(z, z64) < 3e-2, _rel(z, z64)
 
 
+def test_forward_with_large_preactivations_matches_oracle():
+    """Weights scaled so that pre-activations reach |x| >= 16 (GELU returns x
+    or -0 exactly there) and beyond the GELU table's range on both sides."""
+    capi = _lib()
+    dims, m = (100, 128, 192, 2), 3000
+    p = _params(dims, 2)
+    p["W1"] = (p["W1"] * 40).astype(np.float32)
+    p["W2"] = (p["W2"] * 6).astype(np.float32)
+    blob, _ = _blob(dims, p)
+    X = np.random.default_rng(9).uniform(0, 1, (m, dims[0])).astype(np.float32)
+    a1 = X @ p["W1"].T + p["b1"]
+    assert (np.abs(a1) >= 16).mean() > 0.05 and (a1 <= -16).any() and (a1 >= 16).any()
+    Xd = torch.from_numpy(X).cuda()
+    z = _forward(blob, dims, Xd, m, 1024)
+    zo, _ = om.forward(p, X, bf16=True)
+    assert np.isfinite(z).all()
+    assert _rel(z, zo) < 2e-3, _rel(z, zo)
+
+
 def _forward(blob, dims, Xd, m, chunk):
     capi = _lib()
     nb = ctypes.c_size_t(0)

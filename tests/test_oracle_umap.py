@@ -108,6 +108,30 @@ def test_sgd_step_repulsion_closed_form():
     np.testing.assert_allclose(out[1:], Y[1:])
 
 
+def test_sgd_step_index_add_matches_sgd_step():
+    """The CPU baseline's torch index_add_ step against the np.add.at step on
+    the same edges and negatives, in FP64 (repeated heads, tails and
+    negatives, some pairs closer than the 1e-6 floor, some clipped at 4)."""
+    import torch
+
+    rng = np.random.default_rng(7)
+    n, E = 60, 400
+    Y = rng.uniform(-3, 3, (n, 2))
+    Y[5] = Y[6]   # a zero-distance pair
+    ef = rng.integers(0, n, E)
+    et = rng.integers(0, n, E)
+    ef[0], et[0] = 5, 6
+    nf = np.repeat(ef, 5)
+    nt = rng.integers(0, n, 5 * E)
+    a, b, alpha = 1.577, 0.895, 0.7
+    want = ou.sgd_step(Y, ef, et, nf, nt, alpha, a, b, dtype=np.float64)
+    t = lambda v: torch.from_numpy(np.asarray(v, dtype=np.int64))
+    got = ou.sgd_step_index_add(torch.from_numpy(Y.astype(np.float32)), t(ef), t(et), t(nf), t(nt), alpha, a, b)
+    np.testing.assert_allclose(got.numpy(), want, rtol=1e-5, atol=2e-5)
+    # the step must move points (the comparison is not vacuous)
+    assert np.abs(want - Y).max() > 0.1
+
+
 def test_spectral_power_matches_eigsh_on_symmetric_graph():
     from scipy.sparse import coo_matrix
     from scipy.sparse.linalg import eigsh
