@@ -75,13 +75,20 @@ int mv_device_info(int* cu_count, int* lds_per_block, int* wave_size);
  * out_stats (device, nullable, 4 x int32): [0] rows recomputed by the FP64
  * scan, [1] rows whose order needed the FP64 tie check, [2] rows that failed
  * the first certificate, [3] reserved.
- * Limits: 1 <= k <= 64, k <= number of selectable rows. The candidate count
- * is min(rerank*k, 128) (at least k) with the symmetric self-kNN screen (see
+ * Limits: k <= number of selectable rows; 1 <= k <= 64 for any queries,
+ * 1 <= k <= 1024 for a self kNN (Q == X, nq == n) on the symmetric screen
+ * (otherwise k > 64 returns MV_EUNSUPPORTED). The candidate count is
+ * min(rerank*k, 128) (at least k) with the symmetric self-kNN screen (see
  * mv_knn_set_symmetric), min(rerank*k, 64) with the fused screen; a row whose
  * k-th neighbour is not separated from the last candidate by the screen's
  * error bound goes to the FP64 scan (BF16 / FP8: first to a split-BF16
- * re-screen with 64 candidates). d <= 3 uses an exact grid search instead of
- * a screen (dtype and rerank are then ignored).
+ * re-screen with 64 candidates). From k = 57 on, a self kNN on the symmetric
+ * screen instead selects from its n x n matrix: the k-th screen value by radix
+ * select, then the canonical distances of every column within twice the
+ * screen's error bound of it (no certificate, no scan; BF16 and FP8 requests
+ * run the split-BF16 screen there; out_stats stays zero). d <= 3 with k <= 64
+ * uses an exact grid search instead of a screen (dtype and rerank are then
+ * ignored).
  */
 int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rerank,
                     int same_qx, size_t* out_bytes);

@@ -1556,6 +1556,268 @@ __global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
   }
 }
 
+// ------------------------------------------------------- k above 64 (self) --
+
+constexpr int KMAX_LARGE = 1024;   // largest k: selected straight from the symmetric screen's matrix
+// From this k on, a self kNN on the symmetric screen selects from the matrix
+// (at 70,000 x 784: 38.7 ms at k = 56 through the candidate lists, 84 ms at
+// k = 64, where most rows miss the certificate; 44.5 ms at k = 65 here).
+constexpr int K_LARGE_FROM = 57;
+
+// Large-k self kNN, one workgroup per row, from the symmetric screen's n x n
+// matrix (FP32 or split-BF16 screen values s_j, |s_j - t_j| <= a with t_j the
+// true squared distance and a = err_fac (|q|^2 + max |x|^2), the screen's
+// arithmetic bound):
+//   1. B = the k-th smallest block minimum (k distinct columns are <= B, so the
+//      k-th smallest value tau <= B and only blocks with minimum <= B hold
+//      values that can rank below it);
+//   2. tau exactly, by a radix select (11, 11, 10 bits of the order-preserving
+//      key) over the values <= B of those blocks;
+//   3. every column with s_j <= T = tau + 2a is a candidate: the k columns of
+//      smallest screen value have t <= tau + a, so the k-th smallest true
+//      distance is <= tau + a, and any column at or below it has s <= tau + 2a;
+//   4. the candidates' canonical FP64 distances, merged batch by batch into a
+//      sorted list of (distance, index) in LDS; its first k are the result.
+// The result is the canonical order itself, whatever the screen's precision
+// within its bound; the cost is about k candidates' exact distances per row.
+struct LargeArgs {
+  const float* D;   // [n_pad, ldd]
+  int64_t ldd;
+  const float* bmin;   // [n_pad, nb] block minima over the selectable columns, or nullptr
+  int nb;
+  int n, nq, k;
+  const int32_t* q_self;
+  int64_t self_base;
+  const float* sqq;             // centred squared norms of the rows
+  const uint32_t* sqmax_bits;   // largest centred squared norm
+  float err_fac;
+  const float* Q;   // original rows: canonical distances
+  const float* X;
+  int64_t d;
+  int euclid;
+  int32_t* out_idx;
+  float* out_dist;
+};
+
+__device__ __forceinline__ uint32_t fkey(float v) {
+  const uint32_t b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_value(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Block-wide bitonic sort (ascending) of P = power of two <= 2048 entries in
+// LDS, 256 threads; (distance, index) pairs or 32-bit keys.
+__device__ void lds_sort_pairs(double* kd, int32_t* ki, int P) {
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < P / 2; t += 256) {
+        const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        const double a = kd[lo], b = kd[hi];
+        const int32_t ia = ki[lo], ib = ki[hi];
+        const bool gt = a > b || (a == b && ia > ib);
+        if (gt == ((lo & size) == 0)) {
+          kd[lo] = b;
+          kd[hi] = a;
+          ki[lo] = ib;
+          ki[hi] = ia;
+        }
+      }
+    }
+  __syncthreads();
+}
+__device__ void lds_sort_u32(uint32_t* key, int P) {
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < P / 2; t += 256) {
+        const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        const uint32_t a = key[lo], b = key[hi];
+        if ((a > b) == ((lo & size) == 0)) {
+          key[lo] = b;
+          key[hi] = a;
+        }
+      }
+    }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_knn_large(LargeArgs g) {
+  constexpr int LIST = 2048;   // running list (<= k) + one batch (< 512)
+  __shared__ double s_md[LIST];
+  __shared__ int32_t s_mi[LIST];
+  __shared__ uint32_t s_u[2048];   // block-minimum keys, then the radix histogram
+  __shared__ int32_t s_blk[1024];
+  __shared__ int32_t s_bc[512];
+  __shared__ int32_t s_cnt, s_nblk;
+  __shared__ uint32_t s_digit, s_below;
+  const int tid = threadIdx.x;
+  const int64_t row = blockIdx.x;
+  if (row >= g.nq) return;
+  const int k = g.k;
+  int64_t self = -1;
+  if (g.q_self) self = g.q_self[row];
+  else if (g.self_base >= 0) self = g.self_base + row;
+  const float* Dr = g.D + row * g.ldd;
+  const bool lists = g.bmin != nullptr && g.nb <= 1024;
+
+  // 1. B: the k-th smallest block minimum (+inf: read every block)
+  uint32_t bkey = 0xFFFFFFFFu;
+  if (lists && g.nb > k) {
+    for (int b = tid; b < 1024; b += 256) s_u[b] = b < g.nb ? fkey(g.bmin[row * g.nb + b]) : 0xFFFFFFFFu;
+    lds_sort_u32(s_u, 1024);
+    bkey = s_u[k - 1];
+    __syncthreads();
+  }
+  // the blocks whose minimum is at most the bound key
+  auto build_list = [&](uint32_t bound) {
+    if (tid == 0) s_nblk = 0;
+    __syncthreads();
+    for (int b = tid; b < g.nb; b += 256)
+      if (fkey(g.bmin[row * g.nb + b]) <= bound) s_blk[atomicAdd(&s_nblk, 1)] = b;
+    __syncthreads();
+  };
+  int nblk = g.nb;
+  if (lists) {
+    build_list(bkey);
+    nblk = s_nblk;
+  }
+  // visits the selectable values of the listed blocks, two blocks (256 columns)
+  // per step; f(j, key, step_valid) for every thread (valid: a column to offer)
+  auto visit = [&](int count, auto&& f) {
+    for (int i = 0; i < count; i += 2) {
+      const int li = i + (tid >> 7);
+      bool ok = li < count;
+      int64_t j = 0;
+      float v = INFINITY;
+      if (ok) {
+        const int b = lists ? s_blk[li] : li;
+        j = static_cast<int64_t>(b) * 128 + (tid & 127);
+        ok = j < g.n && j != self;
+        if (ok) v = Dr[j];
+      }
+      f(j, fkey(v), ok);
+    }
+  };
+
+  // 2. tau: the k-th smallest key among those <= B, three radix digits
+  uint32_t prefix = 0;
+  uint32_t krem = static_cast<uint32_t>(k);
+  const int shifts[3] = {21, 10, 0};
+  const int widths[3] = {11, 11, 10};
+  for (int p = 0; p < 3; ++p) {
+    const int sh = shifts[p], wd = widths[p];
+    const uint32_t hi_mask = sh + wd >= 32 ? 0u : ~((1u << (sh + wd)) - 1u);
+    const uint32_t dmask = (1u << wd) - 1u;
+    for (int b = tid; b < 2048; b += 256) s_u[b] = 0;
+    __syncthreads();
+    visit(nblk, [&](int64_t, uint32_t key, bool ok) {
+      if (ok && key <= bkey && (key & hi_mask) == (prefix & hi_mask)) atomicAdd(&s_u[(key >> sh) & dmask], 1u);
+    });
+    __syncthreads();
+    // the digit holding rank krem: sums of 8 bins per thread, 4 of those per
+    // lane of wavefront 0, a wavefront scan, then the lane that crosses krem
+    // walks its 32 bins
+    {
+      uint32_t part = 0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) part += s_u[tid * 8 + e];
+      s_bc[tid] = static_cast<int32_t>(part);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      uint32_t p4[4];
+      uint32_t ls = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        p4[e] = static_cast<uint32_t>(s_bc[tid * 4 + e]);
+        ls += p4[e];
+      }
+      uint32_t incl = ls;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (tid >= off) incl += o;
+      }
+      const uint64_t hit = __ballot(incl >= krem);
+      const int lane = hit ? __builtin_ctzll(hit) : 63;
+      if (tid == lane) {
+        uint32_t below = incl - ls;
+        uint32_t dg = static_cast<uint32_t>(tid) * 32;
+        const uint32_t end = dg + 32 <= dmask + 1 ? dg + 32 : dmask + 1;
+        for (; dg + 1 < end; ++dg) {
+          if (below + s_u[dg] >= krem) break;
+          below += s_u[dg];
+        }
+        s_digit = dg;
+        s_below = below;
+      }
+    }
+    __syncthreads();
+    prefix |= s_digit << sh;
+    krem -= s_below;
+    __syncthreads();
+  }
+  const float tau = fkey_value(prefix);
+
+  // 3. candidates: s <= T = tau + 2a (rounded up)
+  const double a = static_cast<double>(g.err_fac) *
+                   (static_cast<double>(g.sqq[row]) + static_cast<double>(__uint_as_float(*g.sqmax_bits))) + 1e-30;
+  const double Td = static_cast<double>(tau) + 2.0 * a;
+  float T = static_cast<float>(Td);
+  if (static_cast<double>(T) < Td) T = nextafterf(T, INFINITY);
+  const uint32_t tkey = fkey(T);
+  if (lists) {
+    build_list(tkey);
+    nblk = s_nblk;
+  }
+
+  // 4. canonical distances of the candidates, merged into the sorted list
+  const float* q = g.Q + row * g.d;
+  int L = 0;
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+  auto flush = [&]() {
+    const int m = s_cnt;
+    for (int e = tid; e < m; e += 256) {
+      const int32_t j = s_bc[e];
+      s_md[L + e] = dist64_seq(q, g.X + static_cast<int64_t>(j) * g.d, g.d);
+      s_mi[L + e] = j;
+    }
+    int P = 1;
+    while (P < L + m) P <<= 1;
+    for (int e = L + m + tid; e < P; e += 256) {
+      s_md[e] = INFINITY;
+      s_mi[e] = 0x7FFFFFFF;
+    }
+    lds_sort_pairs(s_md, s_mi, P);
+    L = L + m < k ? L + m : k;
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+  };
+  for (int i = 0; i < nblk; i += 2) {
+    const int li = i + (tid >> 7);
+    if (li < nblk) {
+      const int b = lists ? s_blk[li] : li;
+      const int64_t j = static_cast<int64_t>(b) * 128 + (tid & 127);
+      if (j < g.n && j != self && fkey(Dr[j]) <= tkey) s_bc[atomicAdd(&s_cnt, 1)] = static_cast<int32_t>(j);
+    }
+    __syncthreads();
+    const int c = s_cnt;
+    __syncthreads();   // every thread has read the count before the next step adds to it
+    if (c >= 256) flush();   // a step adds at most 256 (buffer of 512)
+  }
+  __syncthreads();
+  if (s_cnt > 0) flush();
+  for (int t = tid; t < k; t += 256) {
+    const double dv = s_md[t];
+    g.out_idx[row * k + t] = s_mi[t];
+    g.out_dist[row * k + t] = static_cast<float>(g.euclid ? sqrt(dv) : dv);
+  }
+}
+
 // ------------------------------------------------------------- exact scan --
 
 constexpr int SCAN_CHUNKS = 64;    // column chunks per failing row in the split scan
@@ -1969,7 +2231,7 @@ static int noalloc_scratch(void** out) {
 static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, bool same, int dtype,
                           bool sym_ok) {
   Layout L{};
-  if (d <= LOWD_MAX_D) {
+  if (d <= LOWD_MAX_D && k <= CMAX_K) {
     L.lowd = true;
     L.bytes = lowd_knn_ws_bytes(n);
     return L;
@@ -1982,7 +2244,8 @@ static Layout make_layout(int64_t nq, int64_t n, int64_t d, int k, int rerank, b
   L.nchunks = ceil_div(n, L.rows_per);
   L.dtype = dtype;
   L.same = same;
-  L.sym = same && sym_ok && n >= SYM_MIN_N &&
+  // k above 64 is selected from the symmetric screen's matrix only, at any n
+  L.sym = same && sym_ok && (n >= SYM_MIN_N || k > CMAX_K) &&
           static_cast<size_t>(L.n_pad) * L.n_pad * 4 <= SYM_MAX_BYTES;
   // candidates per row: up to 128 from the symmetric screen's selection, 64
   // from the fused screen's per-lane lists
@@ -2069,12 +2332,39 @@ int launch_screen(const ScreenArgs& sa, int64_t grid, hipStream_t s) {
 // The whole brute-force kNN of mv_knn_brute on a given workspace. `sym_ok`
 // permits the symmetric screen (self kNN), `timed` the phase events of
 // mv_knn_set_timing; `who` names the entry point in error messages.
+// Screen arithmetic bound, in units of (|q|^2 + max |x|^2). FP32: two
+// roundings per product term plus the norms and the final combination.
+// Split BF16: products are exact, one rounding per each of the 3 d_pad
+// accumulated terms, plus the dropped lo*lo term and the split residuals
+// (each <= 2^-16 relative: 3 * 256 units of 2^-24).
+static float screen_err_fac(int dtype, const Layout& L, int64_t row_bytes) {
+  const double units = dtype == MV_BF16X3 ? 3.0 * static_cast<double>(L.d_pad) + 3.0 * 256.0 + 40.0
+                                          : 2.0 * static_cast<double>(row_bytes / elem_bytes(dtype)) + 10.0;
+  return static_cast<float>(units * 5.9604644775390625e-8 * 1.01);
+}
+
+// The selection from the matrix needs screen values that carry only the
+// arithmetic bound: there, BF16 and FP8 requests run the split-BF16 screen
+// (the result does not depend on the screen).
+static int large_k_dtype(int k, int dtype, bool sym) {
+  return sym && k >= K_LARGE_FROM && (dtype == MV_BF16 || dtype == MV_FP8) ? MV_BF16X3 : dtype;
+}
+
 static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, int64_t n, int64_t d, int k,
                    int dtype, int rerank, const int32_t* q_self, int64_t self_base, int euclid, int32_t* out_idx,
                    float* out_dist, int32_t* out_stats, void* ws, size_t ws_bytes, void* stream, bool sym_ok,
                    bool timed) {
   const bool same = (Q == X) && nq == n;
   Layout L = make_layout(nq, n, d, k, rerank, same, dtype, sym_ok);
+  if (large_k_dtype(k, dtype, L.sym) != dtype) {
+    dtype = large_k_dtype(k, dtype, L.sym);
+    L = make_layout(nq, n, d, k, rerank, same, dtype, sym_ok);
+  }
+  if (k > CMAX_K && !L.sym) {
+    mv::set_error("%s: k=%d above %d needs the symmetric self-kNN screen (Q == X, an n x n matrix of at most %zu "
+                  "bytes, mv_knn_set_symmetric on)", who, k, CMAX_K, SYM_MAX_BYTES);
+    return MV_EUNSUPPORTED;
+  }
   if (ws_bytes < L.bytes || !ws) {
     mv::set_error("%s: workspace of %zu bytes < required %zu", who, ws_bytes, L.bytes);
     return MV_EWORKSPACE;
@@ -2212,6 +2502,36 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     hipLaunchKernelGGL(ks, dim3(sgrid), dim3(64 * SYM_W), 0, s, ya);
     MV_LAUNCHED("k_dist_sym");
     if (int rc = tm.mark(5, s)) return rc;
+    if (k >= K_LARGE_FROM) {
+      // exact selection straight from the matrix (FP32 or split-BF16 screen)
+      LargeArgs ga;
+      ga.D = B.Dfull;
+      ga.ldd = L.n_pad;
+      ga.bmin = (q_self == nullptr && self_b <= 0) ? B.bmin : nullptr;
+      ga.nb = static_cast<int>(col_tiles);
+      ga.n = static_cast<int>(n);
+      ga.nq = static_cast<int>(nq);
+      ga.k = k;
+      ga.q_self = q_self;
+      ga.self_base = self_b;
+      ga.sqq = sa.sqq;
+      ga.sqmax_bits = sqmax32;
+      ga.err_fac = screen_err_fac(dtype, L, sa.row_bytes);
+      ga.Q = Q;
+      ga.X = X;
+      ga.d = d;
+      ga.euclid = euclid;
+      ga.out_idx = out_idx;
+      ga.out_dist = out_dist;
+      hipLaunchKernelGGL(k_knn_large, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s, ga);
+      MV_LAUNCHED("k_knn_large");
+      if (int rc = tm.mark(2, s)) return rc;
+      if (int rc = tm.mark(3, s)) return rc;
+      if (int rc = tm.mark(4, s)) return rc;
+      tm.valid = tm.on;
+      tm.last_sym = true;
+      return MV_OK;
+    }
     SelectArgs xa;
     xa.D = B.Dfull;
     xa.ldd = L.n_pad;
@@ -2238,15 +2558,7 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
   }
   if (int rc2 = tm.mark(2, s)) return rc2;
 
-  // Screen arithmetic bound, in units of (|q|^2 + max |x|^2). FP32: two
-  // roundings per product term plus the norms and the final combination.
-  // Split BF16: products are exact, one rounding per each of the 3 d_pad
-  // accumulated terms, plus the dropped lo*lo term and the split residuals
-  // (each <= 2^-16 relative: 3 * 256 units of 2^-24).
-  const double units = dtype == MV_BF16X3
-                           ? 3.0 * static_cast<double>(L.d_pad) + 3.0 * 256.0 + 40.0
-                           : 2.0 * static_cast<double>(sa.row_bytes / elem_bytes(dtype)) + 10.0;
-  const float fac = static_cast<float>(units * 5.9604644775390625e-8 * 1.01);
+  const float fac = screen_err_fac(dtype, L, sa.row_bytes);
   RerankArgs ra;
   ra.Q = Q;
   ra.X = X;
@@ -2391,13 +2703,20 @@ int mv_knn_ws_bytes(int64_t nq, int64_t n, int64_t d, int k, int dtype, int rera
   MV_REQUIRE(out_bytes, "mv_knn_ws_bytes: out_bytes is NULL");
   MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "mv_knn_ws_bytes: bad shape nq=%lld n=%lld d=%lld",
              (long long)nq, (long long)n, (long long)d);
-  MV_REQUIRE(k >= 1 && k <= CMAX_K, "mv_knn_ws_bytes: k=%d outside [1, %d]", k, CMAX_K);
+  MV_REQUIRE(k >= 1 && k <= KMAX_LARGE, "mv_knn_ws_bytes: k=%d outside [1, %d]", k, KMAX_LARGE);
   MV_REQUIRE(rerank >= 1, "mv_knn_ws_bytes: rerank=%d < 1", rerank);
   if (dtype != MV_F32 && dtype != MV_BF16 && dtype != MV_FP8 && dtype != MV_BF16X3) {
     mv::set_error("mv_knn_ws_bytes: dtype %d not supported", dtype);
     return MV_EUNSUPPORTED;
   }
   Layout L = make_layout(nq, n, d, k, rerank, same_qx != 0, dtype, g_symmetric);
+  if (large_k_dtype(k, dtype, L.sym) != dtype)
+    L = make_layout(nq, n, d, k, rerank, same_qx != 0, large_k_dtype(k, dtype, L.sym), g_symmetric);
+  if (k > CMAX_K && !L.sym) {
+    mv::set_error("mv_knn_ws_bytes: k=%d above %d needs the symmetric self-kNN screen (same_qx, an n x n matrix of "
+                  "at most %zu bytes, mv_knn_set_symmetric on)", k, CMAX_K, SYM_MAX_BYTES);
+    return MV_EUNSUPPORTED;
+  }
   *out_bytes = L.bytes;
   return MV_OK;
 }
@@ -2422,7 +2741,7 @@ static int knn_check(const char* who, const float* Q, int64_t nq, const float* X
   MV_REQUIRE(nq >= 0 && n >= 1 && d >= 1, "%s: bad shape nq=%lld n=%lld d=%lld", who, (long long)nq,
              (long long)n, (long long)d);
   MV_REQUIRE(n < (1ll << 31) - 256 && nq < (1ll << 31) - 256, "%s: n or nq too large", who);
-  MV_REQUIRE(k >= 1 && k <= CMAX_K, "%s: k=%d outside [1, %d]", who, k, CMAX_K);
+  MV_REQUIRE(k >= 1 && k <= KMAX_LARGE, "%s: k=%d outside [1, %d]", who, k, KMAX_LARGE);
   MV_REQUIRE(rerank >= 1, "%s: rerank=%d < 1", who, rerank);
   const bool excl = q_self != nullptr || self_base >= 0;
   MV_REQUIRE(k <= n - (excl ? 1 : 0), "%s: k=%d exceeds the %lld selectable rows", who, k,
@@ -2449,6 +2768,10 @@ int mv_knn_brute_noalloc(const float* Q, int64_t nq, const float* X, int64_t n, 
                          float* out_dist, int32_t* out_stats, void* stream) {
   static const char* who = "mv_knn_brute_noalloc";
   if (int rc = knn_check(who, Q, nq, X, n, d, k, dtype, rerank, q_self, self_base, out_idx, out_dist)) return rc;
+  if (k > CMAX_K) {
+    mv::set_error("%s: k=%d above %d needs the symmetric screen's n x n matrix; use mv_knn_brute", who, k, CMAX_K);
+    return MV_EUNSUPPORTED;
+  }
   const Layout L = make_layout(nq, n, d, k, rerank, (Q == X) && nq == n, dtype, false);
   if (L.bytes > NOALLOC_SCRATCH_BYTES) {
     mv::set_error("%s: nq=%lld n=%lld d=%lld k=%d needs %zu bytes of scratch, more than the %zu-byte static "

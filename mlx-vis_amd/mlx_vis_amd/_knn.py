@@ -43,6 +43,8 @@ def knn_brute_device(
     with the symmetric self-kNN screen, 64 otherwise). Default: 2, and for
     the BF16 screen enough for 128, the candidate set its certificate needs on
     clustered data whose 15th and 30th neighbours are close.
+    k: up to 64 for any query set; up to 1,024 for self kNN (Q is None), which
+    then selects from the symmetric screen's n x n matrix (n up to ~113,000).
     Returns (idx int32 (nq, k), dist float32 (nq, k)) CUDA tensors.
     """
     if dtype not in _capi.DTYPES:
@@ -82,7 +84,7 @@ def knn_brute_device(
     # when that would take more than half of the free device memory, this call
     # uses the fused screen instead (same results, no matrix).
     sym_prev = None
-    if same and n >= 4096:
+    if same and n >= 4096 and k <= 64:   # k above 64 needs the matrix
         free, _ = torch.cuda.mem_get_info(X.device)
         if nbytes > free // 2:
             sym_prev = lib.mv_knn_set_symmetric(0)

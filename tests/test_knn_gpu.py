@@ -198,8 +198,12 @@ def test_single_query_and_invalid_arguments():
     oi, od = oknn.exact_knn(Q, X, 10)
     _check(idx, dist, oi, od)
     Xd = torch.from_numpy(X).cuda()
+    from mlx_vis_amd._capi import MivisError
+
+    with pytest.raises(MivisError):
+        knn_brute_device(Xd, 65, torch.from_numpy(Q).cuda())   # k > 64 for separate queries: no matrix
     with pytest.raises(ValueError):
-        knn_brute_device(Xd, 65, dtype="fp32")   # k > 64
+        knn_brute_device(Xd, 1025)                # k above 1,024
     with pytest.raises(ValueError):
         knn_brute_device(Xd, 5, dtype="int8")
     with pytest.raises(ValueError):
@@ -343,3 +347,62 @@ def test_low_dimensional_grid_search_matches_oracle(case):
     else:
         oi, od = oknn.exact_knn(Q, X, k)
         _check(idx, dist, oi, od)
+
+
+@pytest.mark.parametrize("case", ["n3000_k100", "n3000_k1024", "n20000_k100_sampled", "dup_k300", "bf16_k80",
+                                  "no_exclusion_k70", "q_self_k90", "d2_k100"])
+def test_large_k_self_knn_matches_oracle(case):
+    """k above 64: selected from the symmetric screen's matrix (radix select of
+    the k-th screen value, every column within twice the screen's error bound
+    of it, canonical FP64 distances) against the oracle. Cases: fewer blocks
+    than k (every block read) and more (blocks pruned by their minima), k =
+    1,024, 500 copies of one row (a candidate set far beyond k, exact ties
+    resolved by index), a BF16 request (run on the split-BF16 screen), no self
+    exclusion, arbitrary excluded rows, and 2-D points (the grid search stops at
+    k = 64)."""
+    from mlx_vis_amd import knn_brute_device
+
+    print(f"case large k {case}", flush=True)
+    import zlib
+
+    rng = np.random.default_rng(zlib.crc32(case.encode()))
+    kw, rows, q_self = {}, None, None
+    if case == "n3000_k100":
+        X, k = rng.standard_normal((3000, 50)).astype(np.float32), 100
+    elif case == "n3000_k1024":
+        X, k = rng.standard_normal((3000, 20)).astype(np.float32), 1024
+    elif case == "n20000_k100_sampled":
+        centers = rng.standard_normal((10, 64)) * 4
+        X = (centers[rng.integers(0, 10, 20000)] + rng.standard_normal((20000, 64))).astype(np.float32)
+        k, rows = 100, np.sort(rng.choice(20000, 400, replace=False))   # 157 blocks > k: pruned by their minima
+    elif case == "dup_k300":
+        X = rng.standard_normal((2500, 30)).astype(np.float32)
+        X[1000:1500] = X[7]
+        k = 300
+    elif case == "bf16_k80":
+        X, k = rng.standard_normal((4100, 96)).astype(np.float32), 80
+        kw["dtype"] = "bf16"
+    elif case == "no_exclusion_k70":
+        X, k = rng.standard_normal((5000, 33)).astype(np.float32), 70
+        X[4000:4050] = X[:50]
+        kw["exclude_self"] = False
+    elif case == "q_self_k90":
+        X, k = rng.standard_normal((4300, 17)).astype(np.float32), 90
+        q_self = ((np.arange(4300) * 11 + 5) % 4300).astype(np.int32)
+        kw["q_self"] = torch.from_numpy(q_self)
+    else:
+        X, k = rng.uniform(0, 1, (6000, 2)).astype(np.float32), 100
+    n = X.shape[0]
+    Xd = torch.from_numpy(X).cuda()
+    idx, dist = knn_brute_device(Xd, k, **kw)
+    torch.cuda.synchronize()
+    idx, dist = idx.cpu().numpy(), dist.cpu().numpy()
+    if rows is None:
+        rows = np.arange(n)
+    if kw.get("exclude_self") is False:
+        oi, od = oknn.exact_knn(X[rows], X, k)
+    elif q_self is not None:
+        oi, od = oknn.exact_knn(X[rows], X, k, q_self=q_self[rows])
+    else:
+        oi, od = oknn.exact_knn(X[rows], X, k, q_self=rows.astype(np.int32))
+    _check(idx[rows], dist[rows], oi, od)

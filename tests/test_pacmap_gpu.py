@@ -98,3 +98,15 @@ def test_pacmap_high_dimensional_input_uses_pca():
     assert est._pca_solution and emb.shape == (1200, 2) and np.isfinite(emb).all()
     with pytest.raises(ValueError):
         PaCMAP().fit_transform(X, init="bogus")
+
+
+def test_pacmap_more_neighbours_than_64_candidates():
+    """n_neighbors = 20: the reference asks compute_knn for k = n_neighbors + 50
+    = 70 (pacmap.py:529), above the 64 of the candidate lists; the self kNN
+    then selects from the symmetric screen's matrix."""
+    from mlx_vis_amd import PaCMAP
+
+    X = _data(1500, 20, 6)
+    emb = PaCMAP(n_neighbors=20, random_state=2, num_iters=(40, 40, 80), knn_method="brute").fit_transform(X)
+    assert emb.shape == (1500, 2) and np.isfinite(emb).all()
+    assert om.trustworthiness(X, emb, 10) > 0.8
