@@ -187,7 +187,8 @@ int mv_nnd_gather_dists(const float* X, int64_t n, int64_t d, const int32_t* idx
  * umap.py:265-270, 402-403) and epn (epochs_per_next, initialised to eps;
  * +inf for pruned slots). counters (device, 2 x uint32): [0] kept edges,
  * [1] bit pattern of max(w_sym). sigma_out / rho_out (device, n, nullable):
- * per-row bandwidth and rho (umap.py:208-236). k <= 64.
+ * per-row bandwidth and rho (umap.py:208-236). k <= 256 (above 64 the
+ * bisection runs one wavefront per row and sums in a different order).
  */
 int mv_umap_graph_ws_bytes(int64_t n, int k, size_t* out_bytes);
 int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k, int n_epochs,
@@ -212,7 +213,7 @@ int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, in
  * against nodes drawn uniformly from [0, n) by Philox-4x32-10 keyed by seed;
  * the step size is lr * (1 - e / n_epochs). tail = the kNN indices (n, k);
  * (in_slots, in_node, n_in) from mv_umap_in_edges. ws sized by
- * mv_umap_epochs_ws_bytes. dim <= 8, k <= 64. */
+ * mv_umap_epochs_ws_bytes. dim <= 8, k <= 256. */
 int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes);
 int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
                    const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1, int n_epochs,

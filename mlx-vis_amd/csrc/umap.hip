@@ -39,7 +39,8 @@
 namespace mv {
 namespace umap {
 
-constexpr int KMAX = 64;
+constexpr int KMAX = 64;          // neighbours per row of the per-thread fuzzy-row kernel
+constexpr int KMAX_GRAPH = 256;   // largest k of the graph, the spectral init and the epochs
 constexpr int DIM_MAX = 8;   // largest embedding dimension of the layout kernels
 
 __global__ void k_zero_u32(uint32_t* p, int count) {
@@ -97,6 +98,66 @@ __global__ __launch_bounds__(256) void k_fuzzy_rows(const float* __restrict__ di
     if (j < k) wr[j] = expf(-__fdiv_rn(sh[j], den));
   if (sigma_out) sigma_out[i] = sigma;
   if (rho_out) rho_out[i] = rho;
+}
+
+// The same for 64 < k <= 256: one wavefront per row, lane l holding columns
+// l, l + 64, ...; the bisection's sums are wavefront reductions (a different
+// summation order than the one-thread kernel's sequential sum).
+constexpr int FW_PER_LANE = KMAX_GRAPH / 64;
+__global__ __launch_bounds__(256) void k_fuzzy_rows_wave(const float* __restrict__ dist, int64_t n, int k,
+                                                         float target, float* __restrict__ w,
+                                                         float* __restrict__ sigma_out, float* __restrict__ rho_out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  if (i >= n) return;
+  const float* d = dist + i * k;
+  float dv[FW_PER_LANE];
+  float rho = INFINITY;
+#pragma unroll
+  for (int u = 0; u < FW_PER_LANE; ++u) {
+    const int j = l + 64 * u;
+    dv[u] = j < k ? d[j] : 0.f;
+    if (j < k && dv[u] > 0.f) rho = fminf(rho, dv[u]);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) rho = fminf(rho, __shfl_xor(rho, m));
+  rho = fmaxf(rho, 1e-8f);
+  float sh[FW_PER_LANE];
+#pragma unroll
+  for (int u = 0; u < FW_PER_LANE; ++u) sh[u] = fmaxf(dv[u] - rho, 0.f);
+  float lo = 1e-20f, hi = 1e3f, sigma = 1.f;
+  for (int it = 0; it < 64; ++it) {
+    float s = 0.f;
+#pragma unroll
+    for (int u = 0; u < FW_PER_LANE; ++u) {
+      const int j = l + 64 * u;
+      if (j >= 1 && j < k) s += expf(-__fdiv_rn(sh[u], sigma));
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+    const bool conv = fabsf(s - target) < 1e-5f;
+    if (conv) break;
+    if (s > target) {
+      hi = sigma;
+      sigma = (lo + sigma) / 2.f;
+    } else if (s < target) {
+      lo = sigma;
+      sigma = hi >= 1e3f ? sigma * 2.f : (sigma + hi) / 2.f;
+    } else {
+      break;
+    }
+  }
+  const float den = fmaxf(sigma, 1e-10f);
+  float* wr = w + i * k;
+#pragma unroll
+  for (int u = 0; u < FW_PER_LANE; ++u) {
+    const int j = l + 64 * u;
+    if (j < k) wr[j] = expf(-__fdiv_rn(sh[u], den));
+  }
+  if (l == 0) {
+    if (sigma_out) sigma_out[i] = sigma;
+    if (rho_out) rho_out[i] = rho;
+  }
 }
 
 // w_sym[e] = w + w' - w*w' where w' is the weight of the reverse edge if the
@@ -907,7 +968,9 @@ static int run_epoch(const float* Y, float* Ynext, const EpochBufs& B, const int
   if (k <= 8) rc = launch_epoch<D, 8, 1>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
   else if (k <= 16) rc = launch_epoch<D, 8, 2>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
   else if (k <= 32) rc = launch_epoch<D, 16, 2>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
-  else rc = launch_epoch<D, 32, 2>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else if (k <= 64) rc = launch_epoch<D, 32, 2>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else if (k <= 128) rc = launch_epoch<D, 32, 4>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
+  else rc = launch_epoch<D, 64, 4>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
   if (rc) return rc;
   if (n_in > 0) {
     hipLaunchKernelGGL((k_tail_apply<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, B.gtail,
@@ -1055,7 +1118,7 @@ extern "C" {
 
 int mv_umap_graph_ws_bytes(int64_t n, int k, size_t* out_bytes) {
   MV_REQUIRE(out_bytes, "mv_umap_graph_ws_bytes: out_bytes is NULL");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX, "mv_umap_graph_ws_bytes: bad n=%lld k=%d", (long long)n, k);
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX_GRAPH, "mv_umap_graph_ws_bytes: bad n=%lld k=%d", (long long)n, k);
   *out_bytes = align_up(static_cast<size_t>(n) * k * sizeof(float), 256) + 256;
   return MV_OK;
 }
@@ -1064,8 +1127,8 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
                         float* w_sym, float* eps, float* epn, uint32_t* counters, float* sigma_out,
                         float* rho_out, void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(knn_idx && knn_dist && w_sym && eps && epn && counters, "mv_umap_fuzzy_graph: null pointer");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX, "mv_umap_fuzzy_graph: bad n=%lld k=%d (k <= %d)", (long long)n, k,
-             KMAX);
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX_GRAPH, "mv_umap_fuzzy_graph: bad n=%lld k=%d (k <= %d)", (long long)n, k,
+             KMAX_GRAPH);
   MV_REQUIRE(n_epochs >= 1, "mv_umap_fuzzy_graph: n_epochs=%d < 1", n_epochs);
   size_t need = 0;
   mv_umap_graph_ws_bytes(n, k, &need);
@@ -1079,8 +1142,12 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
   const int64_t count = n * k;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counters, 2);
   MV_LAUNCHED("k_zero_u32");
-  hipLaunchKernelGGL(k_fuzzy_rows, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, knn_dist, n, k,
-                     static_cast<float>(log2(static_cast<double>(k))), w, sigma_out, rho_out);
+  if (k <= KMAX)
+    hipLaunchKernelGGL(k_fuzzy_rows, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, knn_dist, n, k,
+                       static_cast<float>(log2(static_cast<double>(k))), w, sigma_out, rho_out);
+  else
+    hipLaunchKernelGGL(k_fuzzy_rows_wave, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, knn_dist, n,
+                       k, static_cast<float>(log2(static_cast<double>(k))), w, sigma_out, rho_out);
   MV_LAUNCHED("k_fuzzy_rows");
   const int64_t fgrid = ceil_div(count, 256) < 4096 ? ceil_div(count, 256) : 4096;
   hipLaunchKernelGGL(k_fuzzy_sym, dim3(static_cast<unsigned>(fgrid)), dim3(256), 0, s, knn_idx, w, n, k, w_sym,
@@ -1095,7 +1162,7 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
 int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, int32_t* in_slots,
                      int32_t* in_node, void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(tail && eps && in_slots && in_node, "mv_umap_in_edges: null pointer");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX, "mv_umap_in_edges: bad n=%lld k=%d", (long long)n, k);
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX_GRAPH, "mv_umap_in_edges: bad n=%lld k=%d", (long long)n, k);
   const int64_t nb = ceil_div(n, 1024);
   const size_t need = 3 * align_up(static_cast<size_t>(n) * 4, 256) + align_up(static_cast<size_t>(nb) * 4, 256);
   if (!ws || ws_bytes < need) {
@@ -1133,7 +1200,7 @@ int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, in
 
 int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes) {
   MV_REQUIRE(out_bytes, "mv_umap_epochs_ws_bytes: out_bytes is NULL");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= 64 && dim >= 1 && dim <= DIM_MAX, "mv_umap_epochs_ws_bytes: bad sizes");
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX_GRAPH && dim >= 1 && dim <= DIM_MAX, "mv_umap_epochs_ws_bytes: bad sizes");
   *out_bytes = epochs_ws(n, k, dim);
   return MV_OK;
 }
@@ -1144,7 +1211,7 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
                           const uint8_t* movable, void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(Y && tail && eps && epn, "mv_umap_epochs: null pointer");
   MV_REQUIRE(n_in == 0 || (in_slots && in_node), "mv_umap_epochs: null in-edge list");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= 64, "mv_umap_epochs: bad n=%lld k=%d", (long long)n, k);
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX_GRAPH, "mv_umap_epochs: bad n=%lld k=%d", (long long)n, k);
   MV_REQUIRE(n_in >= 0 && n_in <= n * k, "mv_umap_epochs: bad in-edge count %lld", (long long)n_in);
   MV_REQUIRE(dim >= 1 && dim <= DIM_MAX, "mv_umap_epochs: dim=%d outside [1, %d]", dim, DIM_MAX);
   MV_REQUIRE(n * k < (1ll << 31), "mv_umap_epochs: n * k = %lld slots, at most 2^31 - 1", (long long)(n * k));
@@ -1230,7 +1297,7 @@ int mv_umap_spectral_ws_bytes(int64_t n, int k, int nc, size_t* out_bytes) {
 int mv_umap_spectral(const float* w_sym, const float* eps, const int32_t* tail, int64_t n, int k, int nc,
                      int iters, float* V, void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(w_sym && eps && tail && V, "mv_umap_spectral: null pointer");
-  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX && nc >= 2 && nc <= NC_MAX && iters >= 0,
+  MV_REQUIRE(n >= 1 && k >= 1 && k <= KMAX_GRAPH && nc >= 2 && nc <= NC_MAX && iters >= 0,
              "mv_umap_spectral: bad n=%lld k=%d nc=%d iters=%d", (long long)n, k, nc, iters);
   if (!ws || ws_bytes < spectral_ws(n, k, nc)) {
     mv::set_error("mv_umap_spectral: workspace of %zu bytes < required %zu", ws_bytes, spectral_ws(n, k, nc));

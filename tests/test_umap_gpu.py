@@ -19,14 +19,17 @@ def _graph_inputs(n=3000, d=32, k=15, seed=0):
     return X, idx.astype(np.int32), np.sqrt(d2).astype(np.float32)
 
 
-def test_fuzzy_graph_matches_oracle():
+@pytest.mark.parametrize("k", [15, 100, 256])
+def test_fuzzy_graph_matches_oracle(k):
+    """k above 64 runs the one-wavefront-per-row bisection (sums reduced in a
+    different order: sigma within the same 1e-5)."""
     from mlx_vis_amd._umap import fuzzy_graph_device
 
-    X, idx, dist = _graph_inputs()
+    X, idx, dist = _graph_inputs(k=k)
     n_epochs = 200
     g = fuzzy_graph_device(torch.from_numpy(idx).cuda(), torch.from_numpy(dist).cuda(), n_epochs,
                            keep_sigma=True)
-    w, sigma, rho = ou.fuzzy_rows(dist, 15)
+    w, sigma, rho = ou.fuzzy_rows(dist, k)
     np.testing.assert_array_equal(g.rho.cpu().numpy(), rho)
     np.testing.assert_allclose(g.sigma.cpu().numpy(), sigma, rtol=1e-5)
     ws = ou.symmetrize(idx, w)
@@ -85,7 +88,8 @@ def test_single_sgd_step_matches_fp64_oracle(dim):
 
 @pytest.mark.parametrize("n,k,dim,hubs,masked", [(5000, 15, 2, True, False), (3000, 10, 3, False, True),
                                                    (300_000, 15, 2, True, True), (4000, 15, 6, True, False),
-                                                   (2000, 12, 8, False, True)])
+                                                   (2000, 12, 8, False, True), (3000, 100, 2, True, False),
+                                                   (2500, 256, 3, False, True)])
 def test_epochs_without_negatives_match_oracle(n, k, dim, hubs, masked):
     """With negative_sample_rate 0 a layout epoch is the attraction step of
     the epoch's active edges from one snapshot, so several epochs can be
@@ -280,6 +284,21 @@ def test_fit_transform_end_to_end():
     assert np.isfinite(Y2).all()
     # isotropic clusters: neighbourhoods inside a cluster are noise, so check
     # trustworthiness rather than 15-NN overlap
+    assert om.trustworthiness(X, Y, 15) > 0.85
+
+
+@pytest.mark.parametrize("n_neighbors", [100, 200])
+def test_fit_transform_with_many_neighbours(n_neighbors):
+    """n_neighbors above 64: kNN from the symmetric screen's matrix, the
+    wavefront fuzzy-row bisection, spectral init and epochs of 100 / 200 slots
+    per row."""
+    from mlx_vis_amd import UMAP
+
+    X, _, _ = _graph_inputs(n=3000, d=24, seed=4)
+    um = UMAP(n_neighbors=n_neighbors, n_epochs=100, random_state=0, knn_method="brute")
+    Y = um.fit_transform(X)
+    assert Y.shape == (3000, 2) and np.isfinite(Y).all()
+    assert um.knn_[0].shape == (3000, n_neighbors)
     assert om.trustworthiness(X, Y, 15) > 0.85
 
 
