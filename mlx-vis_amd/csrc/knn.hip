@@ -269,6 +269,27 @@ __global__ __launch_bounds__(256) void k_split_bf16(const float* __restrict__ Xc
 
 // Gathers the FP32 centred rows listed in rows[0:*count] (original query ids)
 // for the re-screen, with their norms and self indices.
+// A few rows that the quantised certificate could not settle (at most
+// `small`) go straight to the exhaustive FP64 scan: a re-screen costs a full
+// pass over X whatever its row count. Sets the re-screen's row count (0 then)
+// and, for the direct route, the scan's row list.
+__global__ void k_route_small(const int32_t* __restrict__ fb1_count, const int32_t* __restrict__ fb1_rows,
+                              int32_t* __restrict__ fb2_count, int32_t* __restrict__ fb2_rows,
+                              int32_t* __restrict__ rs_count, int32_t* __restrict__ stats, int small) {
+  const int c = *fb1_count;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= small) {
+    if (t < c) fb2_rows[t] = fb1_rows[t];
+    if (t == 0) {
+      *fb2_count = c;
+      *rs_count = 0;
+      if (c > 0) atomicAdd(&stats[3], c);
+    }
+  } else if (t == 0) {
+    *rs_count = c;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_gather_rows(const float* __restrict__ Qc, const float* __restrict__ sqq,
                                                      int64_t d_pad, const int32_t* __restrict__ rows,
                                                      const int32_t* __restrict__ count,
@@ -809,7 +830,7 @@ struct SymArgs {
   int64_t n_slots;     // nsb (nsb + 1) / 2 * SYM_G^2 tile slots (some empty)
   int64_t per_xcd;     // slots per XCD (ceil(n_slots / 8))
   int64_t ldd;         // n_pad
-  float* D;            // [n_pad, n_pad]
+  float* D;            // [n_pad, n_pad], FP32 (BF16 bit patterns for the BF16 and FP8 screens)
   float* bmin;         // [n_pad, nb]: per row, the smallest value of each 128-column block
   int n;
   int diag_self;       // the diagonal is each row's excluded self column
@@ -856,8 +877,25 @@ __device__ __forceinline__ void sym_tile(int64_t t, int nb, int& I, int& J) {
 #endif
 constexpr int SYM_W = MV_SYM_WAVES;
 
+// The reduced-precision screens keep their matrix in BF16 (round to nearest
+// even to 8 significant bits: s' <= s (1 + 2^-8), so the screen value
+// s >= s' (1 - 2^-8) for s >= 0), which halves the matrix traffic; the
+// re-rank's bounds scale the stored values by that factor (STORE16_LO).
+constexpr double STORE16_LO = 1.0 - 1.0 / 256.0;
+__device__ __forceinline__ float screen_round16(float v) {
+  return __uint_as_float(static_cast<uint32_t>(__builtin_bit_cast(uint16_t, static_cast<__bf16>(v))) << 16);
+}
+__device__ __forceinline__ uint2 pack16x4(v4f x) {
+  const uint32_t b0 = __float_as_uint(x[0]) >> 16, b1 = __float_as_uint(x[1]) >> 16;
+  const uint32_t b2 = __float_as_uint(x[2]) >> 16, b3 = __float_as_uint(x[3]) >> 16;
+  return make_uint2(b0 | (b1 << 16), b2 | (b3 << 16));
+}
+__device__ __forceinline__ float unpack16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float unpack16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
 template <int DT, int W>
 __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
+  constexpr bool STORE16 = DT == DT_BF16 || DT == DT_FP8;
   constexpr int NTS = 64 * W;
   constexpr int NI = W == 4 ? 4 : 2;          // 16-column accumulator blocks per wavefront
   constexpr int WCOLS = W == 4 ? 2 : 4;       // wavefront columns
@@ -1025,11 +1063,14 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
     float mn = INFINITY;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      x[k] = Dt[row * TS + c4 + k];
+      x[k] = STORE16 ? screen_round16(Dt[row * TS + c4 + k]) : Dt[row * TS + c4 + k];
       const bool ok = c0 + c4 + k < a.n && !(a.diag_self && I == J && c4 + k == row);
       mn = ok ? fminf(mn, x[k]) : mn;
     }
-    *reinterpret_cast<v4f*>(a.D + (r0 + row) * a.ldd + c0 + c4) = x;
+    if constexpr (STORE16)
+      *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(a.D) + (r0 + row) * a.ldd + c0 + c4) = pack16x4(x);
+    else
+      *reinterpret_cast<v4f*>(a.D + (r0 + row) * a.ldd + c0 + c4) = x;
     mn = min32_to_lane31(mn);
     if ((e & 31) == 31) a.bmin[(r0 + row) * a.nb + J] = mn;
   }
@@ -1042,10 +1083,13 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
       float mn = INFINITY;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        x[k] = Dt[(r4 + k) * TS + col];
+        x[k] = STORE16 ? screen_round16(Dt[(r4 + k) * TS + col]) : Dt[(r4 + k) * TS + col];
         mn = r0 + r4 + k < a.n ? fminf(mn, x[k]) : mn;
       }
-      *reinterpret_cast<v4f*>(a.D + (c0 + col) * a.ldd + r0 + r4) = x;
+      if constexpr (STORE16)
+        *reinterpret_cast<uint2*>(reinterpret_cast<uint16_t*>(a.D) + (c0 + col) * a.ldd + r0 + r4) = pack16x4(x);
+      else
+        *reinterpret_cast<v4f*>(a.D + (c0 + col) * a.ldd + r0 + r4) = x;
       mn = min32_to_lane31(mn);
       if ((e & 31) == 31) a.bmin[(c0 + col) * a.nb + I] = mn;
     }
@@ -1053,7 +1097,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
 }
 
 struct SelectArgs {
-  const float* D;         // [n_pad, ldd]
+  const float* D;         // [n_pad, ldd] (S16: BF16 bit patterns)
   int64_t ldd;
   int n, nq, C;
   const int32_t* q_self;  // nullable
@@ -1098,7 +1142,7 @@ __device__ __forceinline__ void select_offer(uint64_t key, uint64_t& lk, uint64_
 // > tau. Within the blocks read, a float threshold (the list's C-th distance)
 // rejects values; keys that pass wait in an LDS buffer of 64 and are merged
 // into the sorted list 64 at a time.
-template <int KPL>
+template <int KPL, bool S16>
 __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
   __shared__ uint64_t s_pend[4][64];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -1138,7 +1182,13 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
         const int jb = (b0 + __builtin_ctzll(need)) * BN;
         need &= need - 1;
         const int64_t j = jb + 2 * l;
-        const float2 v = *reinterpret_cast<const float2*>(Dr + j);
+        float2 v;
+        if constexpr (S16) {
+          const uint32_t w2 = reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint16_t*>(a.D) + row * a.ldd)[j >> 1];
+          v = make_float2(unpack16_lo(w2), unpack16_hi(w2));
+        } else {
+          v = *reinterpret_cast<const float2*>(Dr + j);
+        }
         const bool ok0 = j < a.n && j != self && v.x <= tf;
         const bool ok1 = j + 1 < a.n && j + 1 != self && v.y <= tf;
         select_offer<KPL>(ok0 ? dkey(v.x, static_cast<uint32_t>(j)) : KEY_NONE, lk, lk1, tk, tf, cnt, pend, C, l);
@@ -1149,7 +1199,15 @@ __global__ __launch_bounds__(256) void k_select(SelectArgs a) {
     // the loop bound is uniform: every lane takes part in the merges
     for (int64_t base = 0; base < a.n; base += 256) {
       const int64_t j0 = base + l * 4;
-      const v4f v = j0 < a.ldd ? *reinterpret_cast<const v4f*>(Dr + j0) : v4f{INFINITY, INFINITY, INFINITY, INFINITY};
+      v4f v = v4f{INFINITY, INFINITY, INFINITY, INFINITY};
+      if (j0 < a.ldd) {
+        if constexpr (S16) {
+          const uint2 w2 = *reinterpret_cast<const uint2*>(reinterpret_cast<const uint16_t*>(a.D) + row * a.ldd + j0);
+          v = v4f{unpack16_lo(w2.x), unpack16_hi(w2.x), unpack16_lo(w2.y), unpack16_hi(w2.y)};
+        } else {
+          v = *reinterpret_cast<const v4f*>(Dr + j0);
+        }
+      }
       bool any = false;
 #pragma unroll
       for (int q = 0; q < 4; ++q) any |= v[q] <= tf;
@@ -1178,6 +1236,8 @@ struct RerankArgs {
   const float* sqq;            // centred query norms (screen space)
   const uint32_t* sqmax_bits;  // largest centred data norm
   float err_fac;               // screen arithmetic bound: err_fac * (sqq[i] + sqmax)
+  double store_lo;             // screen values as listed are at most 1 / store_lo times the
+                               // screen's (1: FP32 lists; STORE16_LO: the BF16-stored matrix)
   const float* eq;             // quantisation error bound per query (nullptr: FP32 screen)
   const uint32_t* emax_bits;   // largest quantisation error bound of the data
   const int32_t* nq_dev;       // nullable: device-side row count (re-screen)
@@ -1360,13 +1420,13 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   //    quantisation errors). The k-th candidate must lie strictly below.
   bool cert = nvalid < C;
   if (!cert) {
-    const float tau = key_dist(tau_key);
+    const double tau = static_cast<double>(key_dist(tau_key)) * a.store_lo;
     const float sqmax = __uint_as_float(*a.sqmax_bits);
     const float arith = a.err_fac * (a.sqq[i] + sqmax) + 1e-30f;
     if (!a.eq) {
-      cert = kth_ub < tau - arith;
+      cert = static_cast<double>(kth_ub) < tau - static_cast<double>(arith);
     } else {
-      const double t = fmax(static_cast<double>(tau) - static_cast<double>(arith), 0.0);
+      const double t = fmax(tau - static_cast<double>(arith), 0.0);
       const double r = sqrt(t) - static_cast<double>(a.eq[i]) - static_cast<double>(__uint_as_float(*a.emax_bits));
       cert = r > 0.0 && static_cast<double>(kth_ub) < r * r * (1.0 - 1e-6);
     }
@@ -1462,12 +1522,12 @@ __global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
   const float kth32_ub = kd + kd * bfac + 1e-35f;
   bool need1 = false;
   if (c1 != KEY_NONE) {
-    const float sv = key_dist(c1);
+    const double sv = static_cast<double>(key_dist(c1)) * a.store_lo;
     double lb;
     if (!a.eq) {
-      lb = static_cast<double>(sv) - static_cast<double>(arith);
+      lb = sv - static_cast<double>(arith);
     } else {
-      const double t = fmax(static_cast<double>(sv) - static_cast<double>(arith), 0.0);
+      const double t = fmax(sv - static_cast<double>(arith), 0.0);
       const double r = sqrt(t) - static_cast<double>(a.eq[i]) - static_cast<double>(__uint_as_float(*a.emax_bits));
       lb = r > 0.0 ? r * r * (1.0 - 1e-6) : 0.0;
     }
@@ -1535,11 +1595,11 @@ __global__ __launch_bounds__(256) void k_rerank2(RerankArgs a) {
 
   bool cert = beyond_ok && nall < C;
   if (beyond_ok && !cert) {
-    const float tau = key_dist(tau_key);
+    const double tau = static_cast<double>(key_dist(tau_key)) * a.store_lo;
     if (!a.eq) {
-      cert = kth_ub < tau - arith;
+      cert = static_cast<double>(kth_ub) < tau - static_cast<double>(arith);
     } else {
-      const double t = fmax(static_cast<double>(tau) - static_cast<double>(arith), 0.0);
+      const double t = fmax(tau - static_cast<double>(arith), 0.0);
       const double r = sqrt(t) - static_cast<double>(a.eq[i]) - static_cast<double>(__uint_as_float(*a.emax_bits));
       cert = r > 0.0 && static_cast<double>(kth_ub) < r * r * (1.0 - 1e-6);
     }
@@ -2123,7 +2183,7 @@ struct Buffers {
   float *sqxq, *sqqq, *ex, *eq, *invx, *invq;   // quantised norms, error bounds, 1/scale
   uint64_t* cand;
   uint64_t* pend;   // screen survivor buffers
-  int32_t* ctr;   // [0] fb1 count, [1] sqmax32, [2] sqmaxq, [3] emax, [4] fb2 count, [8..11] stats
+  int32_t* ctr;   // [0] fb1 count, [1] sqmax32, [2] sqmaxq, [3] emax, [4] fb2 count, [6] re-screen rows, [8..11] stats
   int32_t *fb1, *fb2;
   uint64_t* scan_d;                             // split exhaustive scan partial lists
   uint32_t* scan_i;
@@ -2184,7 +2244,9 @@ static Buffers carve(void* ws, int64_t nq, int64_t n, int64_t d, const Layout& L
   b.scan_d = c.take<uint64_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
   b.scan_i = c.take<uint32_t>(static_cast<size_t>(srows) * SCAN_CHUNKS * CMAX_K);
   if (L.sym) {
-    b.Dfull = c.take<float>(static_cast<size_t>(L.n_pad) * L.n_pad);
+    // BF16 and FP8 screens: BF16 values, half the bytes
+    const bool s16 = L.dtype == MV_BF16 || L.dtype == MV_FP8;
+    b.Dfull = c.take<float>(static_cast<size_t>(L.n_pad) * L.n_pad / (s16 ? 2 : 1));
     b.bmin = c.take<float>(static_cast<size_t>(L.n_pad) * (L.n_pad / BN));
   }
   *total = c.off + 256;
@@ -2546,8 +2608,10 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     xa.bmin = (q_self == nullptr && self_b <= 0) ? B.bmin : nullptr;
     xa.nb = static_cast<int>(col_tiles);
     xa.cand = B.cand;
-    if (L.C > 64) hipLaunchKernelGGL(k_select<2>, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
-    else hipLaunchKernelGGL(k_select<1>, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
+    const bool s16 = dtype == MV_BF16 || dtype == MV_FP8;
+    auto ksel = L.C > 64 ? (s16 ? k_select<2, true> : k_select<2, false>)
+                         : (s16 ? k_select<1, true> : k_select<1, false>);
+    hipLaunchKernelGGL(ksel, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, xa);
     MV_LAUNCHED("k_select");
   } else {
     int rc = dtype == MV_F32      ? launch_screen<DT_F32>(sa, grid, s)
@@ -2574,6 +2638,7 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
   const bool exact_form = dtype == MV_F32 || dtype == MV_BF16X3;   // no quantisation-error term
   ra.sqmax_bits = exact_form ? sqmax32 : sqmaxq;
   ra.err_fac = fac;
+  ra.store_lo = L.sym && (dtype == MV_BF16 || dtype == MV_FP8) ? STORE16_LO : 1.0;
   ra.eq = exact_form ? nullptr : B.eq;
   ra.emax_bits = emax;
   ra.nq_dev = nullptr;
@@ -2595,16 +2660,21 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
   const int32_t* scan_rows = B.fb1;
   if (!exact_form) {
     // Re-screen of the rows the quantised certificate could not settle, with
-    // the split-BF16 screen (FP32-class error bound, no quantisation term).
+    // the split-BF16 screen (FP32-class error bound, no quantisation term),
+    // when there are more than SCAN_ROWS of them; fewer go to the scan.
+    int32_t* rs_count = B.ctr + 6;
+    hipLaunchKernelGGL(k_route_small, dim3(static_cast<unsigned>(ceil_div(SCAN_ROWS, 256))), dim3(256), 0, s,
+                       fb1_count, B.fb1, fb2_count, B.fb2, rs_count, stats, SCAN_ROWS);
+    MV_LAUNCHED("k_route_small");
     hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>(ceil_div(nq, 4))), dim3(256), 0, s, B.Qc, B.sqq,
-                       L.d_pad, B.fb1, fb1_count, q_self, self_b, B.Qc2, B.sqq2, B.self2);
+                       L.d_pad, B.fb1, rs_count, q_self, self_b, B.Qc2, B.sqq2, B.self2);
     MV_LAUNCHED("k_gather_rows");
     const int64_t cxs = L.n_pad * L.d_pad, cqs = L.nq_pad * L.d_pad;
     hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cxs, 256))), dim3(256), 0, s, B.Xc, cxs,
-                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xs), static_cast<const int32_t*>(fb1_count));
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Xs), static_cast<const int32_t*>(rs_count));
     MV_LAUNCHED("k_split_bf16");
     hipLaunchKernelGGL(k_split_bf16, dim3(static_cast<unsigned>(ceil_div(cqs, 256))), dim3(256), 0, s, B.Qc2, cqs,
-                       L.d_pad, reinterpret_cast<uint16_t*>(B.Qs2), static_cast<const int32_t*>(fb1_count));
+                       L.d_pad, reinterpret_cast<uint16_t*>(B.Qs2), static_cast<const int32_t*>(rs_count));
     MV_LAUNCHED("k_split_bf16");
     ScreenArgs sb = sa;
     sb.C = Cf;
@@ -2616,7 +2686,7 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     sb.row_bytes = L.d_pad * 4;
     sb.q_self = B.self2;
     sb.self_base = -1;
-    sb.nq_dev = fb1_count;
+    sb.nq_dev = rs_count;
     if (int rc2 = launch_screen<DT_BF16X3>(sb, grid, s)) return rc2;
     RerankArgs rb2 = ra;
     rb2.S = S;   // the fused screen's lists (also after a symmetric first pass)
@@ -2625,8 +2695,9 @@ static int knn_run(const char* who, const float* Q, int64_t nq, const float* X, 
     rb2.sqmax_bits = sqmax32;
     rb2.err_fac = static_cast<float>((3.0 * static_cast<double>(L.d_pad) + 3.0 * 256.0 + 40.0) *
                                      5.9604644775390625e-8 * 1.01);
+    rb2.store_lo = 1.0;   // the fused screen's FP32 lists
     rb2.eq = nullptr;
-    rb2.nq_dev = fb1_count;
+    rb2.nq_dev = rs_count;
     rb2.row_map = B.fb1;
     rb2.fail_stat = 3;
     rb2.fb_count = fb2_count;
