@@ -459,93 +459,6 @@ __device__ __forceinline__ void mfma_chunk(const float* As, const float* Bs, int
   }
 }
 
-// mfma_chunk for a wavefront tile of 64 rows x 16*NI columns (4 x NI
-// accumulators; the columns start at wc * 16 * NI).
-template <int DT, int NI>
-__device__ __forceinline__ void mfma_chunk_n(const float* As, const float* Bs, int wr, int wc, int l,
-                                             v4f (&acc)[4][NI]) {
-  if constexpr (DT == DT_F32) {
-#pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      v4f af[4], bf[NI];
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-        af[mi] = *reinterpret_cast<const v4f*>(As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4);
-#pragma unroll
-      for (int ni = 0; ni < NI; ++ni)
-        bf[ni] = *reinterpret_cast<const v4f*>(Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < NI; ++ni)
-            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mi][j], bf[ni][j], acc[mi][ni], 0, 0, 0);
-    }
-  } else if constexpr (DT == DT_BF16X3) {
-    // one K group of 32: hi in floats [0, 16) of the row, lo in [16, 32)
-    v8bf ah[4], al[4], bh[NI], bl[NI];
-#pragma unroll
-    for (int mi = 0; mi < 4; ++mi) {
-      const float* p = As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + (l >> 4) * 4;
-      ah[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p));
-      al[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p + 16));
-    }
-#pragma unroll
-    for (int ni = 0; ni < NI; ++ni) {
-      const float* p = Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + (l >> 4) * 4;
-      bh[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p));
-      bl[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(p + 16));
-    }
-#pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < NI; ++ni) {
-        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
-        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-      }
-  } else if constexpr (DT == DT_BF16) {
-#pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      v8bf af[4], bf[NI];
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-        af[mi] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(
-                                              As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4));
-#pragma unroll
-      for (int ni = 0; ni < NI; ++ni)
-        bf[ni] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v4f*>(
-                                              Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + g * 16 + (l >> 4) * 4));
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < NI; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bf[ni], acc[mi][ni], 0, 0, 0);
-    }
-  } else {
-    v8i af[4], bf[NI];
-#pragma unroll
-    for (int mi = 0; mi < 4; ++mi) {
-      const float* p = As + (wr * 64 + mi * 16 + (l & 15)) * LDSA + (l >> 4) * 8;
-      const v4f lo = *reinterpret_cast<const v4f*>(p), hi = *reinterpret_cast<const v4f*>(p + 4);
-      af[mi] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
-    }
-#pragma unroll
-    for (int ni = 0; ni < NI; ++ni) {
-      const float* p = Bs + (wc * (16 * NI) + ni * 16 + (l & 15)) * LDSA + (l >> 4) * 8;
-      const v4f lo = *reinterpret_cast<const v4f*>(p), hi = *reinterpret_cast<const v4f*>(p + 4);
-      bf[ni] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
-    }
-#pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < NI; ++ni)
-        acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[mi], bf[ni], acc[mi][ni], 0, 0, 0,
-                                                                        FP8_UNIT_SCALE, 0, FP8_UNIT_SCALE);
-  }
-}
-
 template <int DT>
 __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
   __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
@@ -658,6 +571,10 @@ __global__ __launch_bounds__(NT, 2) void k_screen(ScreenArgs a) {
       // sink them past the MFMAs to reuse operand registers, exposing their latency)
       __builtin_amdgcn_sched_barrier(0);
       mfma_chunk<DT>(s_stage + cur * (2 * BM * LDSA), s_stage + cur * (2 * BM * LDSA) + BM * LDSA, wr, wc, l, acc);
+      // and the next chunk's LDS stores and the barrier behind them (the
+      // scheduler would otherwise hoist both above the MFMAs, so that every
+      // wavefront waits for its loads with the multiplies not yet issued)
+      __builtin_amdgcn_sched_barrier(0);
       if (!last) MV_LSTORE(cur ^ 1);
       __syncthreads();
     }
@@ -893,17 +810,103 @@ __device__ __forceinline__ uint2 pack16x4(v4f x) {
 __device__ __forceinline__ float unpack16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float unpack16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
+// Staging of k_dist_sym: each 128-byte K chunk of a 128-row operand block
+// lands in LDS straight from global memory (global_load_lds_dwordx4: lane L
+// of an instruction writes bytes [16 L, 16 L + 16) past the wavefront-uniform
+// base), 128 bytes per row with no padding. Row r keeps its 16-byte piece c in
+// slot c ^ ((r >> 1) & 7), so that the MFMA operand reads (16 consecutive rows
+// at one piece index per quarter-wavefront) cover all 64 banks; the swizzle is
+// applied on the load side, each lane fetching the piece its slot holds.
+constexpr int SYM_OPB = BM * 128;                    // bytes per staged operand block
+constexpr int SYM_STAGE_BYTES = 2 * 2 * SYM_OPB;     // two buffers x (A, B)
+constexpr int SYM_LDS_FLOATS = (SYM_STAGE_BYTES / 4 > BM * (BN + 1)) ? SYM_STAGE_BYTES / 4 : BM * (BN + 1);
+
+__device__ __forceinline__ int sym_swz(int r, int c) { return r * 128 + ((c ^ ((r >> 1) & 7)) << 4); }
+__device__ __forceinline__ v4f sym_ld(const char* base, int r, int c) {
+  return *reinterpret_cast<const v4f*>(base + sym_swz(r, c));
+}
+
+// One staged K chunk (128 bytes per row) of A rows wr*64 .. +64 and B rows
+// wc*16*NI .. +16*NI into the 4 x NI accumulators of this wavefront.
+template <int DT, int NI>
+__device__ __forceinline__ void mfma_chunk_sym(const char* As, const char* Bs, int wr, int wc, int l,
+                                               v4f (&acc)[4][NI]) {
+  const int ra = wr * 64 + (l & 15), rb = wc * (16 * NI) + (l & 15), q = l >> 4;
+  if constexpr (DT == DT_F32 || DT == DT_BF16) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      v4f af[4], bf[NI];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) af[mi] = sym_ld(As, ra + mi * 16, g * 4 + q);
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) bf[ni] = sym_ld(Bs, rb + ni * 16, g * 4 + q);
+      if constexpr (DT == DT_F32) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < NI; ++ni)
+              acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[mi][j], bf[ni][j], acc[mi][ni], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < NI; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, af[mi]),
+                                                                  __builtin_bit_cast(v8bf, bf[ni]), acc[mi][ni], 0, 0, 0);
+      }
+    }
+  } else if constexpr (DT == DT_BF16X3) {
+    // one K group of 32: hi in pieces 0..3 of the row, lo in pieces 4..7
+    v8bf ah[4], al[4], bh[NI], bl[NI];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      ah[mi] = __builtin_bit_cast(v8bf, sym_ld(As, ra + mi * 16, q));
+      al[mi] = __builtin_bit_cast(v8bf, sym_ld(As, ra + mi * 16, q + 4));
+    }
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      bh[ni] = __builtin_bit_cast(v8bf, sym_ld(Bs, rb + ni * 16, q));
+      bl[ni] = __builtin_bit_cast(v8bf, sym_ld(Bs, rb + ni * 16, q + 4));
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni) {
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+      }
+  } else {
+    v8i af[4], bf[NI];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const v4f lo = sym_ld(As, ra + mi * 16, 2 * q), hi = sym_ld(As, ra + mi * 16, 2 * q + 1);
+      af[mi] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+    }
+#pragma unroll
+    for (int ni = 0; ni < NI; ++ni) {
+      const v4f lo = sym_ld(Bs, rb + ni * 16, 2 * q), hi = sym_ld(Bs, rb + ni * 16, 2 * q + 1);
+      bf[ni] = __builtin_bit_cast(v8i, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < NI; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[mi], bf[ni], acc[mi][ni], 0, 0, 0,
+                                                                        FP8_UNIT_SCALE, 0, FP8_UNIT_SCALE);
+  }
+}
+
 template <int DT, int W>
 __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
   constexpr bool STORE16 = DT == DT_BF16 || DT == DT_FP8;
   constexpr int NTS = 64 * W;
   constexpr int NI = W == 4 ? 4 : 2;          // 16-column accumulator blocks per wavefront
   constexpr int WCOLS = W == 4 ? 2 : 4;       // wavefront columns
-  constexpr int NQ = 128 / (NTS / 8);         // staged rows per thread and operand
-  // two register sets (a chunk loaded two steps before it is stored to LDS)
-  // where the multiplies of a chunk are short and registers allow
-  constexpr bool PF2 = DT != DT_F32 && W == 4;
-  __shared__ __attribute__((aligned(16))) float s_stage[STAGE_FLOATS];
+  constexpr int NQ = BM / (8 * W);            // load instructions per operand and chunk (8 rows each)
+  __shared__ __attribute__((aligned(16))) float s_stage[SYM_LDS_FLOATS];
   // Workgroups are dispatched round-robin over the 8 XCDs (bid mod 8); XCD x
   // takes a contiguous run of tile slots. Slots are ordered by super-block
   // (SYM_G x SYM_G tiles, upper triangle of super-blocks in row-major order),
@@ -926,43 +929,25 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
   const int64_t rbytes = a.row_bytes;
   const char* Ab = a.Xq + static_cast<int64_t>(I) * BM * rbytes;
   const char* Bb = a.Xq + static_cast<int64_t>(J) * BN * rbytes;
-  const int lr = tid >> 3;            // staged row (plus NTS / 8 * q)
-  const int lkk = (tid & 7) * 4;
   const int nk = static_cast<int>(rbytes / 128);
+  char* stage = reinterpret_cast<char*>(s_stage);
 
-  float4 ra0_0, ra1_0, ra2_0, ra3_0, rb0_0, rb1_0, rb2_0, rb3_0;
-  float4 ra0_1, ra1_1, ra2_1, ra3_1, rb0_1, rb1_1, rb2_1, rb3_1;
-#define MV_SGLOAD(SET, KCC)                                                                 \
-  do {                                                                                    \
-    const int64_t ko_ = static_cast<int64_t>(KCC) * 128 + lkk * 4;                        \
-    constexpr int st_ = NTS / 8;                                                          \
-    ra0_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr) * rbytes + ko_);            \
-    ra1_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + st_) * rbytes + ko_);      \
-    rb0_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr) * rbytes + ko_);            \
-    rb1_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + st_) * rbytes + ko_);      \
-    if constexpr (NQ == 4) {                                                              \
-      ra2_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 2 * st_) * rbytes + ko_); \
-      ra3_##SET = *reinterpret_cast<const float4*>(Ab + static_cast<int64_t>(lr + 3 * st_) * rbytes + ko_); \
-      rb2_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 2 * st_) * rbytes + ko_); \
-      rb3_##SET = *reinterpret_cast<const float4*>(Bb + static_cast<int64_t>(lr + 3 * st_) * rbytes + ko_); \
-    }                                                                                     \
-  } while (0)
-#define MV_SLSTORE(SET, BUF)                                                                \
-  do {                                                                                    \
-    constexpr int st_ = NTS / 8;                                                          \
-    float* As2_ = s_stage + (BUF) * (2 * BM * LDSA);                                      \
-    float* Bs2_ = As2_ + BM * LDSA;                                                       \
-    *reinterpret_cast<float4*>(As2_ + (lr) * LDSA + lkk) = ra0_##SET;                     \
-    *reinterpret_cast<float4*>(As2_ + (lr + st_) * LDSA + lkk) = ra1_##SET;               \
-    *reinterpret_cast<float4*>(Bs2_ + (lr) * LDSA + lkk) = rb0_##SET;                     \
-    *reinterpret_cast<float4*>(Bs2_ + (lr + st_) * LDSA + lkk) = rb1_##SET;               \
-    if constexpr (NQ == 4) {                                                              \
-      *reinterpret_cast<float4*>(As2_ + (lr + 2 * st_) * LDSA + lkk) = ra2_##SET;         \
-      *reinterpret_cast<float4*>(As2_ + (lr + 3 * st_) * LDSA + lkk) = ra3_##SET;         \
-      *reinterpret_cast<float4*>(Bs2_ + (lr + 2 * st_) * LDSA + lkk) = rb2_##SET;         \
-      *reinterpret_cast<float4*>(Bs2_ + (lr + 3 * st_) * LDSA + lkk) = rb3_##SET;         \
-    }                                                                                     \
-  } while (0)
+  // load instruction i of this wavefront: rows (i W + w) 8 + L / 8, slot L % 8
+  int64_t goff[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    const int r = (i * W + w) * 8 + (l >> 3);
+    goff[i] = static_cast<int64_t>(r) * rbytes + (((l & 7) ^ ((r >> 1) & 7)) << 4);
+  }
+  auto issue = [&](int kc, int buf) {
+    char* dA = stage + buf * (2 * SYM_OPB);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      const int64_t o = goff[i] + static_cast<int64_t>(kc) * 128;
+      __builtin_amdgcn_global_load_lds(Ab + o, dA + (i * W + w) * 1024, 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(Bb + o, dA + SYM_OPB + (i * W + w) * 1024, 16, 0, 0);
+    }
+  };
 
   v4f acc[4][NI];
 #pragma unroll
@@ -970,45 +955,23 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
 #pragma unroll
     for (int ni = 0; ni < NI; ++ni) acc[mi][ni] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  float* L0 = s_stage;
-  float* L1 = s_stage + 2 * BM * LDSA;
-  if constexpr (!PF2) {
-    MV_SGLOAD(0, 0);
-    MV_SLSTORE(0, 0);
+  // Chunk kc + 1 streams into the other buffer while chunk kc is multiplied;
+  // that buffer was last read in step kc - 1, which every wavefront has left
+  // (barrier at its end). Each wavefront waits for its own loads, then the
+  // barrier makes every wavefront's loads visible.
+  issue(0, 0);
+  __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int cur = kc & 1;
+    if (kc + 1 < nk) issue(kc + 1, cur ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+    const char* As = stage + cur * (2 * SYM_OPB);
+    mfma_chunk_sym<DT, NI>(As, As + SYM_OPB, wr, wc, l, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
-    for (int kc = 0; kc < nk; ++kc) {
-      const int cur = kc & 1;
-      const bool last = kc + 1 == nk;
-      if (!last) MV_SGLOAD(0, kc + 1);
-      __builtin_amdgcn_sched_barrier(0);
-      mfma_chunk_n<DT, NI>(cur ? L1 : L0, (cur ? L1 : L0) + BM * LDSA, wr, wc, l, acc);
-      if (!last) MV_SLSTORE(0, cur ^ 1);
-      __syncthreads();
-    }
-  } else {
-    MV_SGLOAD(0, 0);
-    MV_SLSTORE(0, 0);
-    if (nk > 1) MV_SGLOAD(1, 1);
-    if (nk > 2) MV_SGLOAD(0, 2);
-    __syncthreads();
-    for (int kc = 0; kc < nk; kc += 2) {
-      // chunk kc in L0, kc + 1 in set 1, kc + 2 in set 0 (in flight)
-      if (kc + 1 < nk) MV_SLSTORE(1, 1);
-      if (kc + 3 < nk) MV_SGLOAD(1, kc + 3);
-      __builtin_amdgcn_sched_barrier(0);
-      mfma_chunk_n<DT, NI>(L0, L0 + BM * LDSA, wr, wc, l, acc);
-      __syncthreads();
-      if (kc + 1 >= nk) break;
-      // chunk kc + 1 in L1, kc + 2 in set 0, kc + 3 in set 1 (in flight)
-      if (kc + 2 < nk) MV_SLSTORE(0, 0);
-      if (kc + 4 < nk) MV_SGLOAD(0, kc + 4);
-      __builtin_amdgcn_sched_barrier(0);
-      mfma_chunk_n<DT, NI>(L1, L1 + BM * LDSA, wr, wc, l, acc);
-      __syncthreads();
-    }
   }
-#undef MV_SGLOAD
-#undef MV_SLSTORE
 
   // Epilogue: lane (g = l / 16, c = l % 16) of (mi, ni) holds rows
   // wr*64 + mi*16 + 4g + r (r = 0..3) of column wc*16*NI + ni*16 + c. The
@@ -1017,7 +980,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_dist_sym(SymArgs a) {
   // so that both the block and its transpose leave as rows of 512 bytes (32
   // lanes x 16 bytes) instead of 64-byte pieces.
   constexpr int TS = BN + 1;
-  static_assert(BM * TS <= STAGE_FLOATS, "the screen tile must fit in the staging area");
+  static_assert(BM * TS <= SYM_LDS_FLOATS, "the screen tile must fit in the staging area");
   float* Dt = s_stage;
   {
     const int g = l >> 4, c = l & 15;

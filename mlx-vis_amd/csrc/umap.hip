@@ -273,12 +273,13 @@ __device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 
 // so that the slot schedules, tail indices, in-edge positions and tail
 // positions load with several gathers in flight per lane. Only the active
 // slots of the epoch (on average a third at 70,000 x 784) carry the expensive
-// part, the negative draws and gradients: they are compacted per wavefront
-// through LDS (slot, in-edge position, head and tail coordinates, in slot
-// order u-major, lane-minor) and processed one item per lane, usually in a
-// single round instead of S partly idle ones. Per item, the negatives
-// (Philox-4x32-10 counter (slot, epoch, group of 4)) are drawn and gathered
-// before any gradient; the item's tail update (minus the attractive gradient)
+// part, the tail and negative gathers and the gradients: they are compacted
+// per wavefront through LDS (slot, in-edge position, tail index, head
+// coordinates, in slot order u-major, lane-minor) and processed one item per
+// lane, usually in a single round instead of S partly idle ones. Per item, the
+// tail's position and the negatives (Philox-4x32-10 counter (slot, epoch,
+// group of 4)) are gathered together, so that an item waits for one memory
+// round trip after its slot's schedule; the item's tail update (minus the attractive gradient)
 // is stored per slot to gtail, its head update goes back through LDS to the
 // slot's lane. The head's updates are reduced over the row's G lanes into the
 // next positions Ynext[i] (Y is double buffered). No atomics: a point that is
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   __shared__ int32_t s_e[4][NI];
   __shared__ int32_t s_pos[4][NI];
   __shared__ float s_yi[4][NI][D];   // head position, then the item's head update
-  __shared__ float s_yj[4][NI][D];
+  __shared__ int32_t s_j[4][NI];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;      // head row
@@ -326,17 +327,10 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   float yi[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) yi[q] = i < n ? Y[i * D + q] : 0.f;
-  float yj[S][D];
 #pragma unroll
   for (int u = 0; u < S; ++u) {
     act[u] = in_row[u] && sch[u] > 0.f && nx[u] <= static_cast<float>(epoch);
-#pragma unroll
-    for (int q = 0; q < D; ++q) yj[u][q] = 0.f;
-    if (act[u]) {
-      epn[e[u]] = nx[u] + sch[u];
-#pragma unroll
-      for (int q = 0; q < D; ++q) yj[u][q] = Y[static_cast<int64_t>(j[u]) * D + q];
-    }
+    if (act[u]) epn[e[u]] = nx[u] + sch[u];
   }
   // compaction of the active slots: item rank[u] of this wavefront
   int rank[S];
@@ -350,11 +344,9 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     if (act[u]) {
       s_e[w][rank[u]] = static_cast<int32_t>(e[u]);
       s_pos[w][rank[u]] = static_cast<int32_t>(pos[u]);
+      s_j[w][rank[u]] = j[u];
 #pragma unroll
-      for (int q = 0; q < D; ++q) {
-        s_yi[w][rank[u]][q] = yi[q];
-        s_yj[w][rank[u]][q] = yj[u][q];
-      }
+      for (int q = 0; q < D; ++q) s_yi[w][rank[u]][q] = yi[q];
     }
   }
   __syncthreads();
@@ -363,11 +355,12 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     if (pi < A) {
       const int32_t ei = s_e[w][pi];
       const int64_t pe = s_pos[w][pi];
+      const int64_t ji = s_j[w][pi];
       float hi_[D], hj[D], h[D];
 #pragma unroll
       for (int q = 0; q < D; ++q) {
+        hj[q] = Y[ji * D + q];
         hi_[q] = s_yi[w][pi][q];
-        hj[q] = s_yj[w][pi][q];
         h[q] = 0.f;
       }
       for (int base = 0; base < neg_rate; base += NEG_CHUNK) {
