@@ -1310,58 +1310,98 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   const uint64_t tau_key = readlane_u64(mine, C - 1);
   const int cidx = mine != KEY_NONE ? static_cast<int>(key_idx(mine)) : 0;
 
-  // 2. FP32 direct-difference distances: 4 candidates at a time, 16 lanes each.
+  // 2. FP32 direct-difference distances, 4 candidates at a time (16 lanes
+  //    each), of the candidates that can still reach the top k: first the k
+  //    with the smallest screen values, whose largest distance plus its bound
+  //    (thr) bounds the k-th true distance from above; then only the others
+  //    whose screen value, less the screen's error bound, is not above thr.
+  //    The rest are farther than all of the first k and are left out of the
+  //    sort (ncomp candidates are computed); they still count for the
+  //    certificate through tau.
   const float* qrow = a.Q + static_cast<int64_t>(orow) * a.d;
   const int g = l >> 4, sub = l & 15;
-  float d32 = INFINITY;
-  for (int r = 0; r * 4 < nvalid; ++r) {
-    const int c = r * 4 + g;
-    const int j = __shfl(cidx, c < 64 ? c : 0);
-    float acc = 0.f;
-    if (c < nvalid) {
-      const float* xr = a.X + static_cast<int64_t>(j) * a.d;
-      if (a.vec4) {
-        // 16 lanes x 16-byte loads cover 64 contiguous floats per step.
-        const float4* q4 = reinterpret_cast<const float4*>(qrow);
-        const float4* x4 = reinterpret_cast<const float4*>(xr);
-        for (int64_t t = sub; t < a.d / 4; t += 16) {
-          const float4 qa = q4[t], xb = x4[t];
-          float df = qa.x - xb.x;
-          acc = fmaf(df, df, acc);
-          df = qa.y - xb.y;
-          acc = fmaf(df, df, acc);
-          df = qa.z - xb.z;
-          acc = fmaf(df, df, acc);
-          df = qa.w - xb.w;
-          acc = fmaf(df, df, acc);
-        }
-      } else {
-        for (int64_t t = sub; t < a.d; t += 16) {
-          const float df = qrow[t] - xr[t];
-          acc = fmaf(df, df, acc);
-        }
-      }
-    }
-    acc += __shfl_xor(acc, 8);
-    acc += __shfl_xor(acc, 4);
-    acc += __shfl_xor(acc, 2);
-    acc += __shfl_xor(acc, 1);
-    const float got = __shfl(acc, (l & 3) * 16);
-    if ((l >> 2) == r) d32 = got;
-  }
-
-  // 3. Sort by (FP32 distance, index); flag pairs FP32 cannot separate.
-  uint64_t k32 = l < nvalid ? dkey(d32, static_cast<uint32_t>(cidx)) : KEY_NONE;
-  k32 = wave_sort_u64(k32);
   // Each lane sums ceil(d/16) fmaf terms in order, then 4 butterfly adds;
   // every term carries the rounding of q - x (relative 2u after squaring).
   const int64_t terms = a.vec4 ? 4 * ((a.d + 63) / 64) : (a.d + 15) / 16;
   const float bfac = static_cast<float>(terms + 8) * 5.9604645e-8f * 1.01f;
+  float d32 = INFINITY;
+  auto compute = [&](uint64_t m) {
+    while (m) {
+      int p[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        p[q] = m ? __builtin_ctzll(m) : -1;
+        m = m ? m & (m - 1) : 0;
+      }
+      const int c = g == 0 ? p[0] : (g == 1 ? p[1] : (g == 2 ? p[2] : p[3]));
+      const int j = __shfl(cidx, c >= 0 ? c : 0);
+      float acc = 0.f;
+      if (c >= 0) {
+        const float* xr = a.X + static_cast<int64_t>(j) * a.d;
+        if (a.vec4) {
+          // 16 lanes x 16-byte loads cover 64 contiguous floats per step.
+          const float4* q4 = reinterpret_cast<const float4*>(qrow);
+          const float4* x4 = reinterpret_cast<const float4*>(xr);
+          for (int64_t t = sub; t < a.d / 4; t += 16) {
+            const float4 qa = q4[t], xb = x4[t];
+            float df = qa.x - xb.x;
+            acc = fmaf(df, df, acc);
+            df = qa.y - xb.y;
+            acc = fmaf(df, df, acc);
+            df = qa.z - xb.z;
+            acc = fmaf(df, df, acc);
+            df = qa.w - xb.w;
+            acc = fmaf(df, df, acc);
+          }
+        } else {
+          for (int64_t t = sub; t < a.d; t += 16) {
+            const float df = qrow[t] - xr[t];
+            acc = fmaf(df, df, acc);
+          }
+        }
+      }
+      acc += __shfl_xor(acc, 8);
+      acc += __shfl_xor(acc, 4);
+      acc += __shfl_xor(acc, 2);
+      acc += __shfl_xor(acc, 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float got = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 16 * q));
+        if (l == p[q]) d32 = got;
+      }
+    }
+  };
+  const int kfirst = nvalid < k ? nvalid : k;
+  const uint64_t first = kfirst >= 64 ? ~0ull : ((1ull << kfirst) - 1);
+  compute(first);
+  {
+    float ub = l < kfirst ? d32 + (d32 * bfac + 1e-35f) : 0.f;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) ub = fmaxf(ub, __shfl_xor(ub, m));
+    const double thr = static_cast<double>(ub);
+    // lower bound of the true squared distance of candidate l from its screen value
+    const double sv = static_cast<double>(key_dist(mine)) * a.store_lo;
+    const float sqmax0 = __uint_as_float(*a.sqmax_bits);
+    const double ar = static_cast<double>(a.err_fac * (a.sqq[i] + sqmax0) + 1e-30f);
+    double lb = sv - ar;
+    if (a.eq) {
+      const double r = sqrt(fmax(sv - ar, 0.0)) - static_cast<double>(a.eq[i]) -
+                       static_cast<double>(__uint_as_float(*a.emax_bits));
+      lb = r > 0.0 ? r * r * (1.0 - 1e-6) : 0.0;
+    }
+    const bool need = l >= kfirst && l < nvalid && lb <= thr;
+    compute(ballot(need));
+  }
+  const int ncomp = __popcll(ballot(d32 < INFINITY));
+
+  // 3. Sort by (FP32 distance, index); flag pairs FP32 cannot separate.
+  uint64_t k32 = d32 < INFINITY ? dkey(d32, static_cast<uint32_t>(cidx)) : KEY_NONE;
+  k32 = wave_sort_u64(k32);
   const float ds = key_dist(k32);
   const float bd = ds * bfac + 1e-35f;
   const float dn = __shfl_down(ds, 1);
   const float bn = __shfl_down(bd, 1);
-  const bool amb = (l < k) && (l + 1 < nvalid) && (dn - bn <= ds + bd);
+  const bool amb = (l < k) && (l + 1 < ncomp) && (dn - bn <= ds + bd);
   const bool any_amb = ballot(amb) != 0;
 
   uint32_t out_i = key_idx(k32);
@@ -1370,9 +1410,9 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs a) {
   if (any_amb) {
     // 4. FP64 tie check: canonical distances for every candidate, then sort.
     double d64 = INFINITY;
-    if (l < nvalid) d64 = dist64_seq(qrow, a.X + static_cast<int64_t>(out_i) * a.d, a.d);
-    uint64_t dbits = l < nvalid ? static_cast<uint64_t>(__double_as_longlong(d64)) : KEY_NONE;
-    uint32_t di = l < nvalid ? out_i : 0xFFFFFFFFu;
+    if (l < ncomp) d64 = dist64_seq(qrow, a.X + static_cast<int64_t>(out_i) * a.d, a.d);
+    uint64_t dbits = l < ncomp ? static_cast<uint64_t>(__double_as_longlong(d64)) : KEY_NONE;
+    uint32_t di = l < ncomp ? out_i : 0xFFFFFFFFu;
     wave_sort_pair64(dbits, di);
     const double dv = __longlong_as_double(static_cast<long long>(dbits));
     out_i = di;
