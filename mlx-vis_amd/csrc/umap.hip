@@ -799,14 +799,26 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
 #pragma unroll
   for (int q = 0; q < NC; ++q) t[q] = 0.f;
   if (i < n) {
-    for (int c = 0; c < k; ++c) {
-      const int64_t e = i * k + c;
-      const float w = wn[e];
-      if (w != 0.f) {
-        const int64_t j = tail[e];
+    // no branch on w: a pruned edge (w = 0, its tail still a valid index) adds
+    // an exact zero, and every gather of the row is in flight at once
+    const int64_t e0 = i * k, elast = e0 + k - 1;
+    for (int c = 0; c < k; c += 8) {
+      float w8[8], u8[8][NC];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) t[q] += w * U[j * NC + q];
+      for (int v = 0; v < 8; ++v) {
+        // past the row: the row's last slot again, with weight 0 (the loads
+        // stay unconditional, so that all of them are in flight together)
+        const int64_t ev = e0 + c + v < elast ? e0 + c + v : elast;
+        const float keep = c + v < k ? 1.f : 0.f;
+        w8[v] = wn[ev] * keep;
+        const int64_t j = tail[ev];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) u8[v][q] = U[j * NC + q];
       }
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) t[q] += w8[v] * u8[v][q];
     }
   }
   float u[NC];
