@@ -270,27 +270,22 @@ __device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 
 
 // One synchronous epoch, phase 1, over the dense slot array. Each head row
 // is served by G lanes, each lane holding S slots (c, c + G, ...; G S >= k),
-// so that the slot schedules, tail indices, in-edge positions and tail
-// positions load with several gathers in flight per lane. Only the active
-// slots of the epoch (on average a third at 70,000 x 784) carry the expensive
-// part, the tail and negative gathers and the gradients: they are compacted
-// per wavefront through LDS (slot, in-edge position, tail index, head
-// coordinates, in slot order u-major, lane-minor) and processed one item per
-// lane, usually in a single round instead of S partly idle ones. Per item, the
-// tail's position and the negatives (Philox-4x32-10 counter (slot, epoch,
-// group of 4)) are gathered together, so that an item waits for one memory
-// round trip after its slot's schedule; the item's tail update (minus the attractive gradient)
-// is stored per slot to gtail, its head update goes back through LDS to the
-// slot's lane. The head's updates are reduced over the row's G lanes into the
-// next positions Ynext[i] (Y is double buffered). No atomics: a point that is
-// the tail of many edges (a hub of the kNN graph) is summed in phase 2. In
-// in-edge order (pos_in set) only active slots store, since phase 2 clears
-// what it reads; in slot order every slot stores (zero when inactive).
+// so that the slot schedules and tail indices load with several gathers in
+// flight per lane. Only the active slots of the epoch (on average a third at
+// 70,000 x 784) carry the expensive part, the tail and negative gathers and
+// the gradients: they are compacted per wavefront through LDS (slot, tail
+// index, head coordinates, in slot order u-major, lane-minor) and processed
+// one item per lane, usually in a single round instead of S partly idle ones.
+// Per item, the tail's position and the negatives (Philox-4x32-10 counter
+// (slot, epoch, group of 4)) are gathered together, so that an item waits for
+// one memory round trip after its slot's schedule; the item's head update goes
+// back through LDS to the slot's lane. The head's updates are reduced over the
+// row's G lanes into the next positions Ynext[i] (Y is double buffered). The
+// tail updates are phase 2's: no atomics here, and nothing is stored per slot.
 constexpr int NEG_CHUNK = 8;
 
 template <int D, int G, int S>
 __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ Ynext,
-                                               float* __restrict__ gtail, const int32_t* __restrict__ pos_in,
                                                float* __restrict__ wbase, const int32_t* __restrict__ span,
                                                const uint8_t* __restrict__ movable, const int32_t* __restrict__ tail,
                                                const float* __restrict__ eps, float* __restrict__ epn, int64_t n,
@@ -298,14 +293,13 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
                                                uint32_t seed_hi) {
   constexpr int NI = 64 * S;   // active items of a wavefront, at most
   __shared__ int32_t s_e[4][NI];
-  __shared__ int32_t s_pos[4][NI];
   __shared__ float s_yi[4][NI][D];   // head position, then the item's head update
   __shared__ int32_t s_j[4][NI];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;      // head row
   const int c = static_cast<int>(gid % G);
-  int64_t e[S], pos[S];
+  int64_t e[S];
   float sch[S], nx[S];
   int32_t j[S];
   bool in_row[S], act[S];
@@ -316,12 +310,10 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     sch[u] = -1.f;
     nx[u] = INFINITY;
     j[u] = 0;
-    pos[u] = -1;
     if (in_row[u]) {
       sch[u] = eps[e[u]];
       nx[u] = epn[e[u]];
       j[u] = tail[e[u]];
-      pos[u] = pos_in ? static_cast<int64_t>(pos_in[e[u]]) : e[u];
     }
   }
   float yi[D];
@@ -343,7 +335,6 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     A += __popcll(m);
     if (act[u]) {
       s_e[w][rank[u]] = static_cast<int32_t>(e[u]);
-      s_pos[w][rank[u]] = static_cast<int32_t>(pos[u]);
       s_j[w][rank[u]] = j[u];
 #pragma unroll
       for (int q = 0; q < D; ++q) s_yi[w][rank[u]][q] = yi[q];
@@ -354,7 +345,6 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     const int pi = r * 64 + l;
     if (pi < A) {
       const int32_t ei = s_e[w][pi];
-      const int64_t pe = s_pos[w][pi];
       const int64_t ji = s_j[w][pi];
       float hi_[D], hj[D], h[D];
 #pragma unroll
@@ -406,9 +396,8 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
       const float ca = law.attract(d2);
 #pragma unroll
       for (int q = 0; q < D; ++q) {
-        const float g = clip4(ca * diff[q]) * law.alpha;
-        h[q] += g;
-        gtail[pe * D + q] = -g;
+        // rounded product (not fused into the add): phase 2 subtracts the same value
+        h[q] += __fmul_rn(clip4(ca * diff[q]), law.alpha);
         s_yi[w][pi][q] = h[q];
       }
     }
@@ -422,10 +411,6 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     if (act[u]) {
 #pragma unroll
       for (int q = 0; q < D; ++q) acc[q] += s_yi[w][rank[u]][q];
-    } else if (in_row[u] && !pos_in) {
-      // slot order: phase 2 reads every slot
-#pragma unroll
-      for (int q = 0; q < D; ++q) gtail[e[u] * D + q] = 0.f;
     }
   }
   // Reduce the head's updates over its G lanes (contiguous in the wavefront).
@@ -457,26 +442,29 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   }
 }
 
-// Phase 2: tail sums, applied. Phase 1 stored the tail gradients in in-edge
-// order (entries of a node contiguous, in_node[t] the node of entry t), read
-// contiguously and cleared, or, for buffers larger than the caches, in slot
-// order, gathered through in_slots (scattered 8-byte writes cost more than
-// scattered reads once the lines are not cache resident). A
-// wavefront-segmented inclusive scan adds each node's entries, and the last
-// lane of a segment adds the sum to Ynext with one atomic, onto the value
-// phase 1 left there: Y + head update for a node whose entries lie in one
-// wavefront (one term onto a fixed value), 0 for a node crossing a wavefront
-// boundary, whose first segment adds wbase[w] (its Y + head update, indexed
-// by the wavefront) with its sum. Over two wavefronts the result does not
-// depend on the order of the two atomics (two terms onto zero commute); only a
-// node with more than 64 in-edges can span more. Every load the wavefront
-// needs is independent of the others, so it waits for one memory latency.
-// Frozen nodes (movable[j] == 0) are skipped.
+// Phase 2: tail sums, applied, over the in-edge entries (entries of a node
+// contiguous, in slot order; in_node[t] the node of entry t, in_head[t] the
+// head of its slot). The entries keep their own copy of the schedule (eps_in,
+// epn_in in entry order), advanced by the same FP32 additions as the slots'
+// epn in phase 1, so both phases agree on the active edges of every epoch. An
+// active entry computes minus the attractive update that phase 1 applied to
+// its head, from the same two positions with the same expression, so the
+// value is the same to the bit. A wavefront-segmented inclusive scan adds
+// each node's entries, and the last lane of a segment adds the sum to Ynext
+// with one atomic, onto the value phase 1 left there: Y + head update for a
+// node whose entries lie in one wavefront (one term onto a fixed value), 0
+// for a node crossing a wavefront boundary, whose first segment adds wbase[w]
+// (its Y + head update, indexed by the wavefront) with its sum. Over two
+// wavefronts the result does not depend on the order of the two atomics (two
+// terms onto zero commute); only a node with more than 64 in-edges can span
+// more. Frozen nodes (movable[j] == 0) are skipped.
 template <int D>
-__global__ __launch_bounds__(256) void k_tail_apply(float* __restrict__ gtail, const int32_t* __restrict__ in_slots,
-                                                    const int32_t* __restrict__ in_node, int64_t m,
-                                                    float* __restrict__ Ynext, const float* __restrict__ wbase,
-                                                    const uint8_t* __restrict__ movable) {
+__global__ __launch_bounds__(256) void k_tail_apply(const int32_t* __restrict__ in_node,
+                                                    const int32_t* __restrict__ in_head,
+                                                    const float* __restrict__ eps_in, float* __restrict__ epn_in,
+                                                    int64_t m, const float* __restrict__ Y, float* __restrict__ Ynext,
+                                                    const float* __restrict__ wbase, const uint8_t* __restrict__ movable,
+                                                    int epoch, Law law) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int l = threadIdx.x & 63;
   const int64_t w0 = t - l;  // first entry of this wavefront
@@ -488,15 +476,24 @@ __global__ __launch_bounds__(256) void k_tail_apply(float* __restrict__ gtail, c
   float base[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) base[q] = l == 63 && valid ? wbase[(w0 >> 6) * D + q] : 0.f;
+  const float es = valid ? eps_in[t] : -1.f;
+  const float ne = valid ? epn_in[t] : INFINITY;
+  const int64_t hd = valid ? in_head[t] : 0;
   float v[D];
-  // in_slots == nullptr: stored in in-edge order; else in slot order, gathered
-  const int64_t src = valid ? (in_slots ? static_cast<int64_t>(in_slots[t]) : t) : 0;
 #pragma unroll
-  for (int q = 0; q < D; ++q) v[q] = valid ? gtail[src * D + q] : 0.f;
-  // in-edge order: phase 1 stores only active slots, so leave the entries cleared
-  if (valid && !in_slots) {
+  for (int q = 0; q < D; ++q) v[q] = 0.f;
+  if (es > 0.f && ne <= static_cast<float>(epoch)) {
+    epn_in[t] = ne + es;
+    float diff[D];
+    float d2 = 0.f;
 #pragma unroll
-    for (int q = 0; q < D; ++q) gtail[src * D + q] = 0.f;
+    for (int q = 0; q < D; ++q) {
+      diff[q] = Y[hd * D + q] - Y[static_cast<int64_t>(node) * D + q];
+      d2 += diff[q] * diff[q];
+    }
+    const float ca = law.attract(d2);
+#pragma unroll
+    for (int q = 0; q < D; ++q) v[q] = -__fmul_rn(clip4(ca * diff[q]), law.alpha);
   }
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -536,12 +533,6 @@ __global__ __launch_bounds__(256) void k_span(const int32_t* __restrict__ in_nod
     atomicMin(reinterpret_cast<uint32_t*>(&span[in_node[t]]), static_cast<uint32_t>((t >> 6) - 1));
 }
 
-// pos_in[e] = t for the in-edge list entry t of slot e (pos_in pre-filled with -1).
-__global__ __launch_bounds__(256) void k_pos_in(const int32_t* __restrict__ in_slots, int64_t m,
-                                                int32_t* __restrict__ pos_in) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t < m) pos_in[in_slots[t]] = static_cast<int32_t>(t);
-}
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int64_t count, int32_t v) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -800,12 +791,13 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
   for (int q = 0; q < NC; ++q) t[q] = 0.f;
   if (i < n) {
     // no branch on w: a pruned edge (w = 0, its tail still a valid index) adds
-    // an exact zero, and every gather of the row is in flight at once
+    // an exact zero, and a batch of gathers of the row is in flight at once
     const int64_t e0 = i * k, elast = e0 + k - 1;
-    for (int c = 0; c < k; c += 8) {
-      float w8[8], u8[8][NC];
+    constexpr int NB = NC <= 5 ? 8 : 4;   // batch (wide vectors: fewer, to stay in registers)
+    for (int c = 0; c < k; c += NB) {
+      float w8[NB], u8[NB][NC];
 #pragma unroll
-      for (int v = 0; v < 8; ++v) {
+      for (int v = 0; v < NB; ++v) {
         // past the row: the row's last slot again, with weight 0 (the loads
         // stay unconditional, so that all of them are in flight together)
         const int64_t ev = e0 + c + v < elast ? e0 + c + v : elast;
@@ -816,7 +808,7 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
         for (int q = 0; q < NC; ++q) u8[v][q] = U[j * NC + q];
       }
 #pragma unroll
-      for (int v = 0; v < 8; ++v)
+      for (int v = 0; v < NB; ++v)
 #pragma unroll
         for (int q = 0; q < NC; ++q) t[q] += w8[v] * u8[v][q];
     }
@@ -920,33 +912,45 @@ static size_t spectral_ws(int64_t n, int k, int nc) {
 }
 
 struct EpochBufs {
-  float* gtail;     // n*k*dim, in in-edge order
-  float* wbase;     // ceil(n*k/64)*dim: Y + head update of the node crossing out of each wavefront
-  float* ybuf;      // n*dim, the other half of the double-buffered positions
-  int32_t* pos_in;  // n*k: in-edge position of each slot (-1: pruned); nullptr: slot order
-  int32_t* span;    // n: wavefront of the node's first in-edge entry if its entries cross a boundary, else -1
+  float* wbase;      // ceil(n*k/64)*dim: Y + head update of the node crossing out of each wavefront
+  float* ybuf;       // n*dim, the other half of the double-buffered positions
+  int32_t* span;     // n: wavefront of the node's first in-edge entry if its entries cross a boundary, else -1
+  int32_t* in_head;  // n*k (n_in used): head of each in-edge entry's slot
+  float* eps_in;     // n*k: the entry's epochs_per_sample
+  float* epn_in;     // n*k: the entry's epochs_per_next, advanced by phase 2
 };
 
-// Tail gradients in in-edge order while their buffer stays cache resident.
-constexpr size_t GTAIL_INEDGE_MAX = size_t(32) << 20;
-
 static size_t epochs_ws(int64_t n, int k, int dim) {
-  return align_up(static_cast<size_t>(n) * k * dim * 4, 256) +
-         align_up(static_cast<size_t>(ceil_div(n * k, 64)) * dim * 4, 256) +
-         align_up(static_cast<size_t>(n) * dim * 4, 256) + align_up(static_cast<size_t>(n) * k * 4, 256) +
-         align_up(static_cast<size_t>(n) * 4, 256) + 256;
+  return align_up(static_cast<size_t>(ceil_div(n * k, 64)) * dim * 4, 256) +
+         align_up(static_cast<size_t>(n) * dim * 4, 256) + align_up(static_cast<size_t>(n) * 4, 256) +
+         3 * align_up(static_cast<size_t>(n) * k * 4, 256) + 256;
 }
 
 static EpochBufs carve_epochs(void* ws, int64_t n, int k, int dim) {
   Carve c(ws);
   EpochBufs b;
-  b.gtail = c.take<float>(static_cast<size_t>(n) * k * dim);
   b.wbase = c.take<float>(static_cast<size_t>(ceil_div(n * k, 64)) * dim);
   b.ybuf = c.take<float>(static_cast<size_t>(n) * dim);
-  b.pos_in = c.take<int32_t>(static_cast<size_t>(n) * k);
   b.span = c.take<int32_t>(static_cast<size_t>(n));
-  if (static_cast<size_t>(n) * k * dim * 4 > GTAIL_INEDGE_MAX) b.pos_in = nullptr;
+  b.in_head = c.take<int32_t>(static_cast<size_t>(n) * k);
+  b.eps_in = c.take<float>(static_cast<size_t>(n) * k);
+  b.epn_in = c.take<float>(static_cast<size_t>(n) * k);
   return b;
+}
+
+// Phase 2's copy of the schedule, in entry order: the head, eps and the
+// current epn of each in-edge entry's slot (the caller's epn at the start of a
+// call, which phase 1 keeps advancing in slot order).
+__global__ __launch_bounds__(256) void k_in_prep(const int32_t* __restrict__ in_slots, int64_t n_in, int k,
+                                                 const float* __restrict__ eps, const float* __restrict__ epn,
+                                                 int32_t* __restrict__ in_head, float* __restrict__ eps_in,
+                                                 float* __restrict__ epn_in) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n_in) return;
+  const int64_t e = in_slots[t];
+  in_head[t] = static_cast<int32_t>(e / k);
+  eps_in[t] = eps[e];
+  epn_in[t] = epn[e];
 }
 
 template <int D, int G, int S>
@@ -955,19 +959,18 @@ static int launch_epoch(const float* Y, float* Ynext, const EpochBufs& B, const 
                         Law law, uint64_t seed, hipStream_t s) {
   const int64_t threads = n * G;
   const unsigned blocks = static_cast<unsigned>(ceil_div(threads, 256));
-  hipLaunchKernelGGL((k_epoch<D, G, S>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.gtail, B.pos_in, B.wbase, B.span,
-                     movable, tail, eps, epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed),
-                     static_cast<uint32_t>(seed >> 32));
+  hipLaunchKernelGGL((k_epoch<D, G, S>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.wbase, B.span, movable, tail, eps,
+                     epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
   MV_LAUNCHED("k_epoch");
   return MV_OK;
 }
 
-// One epoch from positions Y into Ynext (two launches: edge gradients, then
-// tail sums applied).
+// One epoch from positions Y into Ynext (two launches: head updates, then
+// tail updates).
 template <int D>
 static int run_epoch(const float* Y, float* Ynext, const EpochBufs& B, const int32_t* tail, const float* eps,
-                     float* epn, const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int64_t n, int k,
-                     int epoch, int neg_rate, Law law, uint64_t seed, const uint8_t* movable, hipStream_t s) {
+                     float* epn, const int32_t* in_node, int64_t n_in, int64_t n, int k, int epoch, int neg_rate,
+                     Law law, uint64_t seed, const uint8_t* movable, hipStream_t s) {
   int rc;
   // two slots per lane from k = 9 on (G lanes per row, G * S >= k)
   if (k <= 8) rc = launch_epoch<D, 8, 1>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
@@ -978,8 +981,8 @@ static int run_epoch(const float* Y, float* Ynext, const EpochBufs& B, const int
   else rc = launch_epoch<D, 64, 4>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
   if (rc) return rc;
   if (n_in > 0) {
-    hipLaunchKernelGGL((k_tail_apply<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, B.gtail,
-                       B.pos_in ? nullptr : in_slots, in_node, n_in, Ynext, B.wbase, movable);
+    hipLaunchKernelGGL((k_tail_apply<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, in_node,
+                       B.in_head, B.eps_in, B.epn_in, n_in, Y, Ynext, B.wbase, movable, epoch, law);
     MV_LAUNCHED("k_tail_apply");
   }
   return MV_OK;
@@ -1230,19 +1233,10 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
   MV_FAULT_POINT("launch");
   hipStream_t s = static_cast<hipStream_t>(stream);
   EpochBufs B = carve_epochs(ws, n, k, dim);
-  if (B.pos_in) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>(ceil_div(n * k, 256))), dim3(256), 0, s, B.pos_in, n * k,
-                       -1);
-    MV_LAUNCHED("k_fill_i32");
-    // in-edge order: k_epoch stores only active slots, k_tail_apply clears what it reads
-    hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(std::max<int64_t>(n_in, 1) * dim, 256))),
-                       dim3(256), 0, s, B.gtail, n_in * dim);
-    MV_LAUNCHED("k_zero_f32");
-  }
-  if (B.pos_in && n_in > 0) {
-    hipLaunchKernelGGL(k_pos_in, dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, in_slots, n_in,
-                       B.pos_in);
-    MV_LAUNCHED("k_pos_in");
+  if (n_in > 0) {
+    hipLaunchKernelGGL(k_in_prep, dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, in_slots, n_in, k,
+                       eps, epn, B.in_head, B.eps_in, B.epn_in);
+    MV_LAUNCHED("k_in_prep");
   }
   hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, B.span, n, -1);
   MV_LAUNCHED("k_fill_i32");
@@ -1261,14 +1255,14 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
     law.alpha = static_cast<float>(static_cast<double>(lr) * (1.0 - static_cast<double>(ep) / n_epochs));
     int rc = MV_OK;
     switch (dim) {
-      case 1: rc = run_epoch<1>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 2: rc = run_epoch<2>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 3: rc = run_epoch<3>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 4: rc = run_epoch<4>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 5: rc = run_epoch<5>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 6: rc = run_epoch<6>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      case 7: rc = run_epoch<7>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
-      default: rc = run_epoch<8>(cur, nxt, B, tail, eps, epn, in_slots, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 1: rc = run_epoch<1>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 2: rc = run_epoch<2>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 3: rc = run_epoch<3>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 4: rc = run_epoch<4>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 5: rc = run_epoch<5>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 6: rc = run_epoch<6>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      case 7: rc = run_epoch<7>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
+      default: rc = run_epoch<8>(cur, nxt, B, tail, eps, epn, in_node, n_in, n, k, ep, neg_rate, law, seed, movable, s); break;
     }
     if (rc != MV_OK) return rc;
     float* t = cur;
