@@ -98,12 +98,15 @@ __device__ __forceinline__ void gelu_bf16_n(const bf16_t* lut, const uint32_t (&
 #pragma unroll
   for (int k = 0; k < N; ++k) miss |= gelu_lut_off(pb[k]) >= GELU_OFF_BIG_END;
   if (__builtin_amdgcn_ballot_w64(miss) == 0) {
+    uint32_t t[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const uint32_t off = gelu_lut_off(pb[k]);
-      const uint32_t t = lut[((pb[k] >> 15) << 11) + min(off, 2047u)];
-      out[k] = off < 2048u ? t : gelu_big(pb[k]);
-    }
+    for (int k = 0; k < N; ++k) t[k] = lut[((pb[k] >> 15) << 11) + min(gelu_lut_off(pb[k]), 2047u)];
+    // The empty statements use every read before any select: without them
+    // the compiler sinks each read under its select, one branch per value.
+#pragma unroll
+    for (int k = 0; k < N; ++k) asm volatile("" : "+v"(t[k]));
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = gelu_lut_off(pb[k]) < 2048u ? t[k] : gelu_big(pb[k]);
   } else {
 #pragma unroll
     for (int k = 0; k < N; ++k) out[k] = gelu_bf16(lut, static_cast<bf16_t>(pb[k]));

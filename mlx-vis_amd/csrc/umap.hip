@@ -319,6 +319,11 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   float yi[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) yi[q] = i < n ? Y[i * D + q] : 0.f;
+  // the row owner's span entry and movable flag, loaded with the slots so
+  // that the final store does not wait for another memory round trip
+  const bool owner = c == 0 && i < n;
+  const int32_t sp = owner ? span[i] : -1;
+  const bool frozen = owner && movable && !movable[i];
 #pragma unroll
   for (int u = 0; u < S; ++u) {
     act[u] = in_row[u] && sch[u] > 0.f && nx[u] <= static_cast<float>(epoch);
@@ -424,9 +429,8 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   // the wavefront holding its first entry) starts at 0 instead, and Y + head
   // update goes to wbase[span[i]] for that wavefront to add with its sum; a
   // frozen node keeps Y.
-  if (c == 0 && i < n) {
-    const int32_t sp = span[i];
-    if (movable && !movable[i]) {
+  if (owner) {
+    if (frozen) {
 #pragma unroll
       for (int q = 0; q < D; ++q) Ynext[i * D + q] = yi[q];
     } else if (sp >= 0) {
@@ -470,6 +474,7 @@ __global__ __launch_bounds__(256) void k_tail_apply(const int32_t* __restrict__ 
   const int64_t w0 = t - l;  // first entry of this wavefront
   const bool valid = t < m;
   const int node = valid ? in_node[t] : -1;
+  const bool mov = !movable || (valid && movable[node]);   // loaded up front, like every other input
   // the node before the wavefront (lane 0) and after it (lane 63)
   const int before = (l == 0 && w0 > 0 && valid) ? in_node[w0 - 1] : -1;
   const int after = (l == 63 && t + 1 < m) ? in_node[t + 1] : -1;
@@ -509,7 +514,7 @@ __global__ __launch_bounds__(256) void k_tail_apply(const int32_t* __restrict__ 
   const int before_node = __shfl(before, 0);
   const int after_node = __shfl(after, 63);
   const bool last = valid && (l == 63 || next != node);
-  if (last && (!movable || movable[node])) {
+  if (last && mov) {
     const bool from_before = first_node == node && before_node == node;
     const bool to_after = l == 63 && after_node == node;
     if (to_after && !from_before) {
@@ -727,6 +732,27 @@ __global__ __launch_bounds__(256) void k_spec_wnorm(const float* __restrict__ ws
   wn[e] = eps[e] > 0.f ? wsym[e] * dis[i] * dis[tail[e]] : 0.f;
 }
 
+constexpr int NC_MAX = DIM_MAX + 1;   // spectral vectors (the first is dropped)
+constexpr int GRAM_STRIDE = 48;       // >= NC_MAX (NC_MAX + 1) / 2 packed entries
+// Each Gram buffer holds GRAM_COPIES partial sums: block b adds into copy
+// b mod GRAM_COPIES, so that few atomics meet on one address; the reader adds
+// the copies in copy order.
+constexpr int GRAM_COPIES = 32;
+
+// s_g[p] = sum over the copies of the packed Gram entry p (threads p < NP);
+// followed by a barrier.
+template <int NC>
+__device__ __forceinline__ void gram_gather(const double* __restrict__ g, double* s_g) {
+  constexpr int NP = NC * (NC + 1) / 2;
+  if (g && threadIdx.x < NP) {
+    double v = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < GRAM_COPIES; ++c) v += g[c * GRAM_STRIDE + threadIdx.x];
+    s_g[threadIdx.x] = v;
+  }
+  __syncthreads();
+}
+
 // R^-1 of the Cholesky factor (G = R^T R, R upper, pivots sqrt(. + 1e-10) as
 // the reference's norms, umap.py:335) of a packed Gram matrix; identity if g
 // is null.
@@ -773,8 +799,12 @@ __device__ void chol_inverse(const double* __restrict__ g, float* __restrict__ o
 
 // One power-iteration step: Unext = (W_norm U) R^-1(gram_prev), accumulating
 // the Gram matrix of Unext into gram_cur (one atomic per block and entry) and
-// clearing gram_zero for the next step.
-template <int NC>
+// clearing gram_zero for the next step. Row i is served by G lanes holding S
+// adjacent slots each (G S >= k), so that the weights and tail indices of a
+// wavefront's rows load as contiguous runs and all of a lane's gathers are in
+// flight at once; the G partial sums are added by a butterfly over the row's
+// lanes.
+template <int NC, int G, int S>
 __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, const double* __restrict__ gram_prev,
                                                    const float* __restrict__ wn, const int32_t* __restrict__ tail,
                                                    int64_t n, int k, float* __restrict__ Unext,
@@ -782,46 +812,58 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
   constexpr int NP = NC * (NC + 1) / 2;
   __shared__ float s_R[NC * NC];
   __shared__ double s_part[4][NP];
-  if (threadIdx.x == 0) chol_inverse<NC>(gram_prev, s_R);
-  if (blockIdx.x == 0 && threadIdx.x < NP) gram_zero[threadIdx.x] = 0.0;
-  __syncthreads();
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  __shared__ double s_g[NP];
+  if (blockIdx.x == 0)
+    for (int z = threadIdx.x; z < GRAM_COPIES * GRAM_STRIDE; z += blockDim.x) gram_zero[z] = 0.0;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = gid / G;
+  const int c = static_cast<int>(gid % G);
   float t[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) t[q] = 0.f;
   if (i < n) {
-    // no branch on w: a pruned edge (w = 0, its tail still a valid index) adds
-    // an exact zero, and a batch of gathers of the row is in flight at once
-    const int64_t e0 = i * k, elast = e0 + k - 1;
-    constexpr int NB = NC <= 5 ? 8 : 4;   // batch (wide vectors: fewer, to stay in registers)
-    for (int c = 0; c < k; c += NB) {
-      float w8[NB], u8[NB][NC];
+    float w[S];
+    int64_t j[S];
 #pragma unroll
-      for (int v = 0; v < NB; ++v) {
-        // past the row: the row's last slot again, with weight 0 (the loads
-        // stay unconditional, so that all of them are in flight together)
-        const int64_t ev = e0 + c + v < elast ? e0 + c + v : elast;
-        const float keep = c + v < k ? 1.f : 0.f;
-        w8[v] = wn[ev] * keep;
-        const int64_t j = tail[ev];
-#pragma unroll
-        for (int q = 0; q < NC; ++q) u8[v][q] = U[j * NC + q];
+    for (int v = 0; v < S; ++v) {
+      const int cs = c * S + v;
+      w[v] = 0.f;
+      j[v] = 0;
+      if (cs < k) {
+        w[v] = wn[i * k + cs];
+        j[v] = tail[i * k + cs];
       }
-#pragma unroll
-      for (int v = 0; v < NB; ++v)
-#pragma unroll
-        for (int q = 0; q < NC; ++q) t[q] += w8[v] * u8[v][q];
     }
+    float u8[S][NC];
+#pragma unroll
+    for (int v = 0; v < S; ++v)
+#pragma unroll
+      for (int q = 0; q < NC; ++q) u8[v][q] = U[j[v] * NC + q];   // slots past the row: node 0, weight 0
+#pragma unroll
+    for (int v = 0; v < S; ++v)
+#pragma unroll
+      for (int q = 0; q < NC; ++q) t[q] += w[v] * u8[v][q];
   }
+#pragma unroll
+  for (int q = 0; q < NC; ++q)
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) t[q] += __shfl_xor(t[q], m);
+  // R^-1 after the product: the Cholesky factorisation of the previous Gram
+  // matrix (a dependent load and serial FP64 work) overlaps the gathers of
+  // the other wavefronts instead of delaying them all.
+  gram_gather<NC>(gram_prev, s_g);
+  if (threadIdx.x == 0) chol_inverse<NC>(gram_prev ? s_g : nullptr, s_R);
+  __syncthreads();
+  const bool owner = c == 0 && i < n;
   float u[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     float acc = 0.f;
 #pragma unroll
     for (int p = 0; p <= q; ++p) acc += t[p] * s_R[p * NC + q];
-    u[q] = acc;
+    u[q] = owner ? acc : 0.f;
   }
-  if (i < n) {
+  if (owner) {
 #pragma unroll
     for (int q = 0; q < NC; ++q) Unext[i * NC + q] = u[q];
   }
@@ -833,7 +875,7 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
     for (int q = p; q < NC; ++q) {
       double v = static_cast<double>(u[p]) * static_cast<double>(u[q]);
 #pragma unroll
-      for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+      for (int m = 32; m >= G; m >>= 1) v += __shfl_xor(v, m);   // only the row owners (lanes c = 0) hold terms
       if (l == 0) s_part[w][idx] = v;
       ++idx;
     }
@@ -842,7 +884,7 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
   if (threadIdx.x < NP) {
     const double v = s_part[0][threadIdx.x] + s_part[1][threadIdx.x] + s_part[2][threadIdx.x] +
                      s_part[3][threadIdx.x];
-    atomicAdd(&gram_cur[threadIdx.x], v);
+    atomicAdd(&gram_cur[(blockIdx.x % GRAM_COPIES) * GRAM_STRIDE + threadIdx.x], v);
   }
 }
 
@@ -850,8 +892,11 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
 template <int NC>
 __global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U, const double* __restrict__ gram,
                                                     int64_t n, float* __restrict__ V) {
+  constexpr int NP = NC * (NC + 1) / 2;
   __shared__ float s_R[NC * NC];
-  if (threadIdx.x == 0) chol_inverse<NC>(gram, s_R);
+  __shared__ double s_g[NP];
+  gram_gather<NC>(gram, s_g);
+  if (threadIdx.x == 0) chol_inverse<NC>(gram ? s_g : nullptr, s_R);
   __syncthreads();
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -864,8 +909,6 @@ __global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U,
   }
 }
 
-constexpr int NC_MAX = DIM_MAX + 1;   // spectral vectors (the first is dropped)
-constexpr int GRAM_STRIDE = 48;       // >= NC_MAX (NC_MAX + 1) / 2 packed entries
 
 template <int NC>
 static int spectral_run(const float* wsym, const float* eps, const int32_t* tail, int64_t n, int k, int iters,
@@ -875,15 +918,16 @@ static int spectral_run(const float* wsym, const float* eps, const int32_t* tail
   float* wn = cv.take<float>(static_cast<size_t>(n) * k);
   float* U0 = cv.take<float>(static_cast<size_t>(n) * NC);
   float* U1 = cv.take<float>(static_cast<size_t>(n) * NC);
-  double* gram = cv.take<double>(3 * GRAM_STRIDE);   // three rotating packed Gram buffers
+  double* gram = cv.take<double>(3 * GRAM_COPIES * GRAM_STRIDE);   // three rotating packed Gram buffers
   const unsigned rb = static_cast<unsigned>(ceil_div(n, 256));
   hipLaunchKernelGGL(k_spec_prep, dim3(rb), dim3(256), 0, s, wsym, eps, n, k, dis);
   MV_LAUNCHED("k_spec_prep");
   hipLaunchKernelGGL(k_spec_wnorm, dim3(static_cast<unsigned>(ceil_div(n * k, 256))), dim3(256), 0, s, wsym, eps,
                      tail, dis, n, k, wn);
   MV_LAUNCHED("k_spec_wnorm");
-  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, reinterpret_cast<float*>(gram),
-                     static_cast<int64_t>(2 * 3 * GRAM_STRIDE));
+  constexpr int64_t gram_floats = 2 * 3 * GRAM_COPIES * GRAM_STRIDE;   // the three Gram buffers, as floats
+  hipLaunchKernelGGL(k_zero_f32, dim3(static_cast<unsigned>(ceil_div(gram_floats, 256))), dim3(256), 0, s,
+                     reinterpret_cast<float*>(gram), gram_floats);
   MV_LAUNCHED("k_zero_f32");
   hipLaunchKernelGGL(k_copy_f32, dim3(static_cast<unsigned>(ceil_div(n * NC, 256))), dim3(256), 0, s, U0, V,
                      n * NC);
@@ -892,15 +936,26 @@ static int spectral_run(const float* wsym, const float* eps, const int32_t* tail
   float* cur = U0;
   float* nxt = U1;
   for (int it = 0; it < iters; ++it) {
-    const double* prev = it == 0 ? nullptr : gram + GRAM_STRIDE * ((it + 2) % 3);
-    hipLaunchKernelGGL((k_spec_iter<NC>), dim3(rb), dim3(256), 0, s, cur, prev, wn, tail, n, k, nxt,
-                       gram + GRAM_STRIDE * (it % 3), gram + GRAM_STRIDE * ((it + 1) % 3));
+    constexpr int GB = GRAM_COPIES * GRAM_STRIDE;   // one Gram buffer
+    const double* prev = it == 0 ? nullptr : gram + GB * ((it + 2) % 3);
+    double* gcur = gram + GB * (it % 3);
+    double* gzero = gram + GB * ((it + 1) % 3);
+    if (k <= 16)
+      hipLaunchKernelGGL((k_spec_iter<NC, 4, 4>), dim3(static_cast<unsigned>(ceil_div(n * 4, 256))), dim3(256), 0, s,
+                         cur, prev, wn, tail, n, k, nxt, gcur, gzero);
+    else if (k <= 64)
+      hipLaunchKernelGGL((k_spec_iter<NC, 16, 4>), dim3(static_cast<unsigned>(ceil_div(n * 16, 256))), dim3(256), 0,
+                         s, cur, prev, wn, tail, n, k, nxt, gcur, gzero);
+    else
+      hipLaunchKernelGGL((k_spec_iter<NC, 64, 4>), dim3(static_cast<unsigned>(ceil_div(n * 64, 256))), dim3(256), 0,
+                         s, cur, prev, wn, tail, n, k, nxt, gcur, gzero);
     MV_LAUNCHED("k_spec_iter");
     float* t = cur;
     cur = nxt;
     nxt = t;
   }
-  hipLaunchKernelGGL((k_spec_apply<NC>), dim3(rb), dim3(256), 0, s, cur, gram + GRAM_STRIDE * ((iters - 1) % 3), n,
+  hipLaunchKernelGGL((k_spec_apply<NC>), dim3(rb), dim3(256), 0, s, cur,
+                     gram + GRAM_COPIES * GRAM_STRIDE * ((iters - 1) % 3), n,
                      V);
   MV_LAUNCHED("k_spec_apply");
   return MV_OK;
@@ -908,7 +963,7 @@ static int spectral_run(const float* wsym, const float* eps, const int32_t* tail
 
 static size_t spectral_ws(int64_t n, int k, int nc) {
   return align_up(n * 4, 256) + align_up(static_cast<size_t>(n) * k * 4, 256) +
-         2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + align_up(3 * GRAM_STRIDE * 8, 256) + 1024;
+         2 * align_up(static_cast<size_t>(n) * nc * 4, 256) + align_up(3 * GRAM_COPIES * GRAM_STRIDE * 8, 256) + 1024;
 }
 
 struct EpochBufs {
