@@ -29,8 +29,11 @@ libraries independent of this repository's oracle and kernels:
 
 - umap_c0_oracle.json: the oracle's layout of the 10,000 x 128 workload
   (make_umap_c0): kept edges, 15-NN preservation, trustworthiness.
+- umap_c1_oracle.json: the oracle's layouts of the 70,000 x 784 mixture
+  (make_umap_c1, 500 epochs, random and spectral init): kept edges, 15-NN
+  preservation and trustworthiness on the 2,000 rows of knn_c1_sample2000.npz.
 
-Usage: ``python tests/golden/make_golden.py [knn_c0 ab knn_c1 nnd_c1 nnd_c3 umap_c0]``
+Usage: ``python tests/golden/make_golden.py [knn_c0 ab knn_c1 nnd_c1 nnd_c3 umap_c0 umap_c1]``
 (no argument: the fast fixtures knn_c0 and ab).
 """
 
@@ -179,8 +182,72 @@ def make_umap_c0():
     (HERE / "umap_c0_oracle.json").write_text(json.dumps(out, indent=1) + "\n")
 
 
+def make_umap_c1():
+    """The oracle's layouts of the 70,000 x 784 mixture (seed 1): exact kNN
+    (k = 15), fuzzy set pruned for 500 epochs, a/b for (spread 1, min_dist
+    0.1), then 500 epochs of the reference's loop (umap.py:423-447: the
+    schedule of umap.schedule, 5 NumPy-drawn negatives per active edge, seed
+    0) with the torch-CPU scatters of umap.sgd_step_index_add (the same FP32
+    arithmetic as umap.sgd_step, pinned to it by test_oracle_umap), from two
+    inits: "random", RandomState(0).uniform(-10, 10, (n, 2)) (what
+    UMAP(init="random", random_state=0) draws), and "spectral", the reference's
+    _spectral_init (umap.py:298-357) restated by umap.spectral_power (100
+    power iterations from default_rng(0) normal start vectors) and
+    umap.spectral_scale (noise RandomState(random_state or 42), i.e. 42).
+    Records the kept-edge count, and per init 15-NN preservation and
+    trustworthiness (k = 15) over the 2,000 rows of knn_c1_sample2000.npz."""
+    import torch
+
+    from mlx_vis_amd.datasets import gaussian_mixture_784
+    from oracle import knn as oknn
+    from oracle import metrics as om
+    from oracle import umap as ou
+
+    X = gaussian_mixture_784(70_000, seed=1)
+    n, n_epochs, neg_rate = X.shape[0], 500, 5
+    t = time.time()
+    idx, d2 = oknn.exact_knn(X, X, 15, self_base=0)
+    print(f"umap_c1: oracle exact kNN {time.time() - t:.1f} s", flush=True)
+    dist = np.sqrt(d2).astype(np.float32)
+    rows, cols, vals = ou.fuzzy_simplicial_set(idx.astype(np.int32), dist, n_epochs)
+    a, b = ou.find_ab_params(1.0, 0.1)
+    sample = c1_sample_rows()
+    out = {
+        "workload": "70,000 x 784 10-Gaussian mixture seed 1 (min-max scaled), k=15, 500 epochs, 5 negatives",
+        "n_edges": int(rows.size),
+        "rows": "knn_c1_sample2000.npz",
+    }
+    V = ou.spectral_power(rows, cols, vals.astype(np.float64), n, np.random.default_rng(0).standard_normal((n, 3)))
+    inits = {
+        "random": np.random.RandomState(0).uniform(-10.0, 10.0, size=(n, 2)).astype(np.float32),
+        "spectral": ou.spectral_scale(V, np.random.RandomState(42).standard_normal((n, 2))),
+    }
+    for name, Y0 in inits.items():
+        Y = torch.from_numpy(np.ascontiguousarray(Y0, dtype=np.float32))
+        rng = np.random.default_rng(0)
+        sets = ou.schedule(vals, n_epochs)
+        t = time.time()
+        for epoch in range(n_epochs):
+            act = sets[epoch]
+            if act.size == 0:
+                continue
+            ef, et = rows[act].astype(np.int64), cols[act].astype(np.int64)
+            A = act.size
+            nf = ef[np.arange(neg_rate * A) % A]
+            nt = rng.integers(0, n, size=neg_rate * A)
+            alpha = 1.0 * (1.0 - epoch / n_epochs)
+            Y = ou.sgd_step_index_add(Y, *(torch.from_numpy(np.ascontiguousarray(v)) for v in (ef, et, nf, nt)),
+                                      np.float32(alpha), np.float32(a), np.float32(b))
+        print(f"umap_c1: oracle layout ({name} init) {time.time() - t:.1f} s", flush=True)
+        Y = Y.numpy()
+        out[name] = {"knn15_preservation": om.knn_preservation(X, Y, 15, rows=sample),
+                     "trustworthiness_15": om.trustworthiness(X, Y, 15, rows=sample)}
+        print("umap_c1:", name, out[name], flush=True)
+    (HERE / "umap_c1_oracle.json").write_text(json.dumps(out, indent=1) + "\n")
+
+
 MAKERS = {"knn_c0": make_knn, "ab": make_ab, "knn_c1": make_knn_c1, "nnd_c1": make_nnd_c1, "nnd_c3": make_nnd_c3,
-          "umap_c0": make_umap_c0}
+          "umap_c0": make_umap_c0, "umap_c1": make_umap_c1}
 
 
 if __name__ == "__main__":

@@ -396,6 +396,40 @@ def test_c0_end_to_end_layout_quality_matches_oracle():
     assert abs(t - fx["trustworthiness_15"]) < 0.01
 
 
+@pytest.mark.parametrize("init", ["random", "spectral"])
+def test_c1_end_to_end_layout_quality_matches_oracle(init):
+    """The full-shape workload through the public API: UMAP(n_epochs=500,
+    knn_method="brute", random_state=0) with random or spectral (the API
+    default) init on the 70,000 x 784 mixture (seed 1), against the oracle's
+    layouts of the same graph, a/b and inits (tests/golden/umap_c1_oracle.json,
+    written by make_golden.py umap_c1): the kept-edge count up to threshold
+    rounding, and 15-NN preservation and trustworthiness over the 2,000 rows of
+    knn_c1_sample2000.npz within 0.01 (negatives and spectral start vectors
+    differ: Philox here, NumPy there)."""
+    import json
+    from pathlib import Path
+
+    from mlx_vis_amd import UMAP
+    from mlx_vis_amd.datasets import gaussian_mixture_784
+
+    golden = Path(__file__).resolve().parent / "golden"
+    fx = json.loads((golden / "umap_c1_oracle.json").read_text())
+    sample = np.load(golden / "knn_c1_sample2000.npz")["rows"].astype(np.int64)
+    X = gaussian_mixture_784(70_000, seed=1)
+    um = UMAP(n_neighbors=15, min_dist=0.1, n_epochs=500, negative_sample_rate=5, init=init,
+              knn_method="brute", random_state=0)
+    Y = um.fit_transform(X)
+    assert Y.shape == (70_000, 2) and Y.dtype == np.float32 and np.isfinite(Y).all()
+    assert abs(um.graph_.n_edges - fx["n_edges"]) <= max(2, fx["n_edges"] // 10000)
+    p = om.knn_preservation(X, Y, 15, rows=sample)
+    t = om.trustworthiness(X, Y, 15, rows=sample)
+    want = fx[init]
+    print(f"{init}: preservation gpu={p:.4f} oracle={want['knn15_preservation']:.4f}; "
+          f"trust gpu={t:.4f} oracle={want['trustworthiness_15']:.4f}")
+    assert abs(p - want["knn15_preservation"]) < 0.01
+    assert abs(t - want["trustworthiness_15"]) < 0.01
+
+
 def test_umap_normalize_minmax_is_applied_before_the_knn():
     """UMAP(normalize="minmax") gives the embedding of the min-max scaled input
     (mlx_vis/_normalize.py:28-31 applied in fit_transform, umap.py:86-88)."""
