@@ -26,10 +26,12 @@
 //     negative_sample_rate repulsive updates of the head only, each against a
 //     uniformly drawn node (umap.py:433-436, 384-386).
 // Negatives come from Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed
-// by the seed and counted by (epoch, edge slot), and the in-edge lists are in
-// slot order, so a run repeats bit for bit as long as no node has more than
-// 64 in-edges (a larger in-degree spans three or more wavefronts, whose
-// partial sums reach the node by atomics in arrival order).
+// by the seed and counted by (epoch, edge slot), in-edge lists (up to 16,384
+// entries per node) are in slot order, and every tail sum is formed in an
+// order that does not depend on scheduling (in-wavefront scans; two terms
+// onto zero; fixed-point totals for nodes spanning three or more chunks of
+// the tail sums), so a run repeats bit for bit (unless a node has over 16,384
+// in-edges).
 #include <math.h>
 
 #include <algorithm>
@@ -425,10 +427,10 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
     for (int m = G / 2; m >= 1; m >>= 1) acc[q] += __shfl_xor(acc[q], m);
   }
   // Row owner: the next positions start as Y + head update. A node whose
-  // in-edge entries cross a wavefront boundary of phase 2 (span[i] = index of
-  // the wavefront holding its first entry) starts at 0 instead, and Y + head
-  // update goes to wbase[span[i]] for that wavefront to add with its sum; a
-  // frozen node keeps Y.
+  // in-edge entries cross a chunk boundary of phase 2 (span[i] = index of the
+  // chunk holding its first entry) starts at 0 instead, and Y + head update
+  // goes to wbase[span[i]] for that chunk to add with its sum; a frozen node
+  // keeps Y.
   if (owner) {
     if (frozen) {
 #pragma unroll
@@ -453,89 +455,177 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
 // epn in phase 1, so both phases agree on the active edges of every epoch. An
 // active entry computes minus the attractive update that phase 1 applied to
 // its head, from the same two positions with the same expression, so the
-// value is the same to the bit. A wavefront-segmented inclusive scan adds
-// each node's entries, and the last lane of a segment adds the sum to Ynext
-// with one atomic, onto the value phase 1 left there: Y + head update for a
-// node whose entries lie in one wavefront (one term onto a fixed value), 0
-// for a node crossing a wavefront boundary, whose first segment adds wbase[w]
-// (its Y + head update, indexed by the wavefront) with its sum. Over two
-// wavefronts the result does not depend on the order of the two atomics (two
-// terms onto zero commute); only a node with more than 64 in-edges can span
-// more. Frozen nodes (movable[j] == 0) are skipped.
+// value is the same to the bit. Each wavefront takes a chunk of TAIL_CHUNK
+// (128) entries, two per lane (the whole grid is then resident at once); a
+// segmented inclusive scan adds each node's entries (the segment running
+// from the first half of the chunk into the second carries its sum across),
+// and the last lane of a segment adds the sum to Ynext with one atomic, onto
+// the value phase 1 left there: Y + head update for a node whose entries lie
+// in one chunk (one term onto a fixed value), 0 for a node crossing a chunk
+// boundary, whose first segment adds wbase[c] (its Y + head update, indexed
+// by the chunk) with its sum. Over two chunks the result does not depend on
+// the order of the two atomics (two terms onto zero commute). A node whose
+// entries span three or more chunks (a hub: more than 128 in-edges) would
+// depend on their order, so there each chunk rounds its segment sum (the
+// first one with the base) to 64-bit fixed point (units of 2^-32; integer
+// additions are exact and commute) and adds it plus a counting bias to the
+// node's accumulator with one atomic; the value it returns tells the chunk
+// that completed the count, which stores the total as the node's next
+// position and clears the accumulator for the next epoch. With the in-edge
+// lists in slot order every segment sum is the same on every run, so a run
+// repeats bit for bit (a node spanning more than 1,022 chunks, over 130,000
+// in-edges, falls back to float atomics in arrival order). Frozen nodes
+// (movable[j] == 0) are skipped.
+constexpr double FIX_ONE = 4294967296.0;   // fixed-point unit of the hub tail sums: 2^-32
+constexpr unsigned long long FIX_BIAS = 1ull << 54;   // one chunk counted into a hub accumulator
+constexpr int HUB_MAX_WAVES = 1023;                   // chunks a fixed-point hub sum can count
+constexpr int TAIL_CHUNK = 128;                       // in-edge entries per wavefront of k_tail_apply (two per lane)
+
 template <int D>
 __global__ __launch_bounds__(256) void k_tail_apply(const int32_t* __restrict__ in_node,
                                                     const int32_t* __restrict__ in_head,
                                                     const float* __restrict__ eps_in, float* __restrict__ epn_in,
                                                     int64_t m, const float* __restrict__ Y, float* __restrict__ Ynext,
                                                     const float* __restrict__ wbase, const uint8_t* __restrict__ movable,
-                                                    int epoch, Law law) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+                                                    const int32_t* __restrict__ span_w,
+                                                    unsigned long long* __restrict__ hub_acc, int epoch, Law law) {
+  const int64_t wv = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;   // chunk of this wavefront
   const int l = threadIdx.x & 63;
-  const int64_t w0 = t - l;  // first entry of this wavefront
-  const bool valid = t < m;
-  const int node = valid ? in_node[t] : -1;
-  const bool mov = !movable || (valid && movable[node]);   // loaded up front, like every other input
-  // the node before the wavefront (lane 0) and after it (lane 63)
-  const int before = (l == 0 && w0 > 0 && valid) ? in_node[w0 - 1] : -1;
-  const int after = (l == 63 && t + 1 < m) ? in_node[t + 1] : -1;
+  const int64_t c0 = wv * TAIL_CHUNK;   // first entry of the chunk
+  // Every load of the wavefront is independent of the others: issued together.
+  int node[2], crossings[2];
+  bool valid[2], mov[2];
+  float es[2], ne[2];
+  int64_t hd[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t t = c0 + h * 64 + l;
+    valid[h] = t < m;
+    node[h] = valid[h] ? in_node[t] : -1 - h;   // distinct out-of-range keys end the segments
+    mov[h] = !movable || (valid[h] && movable[node[h]]);
+    crossings[h] = valid[h] ? span_w[node[h]] : 0;   // chunk boundaries inside the node's entries
+    es[h] = valid[h] ? eps_in[t] : -1.f;
+    ne[h] = valid[h] ? epn_in[t] : INFINITY;
+    hd[h] = valid[h] ? in_head[t] : 0;
+  }
+  // the node before the chunk (lane 0) and after it (lane 63)
+  const int before = (l == 0 && c0 > 0 && c0 < m) ? in_node[c0 - 1] : -1;
+  const int after = (l == 63 && c0 + TAIL_CHUNK < m) ? in_node[c0 + TAIL_CHUNK] : -1;
   float base[D];
 #pragma unroll
-  for (int q = 0; q < D; ++q) base[q] = l == 63 && valid ? wbase[(w0 >> 6) * D + q] : 0.f;
-  const float es = valid ? eps_in[t] : -1.f;
-  const float ne = valid ? epn_in[t] : INFINITY;
-  const int64_t hd = valid ? in_head[t] : 0;
-  float v[D];
+  for (int q = 0; q < D; ++q) base[q] = l == 63 && valid[1] ? wbase[wv * D + q] : 0.f;
+  float v[2][D];
 #pragma unroll
-  for (int q = 0; q < D; ++q) v[q] = 0.f;
-  if (es > 0.f && ne <= static_cast<float>(epoch)) {
-    epn_in[t] = ne + es;
-    float diff[D];
-    float d2 = 0.f;
+  for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int q = 0; q < D; ++q) {
-      diff[q] = Y[hd * D + q] - Y[static_cast<int64_t>(node) * D + q];
-      d2 += diff[q] * diff[q];
-    }
-    const float ca = law.attract(d2);
+    for (int q = 0; q < D; ++q) v[h][q] = 0.f;
+    if (es[h] > 0.f && ne[h] <= static_cast<float>(epoch)) {
+      epn_in[c0 + h * 64 + l] = ne[h] + es[h];
+      float diff[D];
+      float d2 = 0.f;
 #pragma unroll
-    for (int q = 0; q < D; ++q) v[q] = -__fmul_rn(clip4(ca * diff[q]), law.alpha);
-  }
+      for (int q = 0; q < D; ++q) {
+        diff[q] = Y[hd[h] * D + q] - Y[static_cast<int64_t>(node[h]) * D + q];
+        d2 += diff[q] * diff[q];
+      }
+      const float ca = law.attract(d2);
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int on = __shfl_up(node, off);
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-      const float ov = __shfl_up(v[q], off);
-      if (l >= off && on == node) v[q] += ov;
+      for (int q = 0; q < D; ++q) v[h][q] = -__fmul_rn(clip4(ca * diff[q]), law.alpha);
     }
   }
-  const int next = __shfl_down(node, 1);
-  const int first_node = __shfl(node, 0);
+  // Segmented inclusive scans of the two halves; the segment running from
+  // the end of the first half into the second carries its sum into lane 0 of
+  // the second half before that half is scanned.
+  const int first_half1 = __shfl(node[1], 0);
+  const int last_half0 = __shfl(node[0], 63);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h == 1) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        const float carry = __shfl(v[0][q], 63);
+        if (l == 0 && first_half1 == last_half0) v[1][q] += carry;
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int on = __shfl_up(node[h], off);
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        const float ov = __shfl_up(v[h][q], off);
+        if (l >= off && on == node[h]) v[h][q] += ov;
+      }
+    }
+  }
+  const int first_node = __shfl(node[0], 0);
   const int before_node = __shfl(before, 0);
   const int after_node = __shfl(after, 63);
-  const bool last = valid && (l == 63 || next != node);
-  if (last && mov) {
-    const bool from_before = first_node == node && before_node == node;
-    const bool to_after = l == 63 && after_node == node;
-    if (to_after && !from_before) {
-      // first segment of a node crossing into the next wavefront
+  const int down0 = __shfl_down(node[0], 1), down1 = __shfl_down(node[1], 1);   // every lane takes part
 #pragma unroll
-      for (int q = 0; q < D; ++q) v[q] += base[q];
+  for (int h = 0; h < 2; ++h) {
+    // next entry's node: the lane above, across the halves, or (lane 63 of
+    // the second half) the chunk's end, where a continuing node counts as
+    // crossing into the next chunk
+    const int nxt = l < 63 ? (h == 0 ? down0 : down1) : (h == 0 ? first_half1 : -3);
+    const bool last = valid[h] && nxt != node[h];
+    if (last && mov[h]) {
+    const bool from_before = first_node == node[h] && before_node == node[h];
+    const bool to_after = h == 1 && l == 63 && after_node == node[h];
+    const bool first_seg = to_after && !from_before;   // first segment of a node crossing into the next chunk
+    const bool hub = crossings[h] >= 2 && crossings[h] < HUB_MAX_WAVES;
+    const int64_t nd = static_cast<int64_t>(node[h]) * D;
+    if (!hub) {
+      if (first_seg) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) v[h][q] += base[q];
+      }
+#pragma unroll
+      for (int q = 0; q < D; ++q) atomicAdd(&Ynext[nd + q], v[h][q]);
+    } else {
+      // hub: the fixed-point total of all its chunks, stored by the last
+      // one. Each chunk adds its partial plus FIX_BIAS, so the bits from 2^54
+      // up count the chunks in (the partials stay below 2^53 in magnitude);
+      // the returned value tells each one whether it completed the count and,
+      // if so, the total, so one round trip suffices and no other location
+      // has to be read.
+      long long fv[D];
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        fv[q] = __double2ll_rn(static_cast<double>(v[h][q]) * FIX_ONE);
+        if (first_seg) fv[q] += __double2ll_rn(static_cast<double>(base[q]) * FIX_ONE);
+      }
+      unsigned long long* acc = hub_acc + nd;
+      const unsigned long long W = static_cast<unsigned long long>(crossings[h] + 1);
+      unsigned long long nw[D];
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        const unsigned long long add = static_cast<unsigned long long>(fv[q]) + FIX_BIAS;
+        nw[q] = atomicAdd(&acc[q], add) + add;
+      }
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        if ((((nw[q] + FIX_BIAS / 2) >> 54) & 1023ull) == W) {
+          const long long tot = static_cast<long long>(nw[q] - W * FIX_BIAS);
+          Ynext[nd + q] = static_cast<float>(static_cast<double>(tot) / FIX_ONE);
+          atomicExch(&acc[q], 0ull);   // ready for the next epoch
+        }
+      }
     }
-#pragma unroll
-    for (int q = 0; q < D; ++q) atomicAdd(&Ynext[static_cast<int64_t>(node) * D + q], v[q]);
+    }
   }
 }
 
-// span[j] = index of the wavefront (64 in-edge entries) holding node j's first
-// entry if its entries cross a 64-aligned boundary, else -1 (span pre-filled
+// span[j] = index of the chunk (TAIL_CHUNK in-edge entries) holding node j's
+// first entry if its entries cross a chunk boundary, else -1 (span pre-filled
 // with -1, i.e. 0xFFFFFFFF as unsigned, so the unsigned minimum keeps the
-// first boundary's wavefront).
+// first boundary's chunk); span_w[j] counts the boundaries crossed.
 __global__ __launch_bounds__(256) void k_span(const int32_t* __restrict__ in_node, int64_t m,
-                                              int32_t* __restrict__ span) {
-  const int64_t t = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x + 1) * 64;
-  if (t < m && in_node[t - 1] == in_node[t])
-    atomicMin(reinterpret_cast<uint32_t*>(&span[in_node[t]]), static_cast<uint32_t>((t >> 6) - 1));
+                                              int32_t* __restrict__ span, int32_t* __restrict__ span_w) {
+  const int64_t t = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x + 1) * TAIL_CHUNK;
+  if (t < m && in_node[t - 1] == in_node[t]) {
+    atomicMin(reinterpret_cast<uint32_t*>(&span[in_node[t]]), static_cast<uint32_t>(t / TAIL_CHUNK - 1));
+    atomicAdd(&span_w[in_node[t]], 1);
+  }
 }
 
 
@@ -545,8 +635,8 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, int64
 }
 
 // In-edge lists of the kept slots: counts per tail, exclusive scan, atomic
-// fill, then k_in_sort puts the entries of each node with at most 64 of them
-// in slot order.
+// fill, then k_in_sort (up to 128 entries) and k_in_sort_big (up to 16,384)
+// put the entries of each node in slot order.
 __global__ __launch_bounds__(256) void k_in_count(const int32_t* __restrict__ tail, const float* __restrict__ eps,
                                                   int64_t m, uint32_t* __restrict__ cnt) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -626,23 +716,81 @@ __global__ __launch_bounds__(256) void k_scan_add_u32(uint32_t* __restrict__ out
 
 // Orders each node's in-edge segment [off[j], off[j] + cnt[j]) by slot
 // (k_in_fill places them in atomic order), so the summation order of a tail
-// gradient sum is the same on every run. One wavefront per node: lane t holds
-// entry t, its rank is the number of smaller slots (slots are distinct), and
-// it is stored at its rank. Only segments of at most 64 entries are ordered:
-// a node with more in-edges spans three or more wavefronts of the tail sums,
-// whose partial sums reach it by atomics in arrival order anyway.
+// sum is the same on every run. One wavefront per node: lane t holds entries
+// t and t + 64, the rank of an entry is the number of smaller slots (slots are
+// distinct), and it is stored at its rank. Segments of up to 128 entries are
+// ordered here, longer ones by k_in_sort_big.
 __global__ __launch_bounds__(256) void k_in_sort(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
                                                  int64_t n, int32_t* __restrict__ in_slots) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   if (j >= n) return;
   const int c = static_cast<int>(cnt[j]);
-  if (c < 2 || c > 64) return;
+  if (c < 2 || c > 128) return;
   int32_t* seg = in_slots + off[j];
-  const int32_t v = l < c ? seg[l] : INT32_MAX;
-  int rank = 0;
-  for (int u = 0; u < c; ++u) rank += __shfl(v, u) < v ? 1 : 0;
-  if (l < c) seg[rank] = v;
+  const int32_t v0 = l < c ? seg[l] : INT32_MAX;
+  const int32_t v1 = l + 64 < c ? seg[l + 64] : INT32_MAX;
+  int r0 = 0, r1 = 0;
+  for (int u = 0; u < 64; ++u) {
+    const int32_t a = __shfl(v0, u), b = __shfl(v1, u);
+    r0 += (a < v0) + (b < v0);
+    r1 += (a < v1) + (b < v1);
+  }
+  if (l < c) seg[r0] = v0;
+  if (l + 64 < c) seg[r1] = v1;
+}
+
+// Segments of 129 to IN_SORT_BIG entries: a bitonic sort of the slots in LDS
+// (padded with INT32_MAX to a power of two), written back in slot order, one
+// node at a time per workgroup. Workgroups stride over the nodes 256 at a
+// time (one count per thread, the long segments listed in LDS). Longer
+// segments stay in fill order (their per-wavefront sums may then differ in
+// rounding from run to run).
+constexpr int IN_SORT_BIG = 16384;
+__global__ __launch_bounds__(256) void k_in_sort_big(const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ off, int64_t n,
+                                                     int32_t* __restrict__ in_slots) {
+  __shared__ int32_t s_v[IN_SORT_BIG];
+  __shared__ int32_t s_list[256];
+  __shared__ int s_nlist;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * 256; base < n; base += static_cast<int64_t>(gridDim.x) * 256) {
+    if (threadIdx.x == 0) s_nlist = 0;
+    __syncthreads();
+    const int64_t jt = base + threadIdx.x;
+    if (jt < n) {
+      const uint32_t ct = cnt[jt];
+      if (ct > 128 && ct <= IN_SORT_BIG) s_list[atomicAdd(&s_nlist, 1)] = static_cast<int32_t>(threadIdx.x);
+    }
+    __syncthreads();
+    const int nl = s_nlist;
+    __syncthreads();   // every thread has read the count before the next round resets it
+    for (int b = 0; b < nl; ++b) {
+      const int64_t j = base + s_list[b];
+      const int c = static_cast<int>(cnt[j]);
+      int P = 256;
+      while (P < c) P <<= 1;
+      int32_t* seg = in_slots + off[j];
+      for (int i = threadIdx.x; i < P; i += blockDim.x) s_v[i] = i < c ? seg[i] : INT32_MAX;
+      __syncthreads();
+      for (int kk = 2; kk <= P; kk <<= 1) {
+        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+          for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            const int ixj = i ^ jj;
+            if (ixj > i) {
+              const int32_t x = s_v[i], y = s_v[ixj];
+              if ((x > y) == ((i & kk) == 0)) {
+                s_v[i] = y;
+                s_v[ixj] = x;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (int i = threadIdx.x; i < c; i += blockDim.x) seg[i] = s_v[i];
+      __syncthreads();   // s_v is refilled for the next node
+    }
+  }
 }
 
 // Y += dY; dY = 0.
@@ -967,26 +1115,30 @@ static size_t spectral_ws(int64_t n, int k, int nc) {
 }
 
 struct EpochBufs {
-  float* wbase;      // ceil(n*k/64)*dim: Y + head update of the node crossing out of each wavefront
+  float* wbase;      // ceil(n*k/TAIL_CHUNK)*dim: Y + head update of the node crossing out of each chunk
   float* ybuf;       // n*dim, the other half of the double-buffered positions
-  int32_t* span;     // n: wavefront of the node's first in-edge entry if its entries cross a boundary, else -1
+  int32_t* span;     // n: chunk of the node's first in-edge entry if its entries cross a boundary, else -1
+  int32_t* span_w;   // n: chunk boundaries inside the node's entries
+  unsigned long long* hub_acc;   // n*dim: biased fixed-point tail sums of nodes spanning three or more chunks
   int32_t* in_head;  // n*k (n_in used): head of each in-edge entry's slot
   float* eps_in;     // n*k: the entry's epochs_per_sample
   float* epn_in;     // n*k: the entry's epochs_per_next, advanced by phase 2
 };
 
 static size_t epochs_ws(int64_t n, int k, int dim) {
-  return align_up(static_cast<size_t>(ceil_div(n * k, 64)) * dim * 4, 256) +
-         align_up(static_cast<size_t>(n) * dim * 4, 256) + align_up(static_cast<size_t>(n) * 4, 256) +
-         3 * align_up(static_cast<size_t>(n) * k * 4, 256) + 256;
+  return align_up(static_cast<size_t>(ceil_div(n * k, TAIL_CHUNK)) * dim * 4, 256) +
+         align_up(static_cast<size_t>(n) * dim * 4, 256) + 2 * align_up(static_cast<size_t>(n) * 4, 256) +
+         align_up(static_cast<size_t>(n) * dim * 8, 256) + 3 * align_up(static_cast<size_t>(n) * k * 4, 256) + 256;
 }
 
 static EpochBufs carve_epochs(void* ws, int64_t n, int k, int dim) {
   Carve c(ws);
   EpochBufs b;
-  b.wbase = c.take<float>(static_cast<size_t>(ceil_div(n * k, 64)) * dim);
+  b.wbase = c.take<float>(static_cast<size_t>(ceil_div(n * k, TAIL_CHUNK)) * dim);
   b.ybuf = c.take<float>(static_cast<size_t>(n) * dim);
   b.span = c.take<int32_t>(static_cast<size_t>(n));
+  b.span_w = c.take<int32_t>(static_cast<size_t>(n));
+  b.hub_acc = c.take<unsigned long long>(static_cast<size_t>(n) * dim);
   b.in_head = c.take<int32_t>(static_cast<size_t>(n) * k);
   b.eps_in = c.take<float>(static_cast<size_t>(n) * k);
   b.epn_in = c.take<float>(static_cast<size_t>(n) * k);
@@ -1036,8 +1188,11 @@ static int run_epoch(const float* Y, float* Ynext, const EpochBufs& B, const int
   else rc = launch_epoch<D, 64, 4>(Y, Ynext, B, movable, tail, eps, epn, n, k, epoch, neg_rate, law, seed, s);
   if (rc) return rc;
   if (n_in > 0) {
-    hipLaunchKernelGGL((k_tail_apply<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 256))), dim3(256), 0, s, in_node,
-                       B.in_head, B.eps_in, B.epn_in, n_in, Y, Ynext, B.wbase, movable, epoch, law);
+    // one wavefront per chunk of TAIL_CHUNK entries, four per workgroup
+    hipLaunchKernelGGL((k_tail_apply<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 4 * TAIL_CHUNK))), dim3(256), 0, s,
+                       in_node,
+                       B.in_head, B.eps_in, B.epn_in, n_in, Y, Ynext, B.wbase, movable, B.span_w, B.hub_acc,
+                       epoch, law);
     MV_LAUNCHED("k_tail_apply");
   }
   return MV_OK;
@@ -1258,6 +1413,9 @@ int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, in
   MV_LAUNCHED("k_in_fill");
   hipLaunchKernelGGL(k_in_sort, dim3(static_cast<unsigned>(ceil_div(n, 4))), dim3(256), 0, s, cnt, off, n, in_slots);
   MV_LAUNCHED("k_in_sort");
+  hipLaunchKernelGGL(k_in_sort_big, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 1024))), dim3(256),
+                     0, s, cnt, off, n, in_slots);
+  MV_LAUNCHED("k_in_sort_big");
   return MV_OK;
 }
 
@@ -1295,9 +1453,17 @@ int mv_umap_epochs_masked(float* Y, int64_t n, int dim, const int32_t* tail, int
   }
   hipLaunchKernelGGL(k_fill_i32, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, B.span, n, -1);
   MV_LAUNCHED("k_fill_i32");
-  if (n_in > 64) {
-    hipLaunchKernelGGL(k_span, dim3(static_cast<unsigned>(ceil_div(ceil_div(n_in, 64), 256))), dim3(256), 0, s, in_node,
-                       n_in, B.span);
+  // crossings and fixed-point accumulators start at zero in every call
+  hipLaunchKernelGGL(k_zero_u32, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s,
+                     reinterpret_cast<uint32_t*>(B.span_w), static_cast<int>(n));
+  MV_LAUNCHED("k_zero_u32");
+  hipLaunchKernelGGL(k_zero_u32, dim3(static_cast<unsigned>(ceil_div(2 * n * dim, 256))), dim3(256), 0, s,
+                     reinterpret_cast<uint32_t*>(B.hub_acc), static_cast<int>(2 * n * dim));
+  MV_LAUNCHED("k_zero_u32");
+  if (n_in > TAIL_CHUNK) {
+    hipLaunchKernelGGL(k_span, dim3(static_cast<unsigned>(ceil_div(ceil_div(n_in, TAIL_CHUNK), 256))), dim3(256), 0, s,
+                       in_node,
+                       n_in, B.span, B.span_w);
     MV_LAUNCHED("k_span");
   }
   // positions alternate between Y and ybuf; an odd number of epochs ends in ybuf

@@ -146,6 +146,40 @@ def test_epochs_without_negatives_match_oracle(n, k, dim, hubs, masked):
     np.testing.assert_allclose(got[hub], Yo[hub], rtol=1e-4, atol=1e-3)
 
 
+def test_epochs_repeat_bit_for_bit_with_hubs():
+    """Two runs of 30 epochs with negatives from the same init give the same
+    positions to the bit, on a graph whose hubs have over 1,000 in-edges
+    (their entries span ~20 wavefronts of the tail sums) and whose other
+    nodes cross wavefront boundaries."""
+    from mlx_vis_amd import find_ab_params
+    from mlx_vis_amd._umap import FuzzyGraph, run_epochs
+
+    n, k = 20_000, 15
+    rng = np.random.default_rng(5)
+    tails = ((np.repeat(np.arange(n), k) + 1 + rng.integers(0, n - 1, n * k)) % n).reshape(n, k)
+    pick = rng.uniform(size=n) < 0.3
+    tails[pick, 0] = rng.integers(0, 5, pick.sum())
+    tails = tails.astype(np.int32)
+    eps = rng.uniform(1.0, 3.0, (n, k)).astype(np.float32)
+    kept = np.flatnonzero(eps.ravel() > 0)
+    in_slots = kept[np.argsort(tails.ravel()[kept], kind="stable")].astype(np.int32)
+    in_node = tails.ravel()[in_slots].astype(np.int32)
+    assert np.bincount(in_node, minlength=n).max() > 1000
+    a, b = find_ab_params(1.0, 0.1)
+    Y0 = rng.uniform(-10, 10, (n, 2)).astype(np.float32)
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+    outs = []
+    for _ in range(2):
+        epn0 = eps.copy()
+        g = FuzzyGraph(dev(tails), None, dev(eps), dev(epn0), len(in_slots), 1.0,
+                       in_slots=dev(in_slots), in_node=dev(in_node))
+        Y = dev(Y0.copy())
+        run_epochs(Y, None, g, 1, 31, 200, 5, 1.0, a, b, 11)
+        outs.append(Y.cpu().numpy())
+    assert np.isfinite(outs[0]).all()
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 def test_c1_single_step_on_its_graph():
     """The full-shape single-step check: the 70,000 x 784 mixture (seed 1), its
     exact 15-NN graph and schedule, Y ~ U[-10, 10] (seed 0), the first 4,096
@@ -446,11 +480,10 @@ def test_umap_normalize_minmax_is_applied_before_the_knn():
     assert torch.equal(ua.knn_[0], ub.knn_[0]) and torch.equal(ua.knn_[1], ub.knn_[1])
     oi, _ = oknn.exact_knn(Xs, Xs, 10, self_base=0)
     np.testing.assert_array_equal(ua.knn_[0].cpu().numpy(), oi)
-    # ... and layouts of the same quality (not bitwise: tail sums at nodes with
-    # more than 64 in-edges reach them by atomics in arrival order, and the
-    # SGD amplifies such rounding differences over the epochs)
+    # ... and the same layout, bit for bit: the same graph, init and Philox
+    # negatives, and tail sums formed in an order that does not depend on
+    # scheduling (this input has nodes with up to 99 in-edges, whose entries
+    # span three wavefronts)
     assert Ya.shape == Yb.shape == (3000, 2) and np.isfinite(Ya).all()
-    sample = np.arange(0, 3000, 5)
-    ta, tb = om.trustworthiness(Xs, Ya, 10, rows=sample), om.trustworthiness(Xs, Yb, 10, rows=sample)
-    print(f"trustworthiness normalize=minmax {ta:.4f}, pre-scaled input {tb:.4f}")
-    assert abs(ta - tb) < 0.02
+    assert np.bincount(ua.knn_[0].cpu().numpy().ravel(), minlength=3000).max() > 64
+    np.testing.assert_array_equal(Ya, Yb)
