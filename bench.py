@@ -66,8 +66,10 @@ def parse():
                     help="fresh points mapped by the learned form (default 1,000,000)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form line")
     ap.add_argument("--no-scale", action="store_true", help="skip the 1M x 128 NNDescent + UMAP line")
-    ap.add_argument("--only", choices=["knn", "copy", "learned", "scale", "layout"], default=None,
-                    help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy")
+    ap.add_argument("--only", choices=["knn", "copy", "learned", "scale", "layout", "fit", "nndescent"], default=None,
+                    help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy, "
+                         "'fit' --steps fit_transform calls of the workload, 'nndescent' --steps NNDescent builds "
+                         "(k 15, 12 iterations, max_candidates 30) of the same data")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8", "bf16x3"])
     ap.add_argument("--rerank", type=int, default=None,
                     help="candidates per row = rerank * k for --only knn (default: the library's per-dtype default)")
@@ -480,6 +482,24 @@ def main():
                   knn_method="brute", random_state=42)
         Y = um.fit_transform(Xd)
         return um, Y
+
+    if args.only in ("fit", "nndescent"):
+        # --steps calls of the headline fit_transform or of NNDescent on the same
+        # data, each timed on the host clock after a synchronisation
+        from mlx_vis_amd import NNDescent
+
+        times = []
+        for _ in range(max(args.steps, 1)):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            if args.only == "fit":
+                step()
+            else:
+                NNDescent(k=15, n_iters=12, max_candidates=30, random_state=0).build_device(Xd)
+            torch.cuda.synchronize()
+            times.append(round((time.perf_counter() - t) * 1e3, 2))
+        print(json.dumps({args.only + "_ms": times}), flush=True)
+        return
 
     for i in range(args.warmup):
         log(f"warmup {i + 1}/{args.warmup}")

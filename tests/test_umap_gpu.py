@@ -94,9 +94,9 @@ def test_epochs_without_negatives_match_oracle(n, k, dim, hubs, masked):
     """With negative_sample_rate 0 a layout epoch is the attraction step of
     the epoch's active edges from one snapshot, so several epochs can be
     checked against the oracle's sgd_step (FP64), schedule included. Covers
-    tail nodes with more than 64 in-edges (summed over several wavefronts),
-    nodes without in-edges, pruned slots, frozen nodes, and at 300,000 x 15
-    the slot-order tail buffer used beyond the cache-resident size."""
+    tail nodes with more than 64 in-edges (summed over several 128-entry
+    chunks, in fixed point beyond two), nodes without in-edges, pruned slots,
+    frozen nodes, and a 300,000 x 15 graph."""
     from mlx_vis_amd import find_ab_params
     from mlx_vis_amd._umap import FuzzyGraph, run_epochs
 
