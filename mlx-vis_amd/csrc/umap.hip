@@ -887,17 +887,28 @@ constexpr int GRAM_STRIDE = 48;       // >= NC_MAX (NC_MAX + 1) / 2 packed entri
 // the copies in copy order.
 constexpr int GRAM_COPIES = 32;
 
-// s_g[p] = sum over the copies of the packed Gram entry p (threads p < NP);
-// followed by a barrier.
+// Sum over the copies of the packed Gram entry p = threadIdx.x (threads
+// p < NP; 0 elsewhere), in copy order.
 template <int NC>
-__device__ __forceinline__ void gram_gather(const double* __restrict__ g, double* s_g) {
+__device__ __forceinline__ double gram_sum(const double* __restrict__ g) {
   constexpr int NP = NC * (NC + 1) / 2;
+  double v = 0.0;
   if (g && threadIdx.x < NP) {
-    double v = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < GRAM_COPIES; ++c) v += g[c * GRAM_STRIDE + threadIdx.x];
-    s_g[threadIdx.x] = v;
+    double c8[GRAM_COPIES];
+#pragma unroll
+    for (int c = 0; c < GRAM_COPIES; ++c) c8[c] = g[c * GRAM_STRIDE + threadIdx.x];   // all loads in flight
+#pragma unroll
+    for (int c = 0; c < GRAM_COPIES; ++c) v += c8[c];
   }
+  return v;
+}
+
+// s_g[p] = the summed Gram entry p (threads p < NP hold it in v); followed by
+// a barrier.
+template <int NC>
+__device__ __forceinline__ void gram_store(double v, double* s_g) {
+  constexpr int NP = NC * (NC + 1) / 2;
+  if (threadIdx.x < NP) s_g[threadIdx.x] = v;
   __syncthreads();
 }
 
@@ -963,6 +974,9 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
   __shared__ double s_g[NP];
   if (blockIdx.x == 0)
     for (int z = threadIdx.x; z < GRAM_COPIES * GRAM_STRIDE; z += blockDim.x) gram_zero[z] = 0.0;
+  // the previous step's Gram matrix, loaded before the gathers so that both
+  // arrive together
+  const double gsum = gram_sum<NC>(gram_prev);
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;
   const int c = static_cast<int>(gid % G);
@@ -999,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
   // R^-1 after the product: the Cholesky factorisation of the previous Gram
   // matrix (a dependent load and serial FP64 work) overlaps the gathers of
   // the other wavefronts instead of delaying them all.
-  gram_gather<NC>(gram_prev, s_g);
+  gram_store<NC>(gsum, s_g);
   if (threadIdx.x == 0) chol_inverse<NC>(gram_prev ? s_g : nullptr, s_R);
   __syncthreads();
   const bool owner = c == 0 && i < n;
@@ -1043,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U,
   constexpr int NP = NC * (NC + 1) / 2;
   __shared__ float s_R[NC * NC];
   __shared__ double s_g[NP];
-  gram_gather<NC>(gram, s_g);
+  gram_store<NC>(gram_sum<NC>(gram), s_g);
   if (threadIdx.x == 0) chol_inverse<NC>(gram ? s_g : nullptr, s_R);
   __syncthreads();
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
