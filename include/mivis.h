@@ -198,8 +198,9 @@ int mv_umap_fuzzy_graph(const int32_t* knn_idx, const float* knn_dist, int64_t n
 /* In-edge lists of the kept slots (eps > 0), grouped by tail node: entries
  * t < (number of kept slots) hold the slot in_slots[t] and its tail
  * in_node[t]; nodes appear in ascending order, the slots of a node with at
- * most 64 in-edges in ascending order (of a node with more in no fixed order).
- * Used by mv_umap_epochs to sum tail updates without per-edge atomics.
+ * most 16,384 in-edges in ascending order (of a node with more in no fixed
+ * order). Used by mv_umap_epochs to sum tail updates without per-edge atomics,
+ * in an order that makes a run repeat bit for bit.
  * ws: 3 * align256(4n) + align256(4 * ceil(n / 1024)) bytes. */
 int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, int32_t* in_slots,
                      int32_t* in_node, void* ws, size_t ws_bytes, void* stream);
@@ -213,7 +214,8 @@ int mv_umap_in_edges(const int32_t* tail, const float* eps, int64_t n, int k, in
  * against nodes drawn uniformly from [0, n) by Philox-4x32-10 keyed by seed;
  * the step size is lr * (1 - e / n_epochs). tail = the kNN indices (n, k);
  * (in_slots, in_node, n_in) from mv_umap_in_edges. ws sized by
- * mv_umap_epochs_ws_bytes. dim <= 8, k <= 256. */
+ * mv_umap_epochs_ws_bytes. dim <= 8, k <= 256. The same inputs give the same
+ * Y to the bit on every run (unless a node has more than 16,384 in-edges). */
 int mv_umap_epochs_ws_bytes(int64_t n, int k, int dim, size_t* out_bytes);
 int mv_umap_epochs(float* Y, int64_t n, int dim, const int32_t* tail, int k, const float* eps, float* epn,
                    const int32_t* in_slots, const int32_t* in_node, int64_t n_in, int e0, int e1, int n_epochs,

@@ -286,7 +286,7 @@ __device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 
 // tail updates are phase 2's: no atomics here, and nothing is stored per slot.
 constexpr int NEG_CHUNK = 8;
 
-template <int D, int G, int S>
+template <int D, int G, int S, int NR>   // NR >= 0: negatives per edge fixed at compile time; -1: neg_rate
 __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ Ynext,
                                                float* __restrict__ wbase, const int32_t* __restrict__ span,
                                                const uint8_t* __restrict__ movable, const int32_t* __restrict__ tail,
@@ -360,7 +360,45 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
         hi_[q] = s_yi[w][pi][q];
         h[q] = 0.f;
       }
-      for (int base = 0; base < neg_rate; base += NEG_CHUNK) {
+      if constexpr (NR >= 0) {
+        // a count known at compile time: the gathers are straight-line code,
+        // all in flight together (a branch per negative makes the compiler
+        // wait for earlier gathers at each join); same Philox words, same
+        // order of the additions as the general loop below
+        constexpr int NG = (NR + 3) / 4;
+        uint32_t words[NG > 0 ? 4 * NG : 1];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+          const uint4 r = Philox::gen(make_uint4(static_cast<uint32_t>(ei), 0u, static_cast<uint32_t>(epoch),
+                                                 static_cast<uint32_t>(g)),
+                                      seed_lo, seed_hi);
+          words[4 * g] = r.x;
+          words[4 * g + 1] = r.y;
+          words[4 * g + 2] = r.z;
+          words[4 * g + 3] = r.w;
+        }
+        float yt[NR > 0 ? NR : 1][D];
+#pragma unroll
+        for (int s2 = 0; s2 < NR; ++s2) {
+          const int64_t t = uniform_below(words[s2], static_cast<uint32_t>(n));
+#pragma unroll
+          for (int q = 0; q < D; ++q) yt[s2][q] = Y[t * D + q];
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < NR; ++s2) {
+          float nd[D];
+          float n2 = 0.f;
+#pragma unroll
+          for (int q = 0; q < D; ++q) {
+            nd[q] = hi_[q] - yt[s2][q];
+            n2 += nd[q] * nd[q];
+          }
+          const float cr = law.repulse(n2);
+#pragma unroll
+          for (int q = 0; q < D; ++q) h[q] += clip4(cr * nd[q]) * law.alpha;
+        }
+      }
+      for (int base = 0; NR < 0 && base < neg_rate; base += NEG_CHUNK) {
         float yt[NEG_CHUNK][D];
         const uint4 r0 = Philox::gen(make_uint4(static_cast<uint32_t>(ei), 0u, static_cast<uint32_t>(epoch),
                                                 static_cast<uint32_t>(base >> 2)),
@@ -1180,8 +1218,15 @@ static int launch_epoch(const float* Y, float* Ynext, const EpochBufs& B, const 
                         Law law, uint64_t seed, hipStream_t s) {
   const int64_t threads = n * G;
   const unsigned blocks = static_cast<unsigned>(ceil_div(threads, 256));
-  hipLaunchKernelGGL((k_epoch<D, G, S>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.wbase, B.span, movable, tail, eps,
-                     epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  // the reference's default of 5 negatives per edge runs a specialised copy
+  if (neg_rate == 5)
+    hipLaunchKernelGGL((k_epoch<D, G, S, 5>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.wbase, B.span, movable, tail,
+                       eps, epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed),
+                       static_cast<uint32_t>(seed >> 32));
+  else
+    hipLaunchKernelGGL((k_epoch<D, G, S, -1>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.wbase, B.span, movable, tail,
+                       eps, epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed),
+                       static_cast<uint32_t>(seed >> 32));
   MV_LAUNCHED("k_epoch");
   return MV_OK;
 }
