@@ -286,7 +286,9 @@ __device__ __forceinline__ float clip4(float v) { return fminf(fmaxf(v, -4.0f), 
 // tail updates are phase 2's: no atomics here, and nothing is stored per slot.
 constexpr int NEG_CHUNK = 8;
 
-template <int D, int G, int S, int NR>   // NR >= 0: negatives per edge fixed at compile time; -1: neg_rate
+// NR >= 0: negatives per edge fixed at compile time, -1: neg_rate; MASKED:
+// movable is read (frozen nodes keep their position)
+template <int D, int G, int S, int NR, bool MASKED>
 __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, float* __restrict__ Ynext,
                                                float* __restrict__ wbase, const int32_t* __restrict__ span,
                                                const uint8_t* __restrict__ movable, const int32_t* __restrict__ tail,
@@ -301,31 +303,33 @@ __global__ __launch_bounds__(256) void k_epoch(const float* __restrict__ Y, floa
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;      // head row
   const int c = static_cast<int>(gid % G);
+  // No load is under a branch (a lane outside the slot array reads slot 0 and
+  // row 0, and its values are masked): a branch per load makes the compiler
+  // wait for the loads before it at each join, so the slots' schedules, tail
+  // indices, the head position and the row owner's span entry and movable
+  // flag are all in flight together.
   int64_t e[S];
   float sch[S], nx[S];
   int32_t j[S];
   bool in_row[S], act[S];
+  const int64_t ic = i < n ? i : 0;
 #pragma unroll
   for (int u = 0; u < S; ++u) {
-    e[u] = i * k + c + u * G;
     in_row[u] = i < n && c + u * G < k;
-    sch[u] = -1.f;
-    nx[u] = INFINITY;
-    j[u] = 0;
-    if (in_row[u]) {
-      sch[u] = eps[e[u]];
-      nx[u] = epn[e[u]];
-      j[u] = tail[e[u]];
-    }
+    e[u] = in_row[u] ? i * k + c + u * G : 0;
+    sch[u] = eps[e[u]];
+    nx[u] = epn[e[u]];
+    j[u] = tail[e[u]];
   }
   float yi[D];
 #pragma unroll
-  for (int q = 0; q < D; ++q) yi[q] = i < n ? Y[i * D + q] : 0.f;
-  // the row owner's span entry and movable flag, loaded with the slots so
-  // that the final store does not wait for another memory round trip
+  for (int q = 0; q < D; ++q) yi[q] = Y[ic * D + q];
   const bool owner = c == 0 && i < n;
-  const int32_t sp = owner ? span[i] : -1;
-  const bool frozen = owner && movable && !movable[i];
+  const int32_t sp_r = span[ic];
+  bool frozen = false;
+  if constexpr (MASKED) frozen = movable[ic] == 0;
+  const int32_t sp = owner ? sp_r : -1;
+  frozen = owner && frozen;
 #pragma unroll
   for (int u = 0; u < S; ++u) {
     act[u] = in_row[u] && sch[u] > 0.f && nx[u] <= static_cast<float>(epoch);
@@ -519,7 +523,7 @@ constexpr unsigned long long FIX_BIAS = 1ull << 54;   // one chunk counted into 
 constexpr int HUB_MAX_WAVES = 1023;                   // chunks a fixed-point hub sum can count
 constexpr int TAIL_CHUNK = 128;                       // in-edge entries per wavefront of k_tail_apply (two per lane)
 
-template <int D>
+template <int D, bool MASKED>   // MASKED: movable is read (frozen nodes keep their position)
 __global__ __launch_bounds__(256) void k_tail_apply(const int32_t* __restrict__ in_node,
                                                     const int32_t* __restrict__ in_head,
                                                     const float* __restrict__ eps_in, float* __restrict__ epn_in,
@@ -530,46 +534,82 @@ __global__ __launch_bounds__(256) void k_tail_apply(const int32_t* __restrict__ 
   const int64_t wv = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;   // chunk of this wavefront
   const int l = threadIdx.x & 63;
   const int64_t c0 = wv * TAIL_CHUNK;   // first entry of the chunk
-  // Every load of the wavefront is independent of the others: issued together.
-  int node[2], crossings[2];
-  bool valid[2], mov[2];
-  float es[2], ne[2];
-  int64_t hd[2];
+  if (c0 >= m) return;                  // a wavefront past the last chunk (the grid is whole workgroups)
+  // No load is under a branch (lanes past the end read the last entry and
+  // their values are masked): a branch per load makes the compiler wait for
+  // the loads before it at each join. The entry loads come first, then those
+  // that depend on the node, then the two halves' position gathers together.
+  bool valid[2];
+  int64_t tc[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int64_t t = c0 + h * 64 + l;
     valid[h] = t < m;
-    node[h] = valid[h] ? in_node[t] : -1 - h;   // distinct out-of-range keys end the segments
-    mov[h] = !movable || (valid[h] && movable[node[h]]);
-    crossings[h] = valid[h] ? span_w[node[h]] : 0;   // chunk boundaries inside the node's entries
-    es[h] = valid[h] ? eps_in[t] : -1.f;
-    ne[h] = valid[h] ? epn_in[t] : INFINITY;
-    hd[h] = valid[h] ? in_head[t] : 0;
+    tc[h] = valid[h] ? t : m - 1;
   }
-  // the node before the chunk (lane 0) and after it (lane 63)
-  const int before = (l == 0 && c0 > 0 && c0 < m) ? in_node[c0 - 1] : -1;
-  const int after = (l == 63 && c0 + TAIL_CHUNK < m) ? in_node[c0 + TAIL_CHUNK] : -1;
+  int rn[2];
+  float es[2], ne[2];
+  int64_t hd[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    rn[h] = in_node[tc[h]];
+    es[h] = eps_in[tc[h]];
+    ne[h] = epn_in[tc[h]];
+    hd[h] = in_head[tc[h]];
+  }
+  // the node before the chunk (lane 0) and after it (lane 63); the base of the
+  // node crossing out of the chunk (same address in every lane)
+  const int rb = in_node[c0 > 0 ? c0 - 1 : 0];
+  const int ra = in_node[c0 + TAIL_CHUNK < m ? c0 + TAIL_CHUNK : m - 1];
+  float bq[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) bq[q] = wbase[wv * D + q];
+  int node[2], crossings[2];
+  bool mov[2], act[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    crossings[h] = span_w[rn[h]];   // chunk boundaries inside the node's entries
+    mov[h] = true;
+    if constexpr (MASKED) mov[h] = movable[rn[h]] != 0;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    node[h] = valid[h] ? rn[h] : -1 - h;   // distinct out-of-range keys end the segments
+    act[h] = valid[h] && es[h] > 0.f && ne[h] <= static_cast<float>(epoch);
+    mov[h] = mov[h] && valid[h];
+    crossings[h] = valid[h] ? crossings[h] : 0;
+  }
+  const int before = (l == 0 && c0 > 0) ? rb : -1;
+  const int after = (l == 63 && c0 + TAIL_CHUNK < m) ? ra : -1;
   float base[D];
 #pragma unroll
-  for (int q = 0; q < D; ++q) base[q] = l == 63 && valid[1] ? wbase[wv * D + q] : 0.f;
+  for (int q = 0; q < D; ++q) base[q] = l == 63 && valid[1] ? bq[q] : 0.f;
+  // an inactive entry gathers its own node's position twice (a line the
+  // wavefront reads anyway) and contributes 0
+  float yh[2][D], yn[2][D];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t src = act[h] ? hd[h] : static_cast<int64_t>(rn[h]);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      yh[h][q] = Y[src * D + q];
+      yn[h][q] = Y[static_cast<int64_t>(rn[h]) * D + q];
+    }
+  }
   float v[2][D];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
+    if (act[h]) epn_in[tc[h]] = ne[h] + es[h];
+    float diff[D];
+    float d2 = 0.f;
 #pragma unroll
-    for (int q = 0; q < D; ++q) v[h][q] = 0.f;
-    if (es[h] > 0.f && ne[h] <= static_cast<float>(epoch)) {
-      epn_in[c0 + h * 64 + l] = ne[h] + es[h];
-      float diff[D];
-      float d2 = 0.f;
-#pragma unroll
-      for (int q = 0; q < D; ++q) {
-        diff[q] = Y[hd[h] * D + q] - Y[static_cast<int64_t>(node[h]) * D + q];
-        d2 += diff[q] * diff[q];
-      }
-      const float ca = law.attract(d2);
-#pragma unroll
-      for (int q = 0; q < D; ++q) v[h][q] = -__fmul_rn(clip4(ca * diff[q]), law.alpha);
+    for (int q = 0; q < D; ++q) {
+      diff[q] = yh[h][q] - yn[h][q];
+      d2 += diff[q] * diff[q];
     }
+    const float ca = law.attract(d2);
+#pragma unroll
+    for (int q = 0; q < D; ++q) v[h][q] = act[h] ? -__fmul_rn(clip4(ca * diff[q]), law.alpha) : 0.f;
   }
   // Segmented inclusive scans of the two halves; the segment running from
   // the end of the first half into the second carries its sum into lane 0 of
@@ -1219,14 +1259,10 @@ static int launch_epoch(const float* Y, float* Ynext, const EpochBufs& B, const 
   const int64_t threads = n * G;
   const unsigned blocks = static_cast<unsigned>(ceil_div(threads, 256));
   // the reference's default of 5 negatives per edge runs a specialised copy
-  if (neg_rate == 5)
-    hipLaunchKernelGGL((k_epoch<D, G, S, 5>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.wbase, B.span, movable, tail,
-                       eps, epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed),
-                       static_cast<uint32_t>(seed >> 32));
-  else
-    hipLaunchKernelGGL((k_epoch<D, G, S, -1>), dim3(blocks), dim3(256), 0, s, Y, Ynext, B.wbase, B.span, movable, tail,
-                       eps, epn, n, k, epoch, neg_rate, law, static_cast<uint32_t>(seed),
-                       static_cast<uint32_t>(seed >> 32));
+  // (the masked epochs, used for refinement, run the general copy)
+  auto kern = movable ? k_epoch<D, G, S, -1, true> : neg_rate == 5 ? k_epoch<D, G, S, 5, false> : k_epoch<D, G, S, -1, false>;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, Y, Ynext, B.wbase, B.span, movable, tail, eps, epn, n, k,
+                     epoch, neg_rate, law, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
   MV_LAUNCHED("k_epoch");
   return MV_OK;
 }
@@ -1248,7 +1284,8 @@ static int run_epoch(const float* Y, float* Ynext, const EpochBufs& B, const int
   if (rc) return rc;
   if (n_in > 0) {
     // one wavefront per chunk of TAIL_CHUNK entries, four per workgroup
-    hipLaunchKernelGGL((k_tail_apply<D>), dim3(static_cast<unsigned>(ceil_div(n_in, 4 * TAIL_CHUNK))), dim3(256), 0, s,
+    auto tail_kernel = movable ? k_tail_apply<D, true> : k_tail_apply<D, false>;
+    hipLaunchKernelGGL(tail_kernel, dim3(static_cast<unsigned>(ceil_div(n_in, 4 * TAIL_CHUNK))), dim3(256), 0, s,
                        in_node,
                        B.in_head, B.eps_in, B.epn_in, n_in, Y, Ynext, B.wbase, movable, B.span_w, B.hub_acc,
                        epoch, law);
