@@ -966,19 +966,22 @@ constexpr int GRAM_STRIDE = 48;       // >= NC_MAX (NC_MAX + 1) / 2 packed entri
 constexpr int GRAM_COPIES = 32;
 
 // Sum over the copies of the packed Gram entry p = threadIdx.x (threads
-// p < NP; 0 elsewhere), in copy order.
+// p < NP; 0 elsewhere, and 0 when g is null), in copy order. The loads are
+// not under a branch (every thread reads a valid entry of `any`, a Gram
+// buffer, when g is null or p is out of range), so they stay in flight with
+// the loads that follow.
 template <int NC>
-__device__ __forceinline__ double gram_sum(const double* __restrict__ g) {
+__device__ __forceinline__ double gram_sum(const double* __restrict__ g, const double* __restrict__ any) {
   constexpr int NP = NC * (NC + 1) / 2;
+  const int p = threadIdx.x < NP ? static_cast<int>(threadIdx.x) : 0;
+  const double* src = g ? g : any;
+  double c8[GRAM_COPIES];
+#pragma unroll
+  for (int c = 0; c < GRAM_COPIES; ++c) c8[c] = src[c * GRAM_STRIDE + p];
   double v = 0.0;
-  if (g && threadIdx.x < NP) {
-    double c8[GRAM_COPIES];
 #pragma unroll
-    for (int c = 0; c < GRAM_COPIES; ++c) c8[c] = g[c * GRAM_STRIDE + threadIdx.x];   // all loads in flight
-#pragma unroll
-    for (int c = 0; c < GRAM_COPIES; ++c) v += c8[c];
-  }
-  return v;
+  for (int c = 0; c < GRAM_COPIES; ++c) v += c8[c];
+  return g && threadIdx.x < NP ? v : 0.0;
 }
 
 // s_g[p] = the summed Gram entry p (threads p < NP hold it in v); followed by
@@ -1054,31 +1057,32 @@ __global__ __launch_bounds__(256) void k_spec_iter(const float* __restrict__ U, 
     for (int z = threadIdx.x; z < GRAM_COPIES * GRAM_STRIDE; z += blockDim.x) gram_zero[z] = 0.0;
   // the previous step's Gram matrix, loaded before the gathers so that both
   // arrive together
-  const double gsum = gram_sum<NC>(gram_prev);
+  const double gsum = gram_sum<NC>(gram_prev, gram_cur);
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = gid / G;
   const int c = static_cast<int>(gid % G);
   float t[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) t[q] = 0.f;
-  if (i < n) {
+  {
+    // no load under a branch: slots past the row (or rows past n) read the
+    // row's (row 0's) last slot again with weight 0
+    const int64_t ic = i < n ? i : 0;
     float w[S];
     int64_t j[S];
 #pragma unroll
     for (int v = 0; v < S; ++v) {
       const int cs = c * S + v;
-      w[v] = 0.f;
-      j[v] = 0;
-      if (cs < k) {
-        w[v] = wn[i * k + cs];
-        j[v] = tail[i * k + cs];
-      }
+      const int64_t e = ic * k + (cs < k ? cs : k - 1);
+      w[v] = wn[e];
+      j[v] = tail[e];
+      if (!(i < n && cs < k)) w[v] = 0.f;
     }
     float u8[S][NC];
 #pragma unroll
     for (int v = 0; v < S; ++v)
 #pragma unroll
-      for (int q = 0; q < NC; ++q) u8[v][q] = U[j[v] * NC + q];   // slots past the row: node 0, weight 0
+      for (int q = 0; q < NC; ++q) u8[v][q] = U[j[v] * NC + q];
 #pragma unroll
     for (int v = 0; v < S; ++v)
 #pragma unroll
@@ -1135,7 +1139,7 @@ __global__ __launch_bounds__(256) void k_spec_apply(const float* __restrict__ U,
   constexpr int NP = NC * (NC + 1) / 2;
   __shared__ float s_R[NC * NC];
   __shared__ double s_g[NP];
-  gram_store<NC>(gram_sum<NC>(gram), s_g);
+  gram_store<NC>(gram_sum<NC>(gram, gram), s_g);
   if (threadIdx.x == 0) chol_inverse<NC>(gram ? s_g : nullptr, s_R);
   __syncthreads();
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
