@@ -150,6 +150,14 @@ __device__ __forceinline__ void stage_row(const Src& s, int64_t i, float* lds_f,
 // Dot of the staged row with row j by a 16-lane group (sub = lane % 16); the
 // result is reduced over the group in a fixed order, so a pair's value does
 // not depend on which slot or row computes it.
+// Dot product of the staged row (LDS) with row j, over the 16 lanes of a
+// group (lane `sub` takes the 16-byte pieces sub, sub + 16, ...). The pieces
+// of row j are fetched GD_BATCH at a time with no branch around the loads
+// (an index past the row reads its last piece and meets a zero query piece),
+// so that a batch is in flight together instead of one memory round trip per
+// piece; the FMAs run in the same order as piece by piece (adding 0 * b
+// changes nothing), so the dot is the same to the bit.
+constexpr int GD_BATCH = 4;
 __device__ __forceinline__ float group_dot(const Src& s, int64_t j, const float* lds_f, const __half* lds_h,
                                            int sub, bool live) {
   float acc = 0.f;
@@ -158,29 +166,49 @@ __device__ __forceinline__ float group_dot(const Src& s, int64_t j, const float*
       const uint4* xr = reinterpret_cast<const uint4*>(s.xh + j * s.ld);
       const uint4* qr = reinterpret_cast<const uint4*>(lds_h);
       const int64_t nch = s.ld / 8;
-      for (int64_t t = sub; t < nch; t += 16) {
-        const uint4 a = qr[t], b = xr[t];
-        const uint32_t av[4] = {a.x, a.y, a.z, a.w};
-        const uint32_t bv[4] = {b.x, b.y, b.z, b.w};
+      for (int64_t t0 = sub; t0 < nch; t0 += 16 * GD_BATCH) {
+        uint4 bb[GD_BATCH];
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-          const __half2 ha = *reinterpret_cast<const __half2*>(&av[h]);
-          const __half2 hb = *reinterpret_cast<const __half2*>(&bv[h]);
-          const float2 fa = __half22float2(ha), fb = __half22float2(hb);
-          acc = fmaf(fa.x, fb.x, acc);
-          acc = fmaf(fa.y, fb.y, acc);
+        for (int u = 0; u < GD_BATCH; ++u) {
+          const int64_t t = t0 + 16 * u;
+          bb[u] = xr[t < nch ? t : nch - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < GD_BATCH; ++u) {
+          const int64_t t = t0 + 16 * u;
+          const uint4 a = t < nch ? qr[t] : make_uint4(0u, 0u, 0u, 0u);
+          const uint32_t av[4] = {a.x, a.y, a.z, a.w};
+          const uint32_t bv[4] = {bb[u].x, bb[u].y, bb[u].z, bb[u].w};
+#pragma unroll
+          for (int h = 0; h < 4; ++h) {
+            const __half2 ha = *reinterpret_cast<const __half2*>(&av[h]);
+            const __half2 hb = *reinterpret_cast<const __half2*>(&bv[h]);
+            const float2 fa = __half22float2(ha), fb = __half22float2(hb);
+            acc = fmaf(fa.x, fb.x, acc);
+            acc = fmaf(fa.y, fb.y, acc);
+          }
         }
       }
     } else {
       const float4* xr = reinterpret_cast<const float4*>(s.xf + j * s.ld);
       const float4* qr = reinterpret_cast<const float4*>(lds_f);
       const int64_t nch = s.ld / 4;
-      for (int64_t t = sub; t < nch; t += 16) {
-        const float4 a = qr[t], b = xr[t];
-        acc = fmaf(a.x, b.x, acc);
-        acc = fmaf(a.y, b.y, acc);
-        acc = fmaf(a.z, b.z, acc);
-        acc = fmaf(a.w, b.w, acc);
+      for (int64_t t0 = sub; t0 < nch; t0 += 16 * GD_BATCH) {
+        float4 bb[GD_BATCH];
+#pragma unroll
+        for (int u = 0; u < GD_BATCH; ++u) {
+          const int64_t t = t0 + 16 * u;
+          bb[u] = xr[t < nch ? t : nch - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < GD_BATCH; ++u) {
+          const int64_t t = t0 + 16 * u;
+          const float4 a = t < nch ? qr[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+          acc = fmaf(a.x, bb[u].x, acc);
+          acc = fmaf(a.y, bb[u].y, acc);
+          acc = fmaf(a.z, bb[u].z, acc);
+          acc = fmaf(a.w, bb[u].w, acc);
+        }
       }
     }
   }
