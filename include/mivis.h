@@ -154,9 +154,10 @@ int mv_row_sqnorms(const float* X, int64_t n, int64_t d, float* out, void* strea
  * init_idx (n, k) (nndescent.py:54-56), P1 (d, p1) and P2 (d, p2) (NULL with
  * p = 0 when the projection is not smaller than d). out_changed (HOST, n_iters
  * entries, nullable): changed slots per iteration; out_iters (HOST, nullable):
- * iterations run. out_dist: Euclidean. Limits: k <= 64 (any max_candidates;
- * rows with more than 1,024 candidate slots are merged 1,024 at a time),
- * d <= 4096.
+ * iterations run. out_dist: Euclidean. Limits: k <= 128 (any max_candidates;
+ * rows with more than 1,024 candidate slots are merged 1,024 at a time; from
+ * k = 65 each lane holds two of the row's neighbours), n * k < 2^31,
+ * d <= 4096 (d <= 3968 for k > 64).
  * One device-to-host read of a counter per iteration synchronises `stream`.
  */
 int mv_nnd_ws_bytes(int64_t n, int64_t d, int k, int64_t p1, int64_t p2, size_t* out_bytes);
@@ -166,13 +167,13 @@ int mv_nnd_build(const float* X, int64_t n, int64_t d, int k, int n_iters, int m
                  size_t ws_bytes, void* stream);
 /* Reverse candidates alone (_compute_reverse_candidates, nndescent.py:406-436):
  * rev[j, t] = source row of the t-th edge pointing at j in flat (row-major)
- * order, t < k; slots beyond the in-degree hold 0. ws sized by
- * mv_nnd_ws_bytes(n, 1, k, 0, 0). */
+ * order, t < k; slots beyond the in-degree hold 0. k <= 128, n * k < 2^31.
+ * ws sized by mv_nnd_ws_bytes(n, 1, k, 0, 0). */
 int mv_nnd_reverse(const int32_t* idx, int64_t n, int k, int32_t* rev, void* ws, size_t ws_bytes, void* stream);
 /* Squared distances of (i, idx[i, c]) as _gather_dists (half == 0,
  * nndescent.py:573-612) or _gather_dists_half (half != 0, 615-641: FP16
- * inputs, FP32 accumulation, FP16-rounded dot, FP32 norms). ws sized by
- * mv_nnd_ws_bytes(n, d, k, 0, 0). */
+ * inputs, FP32 accumulation, FP16-rounded dot, FP32 norms). k <= 128. ws
+ * sized by mv_nnd_ws_bytes(n, d, k, 0, 0). */
 int mv_nnd_gather_dists(const float* X, int64_t n, int64_t d, const int32_t* idx, int k, int half,
                         float* out_dist, void* ws, size_t ws_bytes, void* stream);
 

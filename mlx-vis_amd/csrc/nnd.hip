@@ -528,6 +528,162 @@ __global__ __launch_bounds__(256) void k_nnd_update_chunked(UpdateArgs a) {
   if (l == 0 && fm) atomicAdd(a.changed, static_cast<uint32_t>(__popcll(fm)));
 }
 
+// k from 65 to NND_K_WIDE: lane l holds current entries l and 64 + l, and the
+// running top-k list the same way (two keys per lane). Candidate slots are
+// taken in chunks of 1024 and merged as in k_nnd_update_chunked (same slot
+// order, same duplicate and self rules). The source rows of the nn-of-nn
+// expansion (positional, or NEW first in the converging path) are put in
+// order through LDS, since mc_new may exceed 64.
+constexpr int NND_K_WIDE = 128;
+__global__ __launch_bounds__(256) void k_nnd_update_wide(UpdateArgs a) {
+  constexpr int NQ = NND_NQ_MAX;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ int32_t s_first[4][NND_K_WIDE];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (i >= a.n) return;
+  const int k = a.k;
+  const int64_t bytes_per_wave = a.src.xh ? a.src.ld * 2 : a.src.ld * 4;
+  float* lds_f = reinterpret_cast<float*>(smem + w * bytes_per_wave);
+  __half* lds_h = reinterpret_cast<__half*>(smem + w * bytes_per_wave);
+  int32_t* first = s_first[w];
+
+  int32_t cur_id[2];
+  float cur_d[2];
+  bool cur_new[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int e = 64 * j + l;
+    cur_id[j] = e < k ? a.idx[i * k + e] : -1;
+    cur_d[j] = e < k ? a.dist[i * k + e] : 0.f;
+    cur_new[j] = e < k ? a.flags[i * k + e] != 0 : false;
+  }
+  const uint64_t nm0 = ballot(cur_new[0]), nm1 = ballot(cur_new[1]);
+  if (a.converging && (nm0 | nm1) == 0) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int e = 64 * j + l;
+      if (e < k) {
+        a.out_idx[i * k + e] = cur_id[j];
+        a.out_dist[i * k + e] = cur_d[j];
+        a.out_flags[i * k + e] = 0;
+      }
+    }
+    return;
+  }
+  {
+    const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
+    const int n_new = __popcll(nm0) + __popcll(nm1);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int e = 64 * j + l;
+      if (e < k) {
+        int pos = e;
+        if (a.converging) {
+          const int new_before = j == 0 ? __popcll(nm0 & below) : __popcll(nm0) + __popcll(nm1 & below);
+          pos = cur_new[j] ? new_before : n_new + (e - new_before);
+        }
+        first[pos] = cur_id[j];
+      }
+    }
+  }
+  const int n_nn = a.mc_new * a.j_nn;
+  const int n_rev = a.rev ? k : 0;
+  const int total = k + n_nn + n_rev;
+
+  stage_row(a.src, i, lds_f, lds_h, l);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const float sqi = a.src.sq[i];
+  const int g = l >> 4, sub = l & 15;
+
+  uint64_t lk0 = KEY_NONE, lk1 = KEY_NONE;   // running list: lane l holds ranks l and 64 + l
+  for (int base = 0; base < total; base += 64 * NQ) {
+    int32_t sid[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int s = base + 64 * q + l;
+      const int rel = s - k;
+      const int m = (rel >= 0 && rel < n_nn) ? rel / a.j_nn : 0;
+      const int t = (rel >= 0 && rel < n_nn) ? rel % a.j_nn : 0;
+      int32_t id = -1;
+      if (s < k) id = q == 0 ? cur_id[0] : cur_id[1];   // s < k <= 128: base 0, q 0 or 1
+      else if (s < k + n_nn) id = a.idx[static_cast<int64_t>(first[m]) * k + t];
+      else if (s < total) id = a.rev[i * k + (s - k - n_nn)];
+      sid[q] = id;
+    }
+    if (!a.recompute_cur) {
+      for (int u = 0; u < k; ++u) {
+        const int32_t cu = u < 64 ? __shfl(cur_id[0], u) : __shfl(cur_id[1], u - 64);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (base + 64 * q + l >= k && sid[q] == cu) sid[q] = -1;
+      }
+    }
+    float sd[NQ];
+    const int chunk = total - base < 64 * NQ ? total - base : 64 * NQ;
+    const int rounds = (chunk + 3) / 4;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      sd[q] = 0.f;
+      for (int rr = 0; rr < 16 && 16 * q + rr < rounds; ++rr) {
+        const int r = 16 * q + rr;
+        const int sl = 4 * r + g;
+        const int32_t id = __shfl(sid[q], sl & 63);
+        const bool stored = base + sl < k && !a.recompute_cur;
+        const bool live = sl < chunk && id >= 0 && id != i && !stored;
+        const float dot = group_dot(a.src, id, lds_f, lds_h, sub, live);
+        float v = 0.f;
+        if (live) v = pair_dist(a.src, sqi, id, dot);
+        const float got = __shfl(v, (l & 3) * 16);
+        if ((l >> 2) == rr) sd[q] = got;
+      }
+    }
+    uint64_t key[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int s = base + 64 * q + l;
+      float dv = sd[q];
+      if (s < k && !a.recompute_cur) dv = q == 0 ? cur_d[0] : cur_d[1];
+      const bool ok = s < total && sid[q] >= 0 && sid[q] != static_cast<int32_t>(i);
+      key[q] = ok ? dkey(dv, static_cast<uint32_t>(sid[q])) : KEY_NONE;
+    }
+    uint64_t o0 = KEY_NONE, o1 = KEY_NONE;
+    for (int produced = 0; produced < k; ++produced) {
+      uint64_t lm = lk0 < lk1 ? lk0 : lk1;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) lm = key[q] < lm ? key[q] : lm;
+      const uint64_t m = wave_min_u64(lm);
+      if (m == KEY_NONE) break;
+      lk0 = lk0 == m ? KEY_NONE : lk0;
+      lk1 = lk1 == m ? KEY_NONE : lk1;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) key[q] = key[q] == m ? KEY_NONE : key[q];
+      if (l == produced) o0 = m;
+      if (l + 64 == produced) o1 = m;
+    }
+    lk0 = o0;
+    lk1 = o1;
+  }
+  int nflag = 0;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int e = 64 * j + l;
+    const uint64_t lk = j == 0 ? lk0 : lk1;
+    const bool have = e < k && lk != KEY_NONE;
+    const int32_t nid = have ? static_cast<int32_t>(key_idx(lk)) : cur_id[j];
+    const float nd = have ? key_dist(lk) : cur_d[j];
+    const bool flag = e < k && nid != cur_id[j];
+    if (e < k) {
+      a.out_idx[i * k + e] = nid;
+      a.out_dist[i * k + e] = nd;
+      a.out_flags[i * k + e] = flag ? 1 : 0;
+    }
+    nflag += __popcll(ballot(flag));
+  }
+  if (l == 0 && nflag) atomicAdd(a.changed, static_cast<uint32_t>(nflag));
+}
+
 // Distances of (row, idx[row, c]) for all c with src; FP32 (or FP16) like
 // _gather_dists(_half). Optionally sorts each row by (distance, position)
 // (the init sort, nndescent.py:62-65).
@@ -570,6 +726,69 @@ __global__ __launch_bounds__(256) void k_nnd_gather(Src src, const int32_t* __re
   } else if (l < c) {
     out_dist[i * c + l] = dv;
     if (out_idx) out_idx[i * c + l] = my;
+  }
+}
+
+// k_nnd_gather for c from 65 to NND_K_WIDE: lane l holds positions l and
+// 64 + l. The sort places each entry at its rank among the row's
+// (distance, position) keys, which are distinct.
+__global__ __launch_bounds__(256) void k_nnd_gather_wide(Src src, const int32_t* __restrict__ idx, int64_t n,
+                                                         int c, int sort_rows, int32_t* __restrict__ out_idx,
+                                                         float* __restrict__ out_dist) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (i >= n) return;
+  const int64_t bpw = src.xh ? src.ld * 2 : src.ld * 4;
+  float* lds_f = reinterpret_cast<float*>(smem + w * bpw);
+  __half* lds_h = reinterpret_cast<__half*>(smem + w * bpw);
+  const int32_t my0 = l < c ? idx[i * c + l] : -1;
+  const int32_t my1 = 64 + l < c ? idx[i * c + 64 + l] : -1;
+  stage_row(src, i, lds_f, lds_h, l);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const float sqi = src.sq[i];
+  float dv0 = 0.f, dv1 = 0.f;
+  const int g = l >> 4, sub = l & 15;
+  for (int r = 0; 4 * r < c; ++r) {
+    const int s = 4 * r + g;   // rounds 0..15 hold positions 0..63, rounds 16..31 positions 64..127
+    const int32_t id = r < 16 ? __shfl(my0, s & 63) : __shfl(my1, s & 63);
+    const bool live = s < c && id >= 0;
+    const float dot = group_dot(src, id, lds_f, lds_h, sub, live);
+    float v = 0.f;
+    if (live) v = pair_dist(src, sqi, id, dot);
+    const float got = __shfl(v, (l & 3) * 16);
+    if ((l >> 2) == (r & 15)) {
+      if (r < 16) dv0 = got;
+      else dv1 = got;
+    }
+  }
+  if (sort_rows) {
+    const uint64_t k0 = l < c ? dkey(dv0, static_cast<uint32_t>(l)) : KEY_NONE;
+    const uint64_t k1 = 64 + l < c ? dkey(dv1, static_cast<uint32_t>(64 + l)) : KEY_NONE;
+    int r0 = 0, r1 = 0;
+    for (int f = 0; f < c; ++f) {
+      const uint64_t kf = f < 64 ? readlane_u64(k0, f) : readlane_u64(k1, f - 64);
+      r0 += kf < k0 ? 1 : 0;
+      r1 += kf < k1 ? 1 : 0;
+    }
+    if (l < c) {
+      out_idx[i * c + r0] = my0;
+      out_dist[i * c + r0] = key_dist(k0);
+    }
+    if (64 + l < c) {
+      out_idx[i * c + r1] = my1;
+      out_dist[i * c + r1] = key_dist(k1);
+    }
+  } else {
+    if (l < c) {
+      out_dist[i * c + l] = dv0;
+      if (out_idx) out_idx[i * c + l] = my0;
+    }
+    if (64 + l < c) {
+      out_dist[i * c + 64 + l] = dv1;
+      if (out_idx) out_idx[i * c + 64 + l] = my1;
+    }
   }
 }
 
@@ -707,7 +926,8 @@ __global__ __launch_bounds__(256) void k_rev_select_hub(const uint32_t* __restri
         lm = o < lm ? o : lm;
       }
       prev = lm;
-      if (l == 0) rev[j * k + t] = static_cast<int32_t>(lm / static_cast<uint32_t>(k));
+      // slots past the in-degree hold 0 (a hub may have fewer than k in-edges when k > 64)
+      if (l == 0) rev[j * k + t] = static_cast<uint32_t>(t) < c ? static_cast<int32_t>(lm / static_cast<uint32_t>(k)) : 0;
     }
   }
 }
@@ -794,7 +1014,8 @@ static int launch_update(const Plan& P, const Src& src, const int32_t* idx, cons
   const size_t lds = 4 * static_cast<size_t>(src.xh ? src.ld * 2 : src.ld * 4);
   const int total = a.k + mc_new * j_nn + (rev ? a.k : 0);
   const dim3 grid(static_cast<unsigned>(ceil_div(P.n, 4)));
-  if (total <= 64 * 4) hipLaunchKernelGGL(k_nnd_update<4>, grid, dim3(256), lds, s, a);
+  if (a.k > 64) hipLaunchKernelGGL(k_nnd_update_wide, grid, dim3(256), lds, s, a);
+  else if (total <= 64 * 4) hipLaunchKernelGGL(k_nnd_update<4>, grid, dim3(256), lds, s, a);
   else if (total <= 64 * 8) hipLaunchKernelGGL(k_nnd_update<8>, grid, dim3(256), lds, s, a);
   else if (total <= 64 * NND_NQ_MAX) hipLaunchKernelGGL(k_nnd_update<NND_NQ_MAX>, grid, dim3(256), lds, s, a);
   else hipLaunchKernelGGL(k_nnd_update_chunked, grid, dim3(256), lds, s, a);
@@ -805,8 +1026,12 @@ static int launch_update(const Plan& P, const Src& src, const int32_t* idx, cons
 static int launch_gather(const Plan& P, const Src& src, const int32_t* idx, int sort_rows, int32_t* oidx,
                          float* odist, hipStream_t s) {
   const size_t lds = 4 * static_cast<size_t>(src.xh ? src.ld * 2 : src.ld * 4);
-  hipLaunchKernelGGL(k_nnd_gather, dim3(static_cast<unsigned>(ceil_div(P.n, 4))), dim3(256), lds, s, src, idx,
-                     P.n, static_cast<int>(P.k), sort_rows, oidx, odist);
+  if (P.k > 64)
+    hipLaunchKernelGGL(k_nnd_gather_wide, dim3(static_cast<unsigned>(ceil_div(P.n, 4))), dim3(256), lds, s, src,
+                       idx, P.n, static_cast<int>(P.k), sort_rows, oidx, odist);
+  else
+    hipLaunchKernelGGL(k_nnd_gather, dim3(static_cast<unsigned>(ceil_div(P.n, 4))), dim3(256), lds, s, src, idx,
+                       P.n, static_cast<int>(P.k), sort_rows, oidx, odist);
   MV_LAUNCHED("k_nnd_gather");
   return MV_OK;
 }
@@ -861,7 +1086,7 @@ extern "C" {
 
 int mv_nnd_ws_bytes(int64_t n, int64_t d, int k, int64_t p1, int64_t p2, size_t* out_bytes) {
   MV_REQUIRE(out_bytes, "mv_nnd_ws_bytes: out_bytes is NULL");
-  MV_REQUIRE(n >= 2 && d >= 1 && k >= 1 && k <= 64 && k < n, "mv_nnd_ws_bytes: bad n=%lld d=%lld k=%d",
+  MV_REQUIRE(n >= 2 && d >= 1 && k >= 1 && k <= NND_K_WIDE && k < n, "mv_nnd_ws_bytes: bad n=%lld d=%lld k=%d",
              (long long)n, (long long)d, k);
   MV_REQUIRE(p1 >= 0 && p2 >= 0, "mv_nnd_ws_bytes: bad projection sizes");
   Plan P = make_plan(n, d, k, p1, p2);
@@ -871,7 +1096,8 @@ int mv_nnd_ws_bytes(int64_t n, int64_t d, int k, int64_t p1, int64_t p2, size_t*
 
 int mv_nnd_reverse(const int32_t* idx, int64_t n, int k, int32_t* rev, void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(idx && rev, "mv_nnd_reverse: null pointer");
-  MV_REQUIRE(n >= 2 && k >= 1 && k <= 64 && k < n, "mv_nnd_reverse: bad n=%lld k=%d", (long long)n, k);
+  MV_REQUIRE(n >= 2 && k >= 1 && k <= NND_K_WIDE && k < n, "mv_nnd_reverse: bad n=%lld k=%d", (long long)n, k);
+  MV_REQUIRE(n * k < (1ll << 31), "mv_nnd_reverse: n * k must stay below 2^31");
   Plan P = make_plan(n, 1, k, 0, 0);
   if (!ws || ws_bytes < P.bytes) {
     mv::set_error("mv_nnd_reverse: workspace of %zu bytes < required %zu", ws_bytes, P.bytes);
@@ -887,15 +1113,21 @@ int mv_nnd_build(const float* X, int64_t n, int64_t d, int k, int n_iters, int m
                  int32_t* out_idx, float* out_dist, int64_t* out_changed, int* out_iters, void* ws,
                  size_t ws_bytes, void* stream) {
   MV_REQUIRE(X && init_idx && out_idx && out_dist, "mv_nnd_build: null pointer");
-  MV_REQUIRE(n >= 2 && d >= 1 && k >= 1 && k <= 64 && k < n, "mv_nnd_build: bad n=%lld d=%lld k=%d",
+  MV_REQUIRE(n >= 2 && d >= 1 && k >= 1 && k <= NND_K_WIDE && k < n, "mv_nnd_build: bad n=%lld d=%lld k=%d",
              (long long)n, (long long)d, k);
-  MV_REQUIRE(n < (1ll << 31) / 64, "mv_nnd_build: n too large");
+  MV_REQUIRE(n * k < (1ll << 31), "mv_nnd_build: n * k must stay below 2^31");
   MV_REQUIRE(n_iters >= 0, "mv_nnd_build: n_iters < 0");
   MV_REQUIRE((p1 == 0) == (P1 == nullptr) && (p2 == 0) == (P2 == nullptr), "mv_nnd_build: projection mismatch");
   MV_REQUIRE(p1 < d && p2 < d, "mv_nnd_build: projections must reduce the dimension");
   const int mc = max_candidates > 0 ? max_candidates : k;
   if (align_up(d, 8) > static_cast<size_t>(MAX_LD)) {
     mv::set_error("mv_nnd_build: d=%lld exceeds the %d-element LDS row", (long long)d, MAX_LD);
+    return MV_EUNSUPPORTED;
+  }
+  if (k > 64 && align_up(d, 8) > static_cast<size_t>(MAX_LD - NND_K_WIDE)) {
+    // k_nnd_update_wide keeps 4 x NND_K_WIDE source ids in LDS beside the four staged rows
+    mv::set_error("mv_nnd_build: d=%lld exceeds the %d-element LDS row of k > 64", (long long)d,
+                  MAX_LD - NND_K_WIDE);
     return MV_EUNSUPPORTED;
   }
   Plan P = make_plan(n, d, k, p1, p2);
@@ -1008,7 +1240,7 @@ int mv_nnd_build(const float* X, int64_t n, int64_t d, int k, int n_iters, int m
 int mv_nnd_gather_dists(const float* X, int64_t n, int64_t d, const int32_t* idx, int k, int half, float* out_dist,
                         void* ws, size_t ws_bytes, void* stream) {
   MV_REQUIRE(X && idx && out_dist, "mv_nnd_gather_dists: null pointer");
-  MV_REQUIRE(n >= 2 && d >= 1 && k >= 1 && k <= 64 && k < n, "mv_nnd_gather_dists: bad sizes");
+  MV_REQUIRE(n >= 2 && d >= 1 && k >= 1 && k <= NND_K_WIDE && k < n, "mv_nnd_gather_dists: bad sizes");
   if (align_up(d, 8) > static_cast<size_t>(MAX_LD)) {
     mv::set_error("mv_nnd_gather_dists: d too large");
     return MV_EUNSUPPORTED;

@@ -23,9 +23,10 @@ from . import _capi, _device
 
 
 # Largest k of the device NNDescent: one wavefront per row holds the row's k
-# neighbours, one per lane; its candidates (the current k, mc * k neighbours of
-# neighbours and k reverse ones) are taken 1,024 at a time.
-NND_K_MAX = 64
+# neighbours, one per lane (two per lane from k = 65); its candidates (the
+# current k, mc * k neighbours of neighbours and k reverse ones) are taken
+# 1,024 at a time.
+NND_K_MAX = 128
 
 
 def nndescent_fits(k: int, max_candidates: int | None = None) -> bool:

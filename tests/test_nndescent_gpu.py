@@ -27,11 +27,12 @@ def _ws(n, d, k):
     return _device.workspace(nb.value)
 
 
-def test_reverse_candidates_bit_exact():
+@pytest.mark.parametrize("k", [15, 100])
+def test_reverse_candidates_bit_exact(k):
     from mlx_vis_amd import _capi
 
     rng = np.random.default_rng(0)
-    n, k = 5000, 15
+    n = 5000
     idx = rng.integers(0, n, size=(n, k)).astype(np.int32)
     idx[:50] = 7  # a hub with far more than k in-edges
     ws = _ws(n, 1, k)
@@ -42,10 +43,12 @@ def test_reverse_candidates_bit_exact():
 
 
 @pytest.mark.parametrize("half", [0, 1])
-def test_gather_dists(half):
+@pytest.mark.parametrize("k", [15, 100])
+def test_gather_dists(half, k):
+    """k = 100 runs the two-entries-per-lane gather."""
     from mlx_vis_amd import _capi
 
-    n, d, k = 3000, 100, 15
+    n, d = 3000, 100
     X = _data(n, d, 1)
     rng = np.random.default_rng(2)
     idx = rng.integers(0, n, size=(n, k)).astype(np.int32)
@@ -69,11 +72,14 @@ def test_gather_dists(half):
 
 @pytest.mark.parametrize("n,d,seed,k", [(4000, 64, 3, 15), (3000, 784, 4, 15), (6000, 128, 5, 15),
                                         (3000, 64, 7, 20), (2500, 48, 8, 30), (2500, 40, 11, 32),
-                                        (1500, 24, 12, 48), (900, 16, 13, 64)])
+                                        (1500, 24, 12, 48), (900, 16, 13, 64), (1500, 24, 14, 91),
+                                        (1300, 20, 15, 100), (1000, 16, 16, 128)])
 def test_build_recall_not_below_reference(n, d, seed, k):
     """k = 15 uses 256 candidate slots per row, k = 20 and 30 (max_candidates
     = k: k + k^2 + k slots) the 512- and 1024-slot variants, k = 32, 48 and 64
-    (1,088 to 4,224 slots) the chunked merge."""
+    (1,088 to 4,224 slots) the chunked merge, k = 91 (the reference's
+    calibration case, nndescent.py:85-88), 100 and 128 the kernels that hold
+    two of a row's neighbours per lane (init sort included)."""
     from mlx_vis_amd import NNDescent
 
     print(f"case n={n} d={d} k={k}", flush=True)
@@ -113,12 +119,32 @@ def test_knn_method_nndescent_via_compute_knn():
     np.testing.assert_allclose(d1 ** 2, d2, rtol=1e-6)
 
 
-def test_build_rejects_k_beyond_64():
+def test_build_rejects_k_beyond_128():
     from mlx_vis_amd import NNDescent
 
     X = _data(500, 8, 9)
     with pytest.raises(ValueError):
-        NNDescent(k=65, random_state=0).build(X)
+        NNDescent(k=129, random_state=0).build(X)
+
+
+def test_auto_method_runs_nndescent_at_k_100():
+    """compute_knn's auto rule (_knn.py:32-33) picks NNDescent above 20,000
+    rows; at k = 100 its recall against the exact self kNN stays high."""
+    from mlx_vis_amd import compute_knn
+    from mlx_vis_amd._knn import knn_brute_device
+
+    n, d, k = 30000, 32, 100
+    X = _data(n, d, 17, k_centers=20)
+    print(f"case n={n} d={d} k={k}", flush=True)
+    gi, gd = compute_knn(X, k, method="auto")
+    ei, _ = knn_brute_device(torch.from_numpy(X).cuda(), k)
+    ei = ei.cpu().numpy()
+    rec = om.recall(gi, ei)
+    print(f"recall {rec:.4f}")
+    assert gi.shape == (n, k) and (gi != np.arange(n)[:, None]).all()
+    assert all(len(set(r)) == k for r in gi[::97])
+    assert np.isfinite(gd).all()
+    assert rec > 0.9
 
 
 def _fixture(name):
