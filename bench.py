@@ -66,10 +66,12 @@ def parse():
                     help="fresh points mapped by the learned form (default 1,000,000)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-form line")
     ap.add_argument("--no-scale", action="store_true", help="skip the 1M x 128 NNDescent + UMAP line")
-    ap.add_argument("--only", choices=["knn", "copy", "learned", "scale", "layout", "fit", "nndescent"], default=None,
+    ap.add_argument("--only", choices=["knn", "copy", "learned", "scale", "layout", "fit", "nndescent", "cpu-full"],
+                    default=None,
                     help="profiling helpers: 'knn' runs only the brute kNN (--dtype), 'copy' a 1 GiB device copy, "
                          "'fit' --steps fit_transform calls of the workload, 'nndescent' --steps NNDescent builds "
-                         "(k 15, 12 iterations, max_candidates 30) of the same data")
+                         "(k 15, 12 iterations, max_candidates 30) of the same data, 'cpu-full' one unsampled "
+                         "run of the oracle's fit_transform on the host cores (no GPU; about five minutes)")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8", "bf16x3"])
     ap.add_argument("--rerank", type=int, default=None,
                     help="candidates per row = rerank * k for --only knn (default: the library's per-dtype default)")
@@ -407,6 +409,67 @@ def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs
     }
 
 
+def cpu_full_run(X, n_epochs, block=1000):
+    """One full, unsampled run of the 70,000 x 784 fit_transform by the oracle
+    restatement on the host cores (same thread settings and parts as
+    cpu_baseline): kNN of every row in blocks of `block` rows, fuzzy graph,
+    a/b, 100 spectral power iterations and every layout epoch. It takes about
+    five minutes, so it is a separate mode, not part of the default line."""
+    import numpy as np
+    import torch
+
+    from oracle import knn as oknn
+    from oracle import umap as ou
+
+    cores = min(16, len(os.sched_getaffinity(0)))
+    from threadpoolctl import threadpool_limits
+
+    torch.set_num_threads(1)
+    n = X.shape[0]
+    rng = np.random.default_rng(0)
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        idx = np.empty((n, 15), np.int32)
+        dist = np.empty((n, 15), np.float32)
+        for b0 in range(0, n, block):
+            rows = np.arange(b0, min(n, b0 + block))
+            idx[rows], dist[rows] = oknn.reference_knn_rows(X, rows, 15)
+            if (b0 // block) % 7 == 6:
+                log(f"cpu-full: kNN {rows[-1] + 1} of {n} rows, {time.perf_counter() - t0:.1f} s")
+        t_knn = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r, c, v = ou.fuzzy_simplicial_set(idx, dist, n_epochs)
+        ou.find_ab_params(1.0, 0.1)
+        t_graph = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ou.spectral_power(r, c, v.astype(np.float64), n, rng.standard_normal((n, 3)), iters=100)
+        t_spec = time.perf_counter() - t0
+        log(f"cpu-full: graph {t_graph:.1f} s, spectral {t_spec:.1f} s")
+        sets = ou.schedule(v, n_epochs)
+        Y = torch.from_numpy(rng.uniform(-10, 10, size=(n, 2)).astype(np.float32))
+        rt, ct = torch.from_numpy(r.astype(np.int64)), torch.from_numpy(c.astype(np.int64))
+        t0 = time.perf_counter()
+        for e in range(n_epochs):
+            if not sets[e].size:
+                continue
+            act = torch.from_numpy(sets[e].astype(np.int64))
+            A = act.numel()
+            ef, et = rt[act], ct[act]
+            nf = ef[torch.arange(5 * A) % A]
+            nt = torch.from_numpy(rng.integers(0, n, size=5 * A))
+            Y = ou.sgd_step_index_add(Y, ef, et, nf, nt, float(1.0 - e / n_epochs), 1.577, 0.895)
+            if e % 100 == 99:
+                log(f"cpu-full: layout epoch {e + 1}, {time.perf_counter() - t0:.1f} s")
+        t_layout = time.perf_counter() - t0
+    return {"total_s": round(t_knn + t_graph + t_spec + t_layout, 2),
+            "parts_s": {"knn": round(t_knn, 2), "graph": round(t_graph, 2), "spectral": round(t_spec, 2),
+                        "layout": round(t_layout, 2)},
+            "cores": int(cores), "threads": {"blas": int(cores), "torch_layout": 1},
+            "cpu_model": cpu_model(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "workload": f"{n} x {X.shape[1]} mixture, oracle restatement (NumPy/torch-CPU, not MLX), one run, "
+                        f"every row and epoch"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -433,6 +496,9 @@ def main():
 
     log("generating the 10-Gaussian 784-D mixture")
     X = gaussian_mixture_784(args.n, seed=1)
+    if args.only == "cpu-full":
+        print(json.dumps({"cpu_full_run": cpu_full_run(X, args.epochs)}), flush=True)
+        return
     Xd = torch.from_numpy(X).cuda()
     n, d = X.shape
     torch.cuda.synchronize()
