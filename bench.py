@@ -314,7 +314,7 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs=8, spec_iters=5, c0_runs=3):
+def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs=16, spec_iters=5, c0_runs=3):
     """The oracle restatement (NumPy with BLAS threads, torch-CPU index_add_
     scatters), not MLX: a full run of the small plain-method workload
     (10,000 x 128, 200 epochs; one warm-up on 2,000 rows, then the median of
@@ -366,9 +366,14 @@ def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs
         ou.spectral_power(r, c, v.astype(np.float64), n, rng.standard_normal((n, 3)), iters=spec_iters)
         t_spec = (time.perf_counter() - t0) * 100 / spec_iters
 
+        # layout: budget_epochs epochs spread evenly over the schedule's
+        # non-empty epochs, scaled by their count (late epochs hold few edges
+        # but keep their fixed per-epoch cost, so scaling the first epochs by
+        # edge samples would undercount)
         sets = ou.schedule(v, n_epochs)
-        total = sum(s_.size for s_ in sets)
-        ep = [e for e in range(n_epochs) if sets[e].size][:budget_epochs]
+        nonempty = [e for e in range(n_epochs) if sets[e].size]
+        pick = np.unique(np.linspace(0, len(nonempty) - 1, budget_epochs).round().astype(int))
+        ep = [nonempty[j] for j in pick]
         Y = torch.from_numpy(rng.uniform(-10, 10, size=(n, 2)).astype(np.float32))
         rt, ct = torch.from_numpy(r.astype(np.int64)), torch.from_numpy(c.astype(np.int64))
         done = 0
@@ -381,7 +386,7 @@ def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs
             nt = torch.from_numpy(rng.integers(0, n, size=5 * A))
             Y = ou.sgd_step_index_add(Y, ef, et, nf, nt, float(1.0 - e / n_epochs), 1.577, 0.895)
             done += A
-        t_layout = (time.perf_counter() - t0) * total / max(done, 1)
+        t_layout = (time.perf_counter() - t0) * len(nonempty) / len(ep)
     finally:
         if ctx is not None:
             ctx.__exit__(None, None, None)
@@ -395,8 +400,8 @@ def cpu_baseline(X, knn_idx, knn_dist, n_epochs, budget_rows=1000, budget_epochs
         "sample": (f"NumPy restatement of the reference (oracle), not MLX, extrapolated: kNN (FP32 expansion + "
                    f"full-row argsort) on {budget_rows} of {n} query rows (x{n / budget_rows:.0f}); fuzzy graph + "
                    f"a/b in full; spectral init {spec_iters} of 100 iterations (x{100 / spec_iters:.0f}); layout "
-                   f"epochs {ep[0]}-{ep[-1]} ({len(ep)} epochs, torch-CPU index_add_, scaled by edge samples "
-                   f"{total}/{done})"),
+                   f"{len(ep)} epochs spread over {ep[0]}-{ep[-1]} (torch-CPU index_add_, {done} edge samples, "
+                   f"scaled by epochs {len(nonempty)}/{len(ep)})"),
         "parts_s": {"knn": round(t_knn, 2), "graph": round(t_graph, 2), "spectral": round(t_spec, 2),
                     "layout": round(t_layout, 2)},
         "cpu_model": cpu_model(),
