@@ -336,6 +336,26 @@ def test_fit_transform_with_many_neighbours(n_neighbors):
     assert om.trustworthiness(X, Y, 15) > 0.85
 
 
+def test_fit_transform_nndescent_with_100_neighbours():
+    """Above 20,000 rows the auto rule (_knn.py:32-33) takes NNDescent; at
+    n_neighbors = 100 its kernels hold two of a row's neighbours per lane."""
+    from mlx_vis_amd import UMAP, trustworthiness
+
+    rng = np.random.default_rng(21)
+    n, d = 24000, 24
+    centers = rng.standard_normal((8, d)) * 3
+    X = (centers[rng.integers(0, 8, n)] + rng.standard_normal((n, d))).astype(np.float32)
+    print(f"case UMAP n={n} d={d} n_neighbors=100 auto kNN", flush=True)
+    um = UMAP(n_neighbors=100, n_epochs=100, random_state=0)
+    Y = um.fit_transform(X)
+    assert Y.shape == (n, 2) and np.isfinite(Y).all()
+    assert um.knn_[0].shape == (n, 100)
+    rows = np.random.default_rng(1).choice(n, 2000, replace=False)
+    t = trustworthiness(X, Y, 15, rows=rows)
+    print(f"trustworthiness {t:.4f}")
+    assert t > 0.85
+
+
 def test_random_normal_draws():
     """mv_random_normal (spectral start vectors): standard normal moments,
     reproducible by seed, different across seeds."""
