@@ -73,6 +73,7 @@ def parse():
                          "(k 15, 12 iterations, max_candidates 30) of the same data, 'cpu-full' one unsampled "
                          "run of the oracle's fit_transform on the host cores (no GPU; about five minutes)")
     ap.add_argument("--dtype", default="fp32", choices=["fp32", "bf16", "fp8", "bf16x3"])
+    ap.add_argument("--nnd-k", type=int, default=15, help="k of the --only nndescent builds (default 15)")
     ap.add_argument("--rerank", type=int, default=None,
                     help="candidates per row = rerank * k for --only knn (default: the library's per-dtype default)")
     ap.add_argument("--no-extras", action="store_true", help="skip the precision-form and NNDescent lines")
@@ -566,10 +567,22 @@ def main():
             if args.only == "fit":
                 step()
             else:
-                NNDescent(k=15, n_iters=12, max_candidates=30, random_state=0).build_device(Xd)
+                nn = NNDescent(k=args.nnd_k, n_iters=12, max_candidates=30, random_state=0)
+                gi, _ = nn.build_device(Xd)
             torch.cuda.synchronize()
             times.append(round((time.perf_counter() - t) * 1e3, 2))
-        print(json.dumps({args.only + "_ms": times}), flush=True)
+        out = {args.only + "_ms": times}
+        if args.only == "nndescent":
+            # recall of the last build against the exact self kNN of the same k
+            from mlx_vis_amd import knn_brute_device
+
+            ei, _ = knn_brute_device(Xd, args.nnd_k)
+            hits = 0
+            for r0 in range(0, n, 8192):
+                g_, e_ = gi[r0:r0 + 8192], ei[r0:r0 + 8192]
+                hits += int((g_[:, :, None] == e_[:, None, :]).any(-1).sum().item())
+            out.update({"k": args.nnd_k, "iterations": nn.iterations_, "recall_vs_exact": round(hits / (n * args.nnd_k), 4)})
+        print(json.dumps(out), flush=True)
         return
 
     for i in range(args.warmup):
